@@ -1,0 +1,215 @@
+// fvwb_device.h — device-side building blocks shared by the FV kernels.
+//
+// Pointwise physics of the Euler-with-gravity residual in deviation form
+// (benchmark_2d.f90:145-171 primitive/conservative, 283-295 speed, 299-325 flux,
+// 327-350 source, 353-367 Rusanov) and the cross-lane moves used by the
+// sliding-window stage kernel.  Two flux forms:
+//   * PLAIN (FP64 drop-in): full states U_eq(face)+dU, full fluxes, minus the
+//     equilibrium flux — the operation structure of compute_update_exact;
+//   * DEV (FP32 bulk): the flux deviation written algebraically for an
+//     equilibrium at rest, so no O(1) quantity is ever subtracted from another.
+// Both give exactly zero at dU == 0 under FMA contraction (see comments).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fvwb {
+
+// ---- cross-lane moves (wave64) -------------------------------------------------
+// lane l receives the value of lane l-1 (lane 0 receives its own value).
+__device__ __forceinline__ float lane_from_left(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+      __builtin_bit_cast(int, x), __builtin_bit_cast(int, x), 0x138 /*wave_shr:1*/, 0xF, 0xF, false));
+}
+// lane l receives the value of lane l+1 (lane 63 receives its own value).
+__device__ __forceinline__ float lane_from_right(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+      __builtin_bit_cast(int, x), __builtin_bit_cast(int, x), 0x130 /*wave_shl:1*/, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double lane_from_left(double x) {
+  int2 v = __builtin_bit_cast(int2, x);
+  v.x = __builtin_amdgcn_update_dpp(v.x, v.x, 0x138, 0xF, 0xF, false);
+  v.y = __builtin_amdgcn_update_dpp(v.y, v.y, 0x138, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, v);
+}
+__device__ __forceinline__ double lane_from_right(double x) {
+  int2 v = __builtin_bit_cast(int2, x);
+  v.x = __builtin_amdgcn_update_dpp(v.x, v.x, 0x130, 0xF, 0xF, false);
+  v.y = __builtin_amdgcn_update_dpp(v.y, v.y, 0x130, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, v);
+}
+
+// ---- small math helpers --------------------------------------------------------
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double tabs(double x) { return __builtin_fabs(x); }
+// reciprocal: IEEE division in FP64, hardware v_rcp_f32 (1 ulp) in FP32
+__device__ __forceinline__ double trcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float trcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// minmod of limiters.f90:1-21 with the third argument equal to the second:
+// s*min(|a|,|b|) if a and b carry the same sign (sign(1,0) = +1), else 0.
+template <typename T>
+__device__ __forceinline__ T minmod2(T a, T b) {
+  bool sa = __builtin_signbit(a), sb = __builtin_signbit(b);
+  T m = tmin(tabs(a), tabs(b));
+  return sa == sb ? (sa ? -m : m) : T(0);
+}
+
+template <typename T>
+struct Vec4 { T r, mx, my, e; };
+
+template <typename T>
+__device__ __forceinline__ Vec4<T> operator+(Vec4<T> a, Vec4<T> b) { return {a.r + b.r, a.mx + b.mx, a.my + b.my, a.e + b.e}; }
+template <typename T>
+__device__ __forceinline__ Vec4<T> operator-(Vec4<T> a, Vec4<T> b) { return {a.r - b.r, a.mx - b.mx, a.my - b.my, a.e - b.e}; }
+template <typename T>
+__device__ __forceinline__ Vec4<T> scale(T s, Vec4<T> a) { return {s * a.r, s * a.mx, s * a.my, s * a.e}; }
+
+template <typename T>
+__device__ __forceinline__ Vec4<T> minmod2(Vec4<T> a, Vec4<T> b) {
+  return {minmod2(a.r, b.r), minmod2(a.mx, b.mx), minmod2(a.my, b.my), minmod2(a.e, b.e)};
+}
+template <typename T>
+__device__ __forceinline__ Vec4<T> from_left(Vec4<T> a) {
+  return {lane_from_left(a.r), lane_from_left(a.mx), lane_from_left(a.my), lane_from_left(a.e)};
+}
+template <typename T>
+__device__ __forceinline__ Vec4<T> from_right(Vec4<T> a) {
+  return {lane_from_right(a.r), lane_from_right(a.mx), lane_from_right(a.my), lane_from_right(a.e)};
+}
+
+// ---- physical constants handed to the flux routines ------------------------------
+template <typename T>
+struct Phys {
+  T gamma, gm1, half;   // gamma, gamma-1, 0.5
+  T g;                  // |grad phi| per axis
+};
+
+// Face equilibrium at rest: density, pressure and energy p/(gamma-1) (the
+// reference's compute_conservative of a resting state, benchmark_2d.f90:169).
+template <typename T>
+struct FaceEq { T rho, p, e; };
+
+enum { FLUX_RUSANOV = 0, FLUX_HLL = 1 };
+enum { FORM_PLAIN = 0, FORM_DEV = 1 };
+
+// Per-side quantities.
+template <typename T>
+struct Side {
+  Vec4<T> f;      // flux (PLAIN) or flux deviation (DEV) along the face normal
+  T vn;           // signed normal velocity
+  T c;            // sound speed with the reference's floors
+  T vmag;         // |v| (Rusanov bound of the reference)
+};
+
+// PLAIN form: U = U_eq + dU in full, primitive and flux exactly as
+// compute_primitive / compute_flux (benchmark_2d.f90:153-156, 315-323).
+template <typename T, int DIR>
+__device__ __forceinline__ Side<T> side_plain(const Phys<T>& P, const FaceEq<T>& q, Vec4<T> d, Vec4<T>& U) {
+  U = {q.rho + d.r, d.mx, d.my, q.e + d.e};
+  T w0 = U.r;
+  T w1 = U.mx / w0;
+  T w2 = U.my / w0;
+  T w3 = P.gm1 * (U.e - P.half * w0 * (w1 * w1 + w2 * w2));
+  Side<T> s;
+  if (DIR == 0) {
+    s.f = {w1 * U.r, w1 * U.mx + w3, w0 * w1 * w2, w1 * U.e + w1 * w3};
+    s.vn = w1;
+  } else {
+    s.f = {U.r * w2, U.mx * w2, U.my * w2 + w3, w2 * U.e + w2 * w3};
+    s.vn = w2;
+  }
+  s.c = tsqrt(P.gamma * tmax(w3, T(1e-10)) / tmax(w0, T(1e-10)));
+  s.vmag = tsqrt(w1 * w1 + w2 * w2);
+  return s;
+}
+
+// DEV form: with m_eq = 0, F(U_eq+dU) - F(U_eq) along x is
+//   (dm_x, m_x u + dp, m_y u, (E+p) u),  dp = (gamma-1)(dE - |m|^2/(2 rho)),
+// and symmetrically along y; no cancellation of O(1) terms occurs.
+template <typename T, int DIR>
+__device__ __forceinline__ Side<T> side_dev(const Phys<T>& P, const FaceEq<T>& q, Vec4<T> d) {
+  T rho = q.rho + d.r;
+  T inv = trcp(rho);
+  T u = d.mx * inv, v = d.my * inv;
+  T dp = P.gm1 * (d.e - P.half * (d.mx * u + d.my * v));
+  T p = q.p + dp;
+  T ep = (q.e + d.e) + p;
+  Side<T> s;
+  if (DIR == 0) {
+    s.f = {d.mx, d.mx * u + dp, d.my * u, ep * u};
+    s.vn = u;
+  } else {
+    s.f = {d.my, d.mx * v, d.my * v + dp, ep * v};
+    s.vn = v;
+  }
+  s.c = tsqrt(P.gamma * tmax(p, T(1e-10)) * trcp(tmax(rho, T(1e-10))));
+  s.vmag = tsqrt(u * u + v * v);
+  return s;
+}
+
+// Numerical flux minus the equilibrium flux at one face, given the deviations
+// on the left (dl) and right (dr).  At dl == dr the Rusanov and HLL forms used
+// here return the one-sided flux exactly (0.5*(f+f) = f; f + w*0 - q*0 = f), so
+// the deviation is exactly zero at equilibrium.
+template <typename T, int FLUX, int DIR, int FORM>
+__device__ __forceinline__ Vec4<T> face_flux(const Phys<T>& P, const FaceEq<T>& q, Vec4<T> dl, Vec4<T> dr) {
+  Side<T> L, R;
+  Vec4<T> fe = {T(0), T(0), T(0), T(0)};
+  Vec4<T> jl = dl, jr = dr;  // states entering the jump terms
+  if (FORM == FORM_PLAIN) {
+    L = side_plain<T, DIR>(P, q, dl, jl);
+    R = side_plain<T, DIR>(P, q, dr, jr);
+    // equilibrium flux through compute_flux of the resting face state: its
+    // pressure is (gamma-1)*(E_eq - 0), equal to the one-sided flux at dU == 0
+    // bit for bit (also under FMA contraction, since every dropped term is 0).
+    T pe = P.gm1 * q.e;
+    if (DIR == 0) fe.mx = pe; else fe.my = pe;
+  } else {
+    L = side_dev<T, DIR>(P, q, dl);
+    R = side_dev<T, DIR>(P, q, dr);
+  }
+  Vec4<T> fh;
+  if (FLUX == FLUX_RUSANOV) {
+    // compute_llflux: 0.5*(fR+fL) + 0.5*cmax*(uL-uR), cmax from |v|+c
+    T cmax = tmax(L.vmag + L.c, R.vmag + R.c);
+    T hc = P.half * cmax;
+    fh = {P.half * (R.f.r + L.f.r) + hc * (jl.r - jr.r),
+          P.half * (R.f.mx + L.f.mx) + hc * (jl.mx - jr.mx),
+          P.half * (R.f.my + L.f.my) + hc * (jl.my - jr.my),
+          P.half * (R.f.e + L.f.e) + hc * (jl.e - jr.e)};
+  } else {
+    // HLL, Davis bounds on the signed normal velocity, written as
+    // F_L + w (F_R - F_L) - q (U_R - U_L), w = a-/(a+ + a-), q = a+ a-/(a+ + a-)
+    T sl = tmin(L.vn - L.c, R.vn - R.c);
+    T sr = tmax(L.vn + L.c, R.vn + R.c);
+    T ap = tmax(T(0), sr), am = tmax(T(0), -sl);
+    T inv = trcp(ap + am);
+    T w = am * inv, qq = ap * am * inv;
+    fh = {L.f.r + w * (R.f.r - L.f.r) - qq * (jr.r - jl.r),
+          L.f.mx + w * (R.f.mx - L.f.mx) - qq * (jr.mx - jl.mx),
+          L.f.my + w * (R.f.my - L.f.my) - qq * (jr.my - jl.my),
+          L.f.e + w * (R.f.e - L.f.e) - qq * (jr.e - jl.e)};
+  }
+  if (FORM == FORM_PLAIN) fh = fh - fe;
+  return fh;
+}
+
+// Wave speed |v| + c of the cell state U_eq(centre) + dU (compute_speed,
+// benchmark_2d.f90:283-295); ok is false for rho<=0, p<=0 or non-finite.
+template <typename T>
+__device__ __forceinline__ T cell_speed(const Phys<T>& P, T rho_eq, T e_eq, Vec4<T> d, bool& ok) {
+  T rho = rho_eq + d.r;
+  T inv = T(1) / rho;
+  T u = d.mx * inv, v = d.my * inv;
+  T p = P.gm1 * ((e_eq + d.e) - P.half * rho * (u * u + v * v));
+  ok = (rho > T(0)) && (p > T(0)) && (p < T(3.0e38)) && (rho < T(3.0e38));
+  return tsqrt(u * u + v * v) + tsqrt(P.gamma * tmax(p, T(1e-10)) / tmax(rho, T(1e-10)));
+}
+
+}  // namespace fvwb

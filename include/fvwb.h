@@ -1,0 +1,131 @@
+/*
+ * fvwb.h — C ABI of the MI355X well-balanced Euler-with-gravity solver.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns an
+ * int status (FVWB_OK == 0, negative on error) and never throws.  Host arrays use
+ * the reference's Fortran layout: a conservative state `u(1:nvar,1:nx,1:ny)` is a
+ * contiguous double array with the variable index fastest, then x, then y
+ * (element (v,i,j) at offset v + nvar*(i + nx*j), zero-based).
+ *
+ * Physics: 2D Euler, gamma-law gas, linear potential phi = g*(x+y) on
+ * [0,boxlen_x]x[0,boxlen_y].  The stored hydrostatic equilibrium is the
+ * isothermal atmosphere rho = rho0*exp(-k*(x+y)), p = p0*exp(-k*(x+y)),
+ * k = rho0*g/p0, at rest (benchmark_2d.f90:174-218, get_equilibrium_solution).
+ * The device state is the deviation dU = U - U_eq in SoA planes; the residual is
+ * the exact deviation form of benchmark_2d.f90:465-618 (compute_update_exact).
+ *
+ * Threading: a handle owns device memory and one HIP stream; calls on one handle
+ * must not overlap.  Host-array entry points block until the result is on the host.
+ */
+#ifndef FVWB_H
+#define FVWB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define FVWB_OK               0
+#define FVWB_ERR_ARG         (-1)  /* bad argument / unsupported combination   */
+#define FVWB_ERR_HIP         (-2)  /* a HIP runtime call failed                */
+#define FVWB_ERR_OOM         (-3)  /* device allocation failed                 */
+#define FVWB_ERR_NONPHYSICAL (-4)  /* rho<=0, p<=0 or a non-finite value seen  */
+#define FVWB_ERR_STATE       (-5)  /* call order (e.g. advance before upload)  */
+
+/* ---- scheme selectors ---------------------------------------------------- */
+/* boundary: FROZEN_RING is the reference's (benchmark_2d.f90:611-614: dudt=0 on
+ * the outer ring of cells); REFLECT mirrors the deviation with the normal
+ * momentum negated (solid walls). */
+#define FVWB_BC_FROZEN_RING   0
+#define FVWB_BC_REFLECT       1
+/* reconstruction of the deviation at faces: FIRST_ORDER is the reference's
+ * piecewise-constant state (benchmark_2d.f90:533-537); MUSCL_MINMOD uses the
+ * minmod slope of limiters.f90:1-21 on conservative deviations. */
+#define FVWB_RECON_FIRST_ORDER   0
+#define FVWB_RECON_MUSCL_MINMOD  1
+/* numerical flux: RUSANOV is compute_llflux (benchmark_2d.f90:353-367, wave
+ * speed |v|+c with |v| the velocity magnitude); HLL is the two-wave HLL flux with
+ * Davis speed bounds on the signed normal velocity. */
+#define FVWB_FLUX_RUSANOV     0
+#define FVWB_FLUX_HLL         1
+/* time integrator: RK2_HEUN is the reference's evolve (benchmark_2d.f90:246-250);
+ * SSPRK3 is the Shu-Osher scheme of dg_with_source.f90:194-204. */
+#define FVWB_RK2_HEUN         0
+#define FVWB_SSPRK3           1
+/* storage / arithmetic precision of the deviation state */
+#define FVWB_FP64             0   /* plain FP64: full-flux differences, drop-in   */
+#define FVWB_FP32             1   /* FP32 bulk: cancellation-free deviation flux  */
+
+typedef struct fvwb_params {
+  int32_t nvar;        /* must be 4                                           */
+  int32_t nx, ny;      /* cells; nx, ny >= 4                                  */
+  int32_t bc;          /* FVWB_BC_*                                           */
+  int32_t recon;       /* FVWB_RECON_*                                        */
+  int32_t flux;        /* FVWB_FLUX_*                                         */
+  int32_t integrator;  /* FVWB_RK2_HEUN / FVWB_SSPRK3                         */
+  int32_t precision;   /* FVWB_FP64 / FVWB_FP32                               */
+  double  gamma;       /* ratio of specific heats (reference: 1.4 as an FP32
+                          literal, i.e. 1.39999997615814209)                  */
+  double  boxlen_x, boxlen_y;
+  double  cfl;         /* dt = 0.5*dx/cmax*cfl (benchmark_2d.f90:242)         */
+  double  rho0, p0, g; /* equilibrium constants; nequilibrium=1 is (1,1,1),
+                          nequilibrium=2/3 is (1.21 as FP32 literal, 1, 1)     */
+} fvwb_params;
+
+typedef struct fvwb_ctx* fvwb_handle;
+
+/* Fill `p` with the reference's benchmark_2d set-up (parameters_2d.f90:1-24):
+ * nx x ny cells, gamma/rho0 rounded through FP32 as the reference's literals
+ * are, first-order Rusanov, frozen ring, Heun RK2, FP64, cfl 0.5. */
+int fvwb_params_reference_2d(fvwb_params* p, int32_t nx, int32_t ny, int32_t nequilibrium);
+
+/* Create / destroy a single-instance context (device buffers + one stream). */
+int fvwb_create(fvwb_handle* h, const fvwb_params* p);
+int fvwb_destroy(fvwb_handle h);
+/* Message of the last failure on `h` (or of the last failed create if h==NULL). */
+const char* fvwb_last_error(fvwb_handle h);
+
+/* ---- per-stage drop-in (host arrays, blocking) ---------------------------
+ * Replaces `subroutine compute_update_exact(u, w_eq, dudt)`
+ * (benchmark_2d.f90:465-618).  u: conservative state AoS(4,nx,ny); w_eq:
+ * PRIMITIVE equilibrium at cell centres AoS(4,nx,ny), at rest; dudt (out):
+ * AoS(4,nx,ny).  Uses the handle's recon/flux/bc (the reference set-up is
+ * FIRST_ORDER/RUSANOV/FROZEN_RING).  Always computed in FP64. */
+int fvwb_compute_update_exact(fvwb_handle h, const double* u, const double* w_eq, double* dudt);
+
+/* Replaces `subroutine compute_max_speed(u, cmax)` (benchmark_2d.f90:264-279):
+ * cmax = max over cells of sqrt(vx^2+vy^2) + sqrt(gamma*max(p,1e-10)/max(rho,1e-10)). */
+int fvwb_compute_max_speed(fvwb_handle h, const double* u, double* cmax);
+
+/* ---- device-resident stepping (replaces the loop of `evolve`,
+ *      benchmark_2d.f90:221-260) --------------------------------------------
+ * upload: dU = u - conservative(w_eq) is formed on the device in FP64 and
+ * stored in the handle's precision; t is reset to 0. */
+int fvwb_upload(fvwb_handle h, const double* u, const double* w_eq);
+/* Initialise on the device (no host arrays): dU = 0 except a pressure bump
+ * dp = amp*exp(-width*((x-x0)^2+(y-y0)^2)), i.e. dE = dp/(gamma-1).  w_eq is
+ * the closed-form equilibrium at cell centres. */
+int fvwb_init_gaussian(fvwb_handle h, double amp, double x0, double y0, double width);
+/* Advance at most `nsteps` steps while t < t_end (pass t_end = +inf for a fixed
+ * count).  dt is computed on the device each step from the fused CFL reduction,
+ * so the host does not synchronise between steps.  t_out / steps_done may be
+ * NULL.  Returns FVWB_ERR_NONPHYSICAL if any step met rho<=0, p<=0 or NaN. */
+int fvwb_advance(fvwb_handle h, int32_t nsteps, double t_end, double* t_out, int32_t* steps_done);
+/* Same, but enqueue only (no host synchronisation, no status check); used for
+ * timing with events on the handle's stream. */
+int fvwb_advance_async(fvwb_handle h, int32_t nsteps, double t_end);
+/* Copy back u = conservative(w_eq) + dU as AoS(4,nx,ny) FP64. */
+int fvwb_download(fvwb_handle h, double* u);
+/* Copy back the stored deviation dU as AoS(4,nx,ny) FP64 (exact widening). */
+int fvwb_download_deviation(fvwb_handle h, double* du);
+/* Simulated time and last dt after the enqueued steps (synchronises). */
+int fvwb_get_time(fvwb_handle h, double* t, double* dt_last, int32_t* steps_done);
+/* The handle's stream (a hipStream_t), for event timing by the caller. */
+int fvwb_get_stream(fvwb_handle h, void** stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVWB_H */

@@ -1,0 +1,76 @@
+/*
+ * fvwb_oracle.h — CPU FP64 oracle for the well-balanced Euler-with-gravity path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product (libfvwb.so) never links it.
+ *
+ * Arrays use the reference's Fortran layout: u(1:nvar,1:nx,1:ny) with the
+ * variable index fastest.  Two families:
+ *   orc_ref2d_*  — statement-by-statement restatements of benchmark_2d.f90
+ *                  (same operation order, no FMA contraction), pinned against the
+ *                  flang build of the reference in oracle/_ref;
+ *   orc_fv2d_*   — the configured scheme (MUSCL-minmod / HLL / reflecting walls /
+ *                  SSP-RK3) in the same plain FP64 form, pinned by invariants
+ *                  and closed forms in tests/.
+ */
+#ifndef FVWB_ORACLE_H
+#define FVWB_ORACLE_H
+
+#include "../include/fvwb.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- pointwise physics (benchmark_2d.f90:145-171, 283-367) -------------- */
+void   orc_ref_primitive(double gamma, const double u[4], double w[4]);
+void   orc_ref_conservative(double gamma, const double w[4], double u[4]);
+double orc_ref_speed(double gamma, const double u[4]);
+void   orc_ref_flux(double gamma, const double u[4], double fx[4], double fy[4]);
+void   orc_ref_llflux(double gamma, const double ul[4], const double ur[4],
+                      const double fl[4], const double fr[4], double out[4]);
+/* limiters.f90:1-21 */
+double orc_minmod(double x, double y, double z);
+/* 2d/limiters.f90:30-54 (TVB) with M and dx given explicitly */
+double orc_generalized_minmod(double x, double y, double z, double M, double dx);
+/* 2d/benchmark_2d_dg.f90:1008-1026 compute_hllflux, reproduced as written
+ * (wave-speed bounds from the velocity magnitude); flag 1 = x, 2 = y. */
+void   orc_ref_hllflux(double gamma, const double ul[4], const double ur[4],
+                       const double fl[4], const double fr[4], double out[4]);
+
+/* ---- benchmark_2d.f90 restatements --------------------------------------- */
+/* get_equilibrium_solution (174-218) at nx*ny points (x,y given per point). */
+void   orc_ref2d_equilibrium(int nequilibrium, int n, const double* x, const double* y, double* w);
+/* compute_update_exact (465-618) on the reference's set-up: nequilibrium
+ * selects the closed form of the face equilibrium; gamma and box from p. */
+void   orc_ref2d_update_exact(const fvwb_params* p, int nequilibrium,
+                              const double* u, const double* w_eq, double* dudt);
+/* compute_max_speed (264-279) */
+double orc_ref2d_max_speed(const fvwb_params* p, const double* u);
+/* evolve (221-260): Heun RK2 while t < t_end, at most max_steps steps. */
+void   orc_ref2d_evolve(const fvwb_params* p, int nequilibrium, double* u, const double* w_eq,
+                        double t_end, int max_steps, double* t_out, int* steps_out);
+
+/* ---- configured scheme (plain FP64 form) --------------------------------- */
+/* closed-form equilibrium of p (rho0,p0,g) at one point, primitive */
+void   orc_fv2d_equilibrium(const fvwb_params* p, double x, double y, double w[4]);
+/* residual L(u) with p's recon / flux / bc; w_eq primitive at centres */
+void   orc_fv2d_residual(const fvwb_params* p, const double* u, const double* w_eq, double* dudt);
+/* numerical flux minus equilibrium flux at one face (dir 0 = x, 1 = y) */
+void   orc_fv2d_face_flux(const fvwb_params* p, int dir, const double ueq[4], const double dl[4],
+                          const double dr[4], double out[4]);
+/* one step of p's integrator with the given dt */
+void   orc_fv2d_step(const fvwb_params* p, double* u, const double* w_eq, double dt);
+/* max over cells of |v|+c (the reference's compute_max_speed for any gamma) */
+double orc_fv2d_max_speed(const fvwb_params* p, const double* u);
+/* advance: dt = 0.5*min(dx,dy)/cmax*cfl each step, integrator from p.
+ * Runs while t < t_end and steps < max_steps. */
+void   orc_fv2d_advance(const fvwb_params* p, double* u, const double* w_eq,
+                        double t_end, int max_steps, double* t_out, int* steps_out,
+                        double* dt_last);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

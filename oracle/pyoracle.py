@@ -1,0 +1,242 @@
+"""ctypes binding of the CPU FP64 oracle (liboracle_fvwb.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
+never by the product package.
+
+Arrays are numpy float64 in the reference's Fortran layout AoS(4,nx,ny): use
+``np.zeros((4, nx, ny), order="F")`` or equivalently a C array of shape
+(ny, nx, 4).  The helpers below accept and return Fortran-ordered (4,nx,ny).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_fvwb.so")
+
+BC_FROZEN_RING, BC_REFLECT = 0, 1
+RECON_FIRST_ORDER, RECON_MUSCL_MINMOD = 0, 1
+FLUX_RUSANOV, FLUX_HLL = 0, 1
+RK2_HEUN, SSPRK3 = 0, 1
+FP64, FP32 = 0, 1
+
+GAMMA_REF = float(np.float32(1.4))      # parameters_2d.f90:19, default-real literal
+RHO0_REF = float(np.float32(1.21))      # benchmark_2d.f90:196
+
+
+class Params(ctypes.Structure):
+    """Mirror of ``fvwb_params`` (include/fvwb.h)."""
+
+    _fields_ = [
+        ("nvar", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("bc", ctypes.c_int32), ("recon", ctypes.c_int32), ("flux", ctypes.c_int32),
+        ("integrator", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("boxlen_x", ctypes.c_double),
+        ("boxlen_y", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("rho0", ctypes.c_double), ("p0", ctypes.c_double), ("g", ctypes.c_double),
+    ]
+
+
+def make_params(nx, ny, *, bc=BC_FROZEN_RING, recon=RECON_FIRST_ORDER, flux=FLUX_RUSANOV,
+                integrator=RK2_HEUN, precision=FP64, gamma=GAMMA_REF, boxlen_x=1.0,
+                boxlen_y=1.0, cfl=0.5, rho0=RHO0_REF, p0=1.0, g=1.0) -> Params:
+    return Params(4, nx, ny, bc, recon, flux, integrator, precision, gamma, boxlen_x,
+                  boxlen_y, cfl, rho0, p0, g)
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-B", "-C", HERE, "liboracle_fvwb.so"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        pp = ctypes.POINTER(Params)
+        L.orc_ref2d_update_exact.argtypes = [pp, ctypes.c_int, dp, dp, dp]
+        L.orc_ref2d_max_speed.argtypes = [pp, dp]
+        L.orc_ref2d_max_speed.restype = ctypes.c_double
+        L.orc_ref2d_evolve.argtypes = [pp, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_int,
+                                       dp, ctypes.POINTER(ctypes.c_int)]
+        L.orc_fv2d_residual.argtypes = [pp, dp, dp, dp]
+        L.orc_fv2d_max_speed.argtypes = [pp, dp]
+        L.orc_fv2d_max_speed.restype = ctypes.c_double
+        L.orc_fv2d_advance.argtypes = [pp, dp, dp, ctypes.c_double, ctypes.c_int, dp,
+                                       ctypes.POINTER(ctypes.c_int), dp]
+        L.orc_fv2d_face_flux.argtypes = [pp, ctypes.c_int, dp, dp, dp, dp]
+        L.orc_fv2d_step.argtypes = [pp, dp, dp, ctypes.c_double]
+        L.orc_minmod.argtypes = [ctypes.c_double] * 3
+        L.orc_minmod.restype = ctypes.c_double
+        L.orc_generalized_minmod.argtypes = [ctypes.c_double] * 5
+        L.orc_generalized_minmod.restype = ctypes.c_double
+        for name in ("orc_ref_llflux", "orc_ref_hllflux"):
+            getattr(L, name).argtypes = [ctypes.c_double, dp, dp, dp, dp, dp]
+        L.orc_ref_flux.argtypes = [ctypes.c_double, dp, dp, dp]
+        L.orc_ref_speed.argtypes = [ctypes.c_double, dp]
+        L.orc_ref_speed.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1)
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def fortran(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+# ---- reference restatements ------------------------------------------------
+def ref2d_update_exact(p: Params, nequilibrium, u, w_eq):
+    u, w_eq = fortran(u), fortran(w_eq)
+    out = np.zeros_like(u, order="F")
+    lib().orc_ref2d_update_exact(ctypes.byref(p), nequilibrium, _p(u), _p(w_eq), _p(out))
+    return out
+
+
+def ref2d_max_speed(p: Params, u):
+    u = fortran(u)
+    return lib().orc_ref2d_max_speed(ctypes.byref(p), _p(u))
+
+
+def ref2d_evolve(p: Params, nequilibrium, u, w_eq, t_end, max_steps=1 << 30):
+    u = fortran(u).copy(order="F")
+    w_eq = fortran(w_eq)
+    t = ctypes.c_double()
+    n = ctypes.c_int()
+    lib().orc_ref2d_evolve(ctypes.byref(p), nequilibrium, _p(u), _p(w_eq), t_end, max_steps,
+                           ctypes.byref(t), ctypes.byref(n))
+    return u, t.value, n.value
+
+
+# ---- configured scheme ---------------------------------------------------------
+def fv2d_residual(p: Params, u, w_eq):
+    u, w_eq = fortran(u), fortran(w_eq)
+    out = np.zeros_like(u, order="F")
+    lib().orc_fv2d_residual(ctypes.byref(p), _p(u), _p(w_eq), _p(out))
+    return out
+
+
+def fv2d_advance(p: Params, u, w_eq, t_end=float("inf"), max_steps=1):
+    u = fortran(u).copy(order="F")
+    w_eq = fortran(w_eq)
+    t, dt = ctypes.c_double(), ctypes.c_double()
+    n = ctypes.c_int()
+    lib().orc_fv2d_advance(ctypes.byref(p), _p(u), _p(w_eq), t_end, max_steps, ctypes.byref(t),
+                           ctypes.byref(n), ctypes.byref(dt))
+    return u, t.value, n.value, dt.value
+
+
+def fv2d_step(p: Params, u, w_eq, dt):
+    u = fortran(u).copy(order="F")
+    w_eq = fortran(w_eq)
+    lib().orc_fv2d_step(ctypes.byref(p), _p(u), _p(w_eq), dt)
+    return u
+
+
+def fv2d_face_flux(p: Params, direction, ueq, dl, dr):
+    out = np.zeros(4)
+    ueq, dl, dr = map(_vec, (ueq, dl, dr))
+    lib().orc_fv2d_face_flux(ctypes.byref(p), direction, _p(ueq), _p(dl), _p(dr), _p(out))
+    return out
+
+
+def minmod(x, y, z):
+    return lib().orc_minmod(x, y, z)
+
+
+def generalized_minmod(x, y, z, M, dx):
+    return lib().orc_generalized_minmod(x, y, z, M, dx)
+
+
+def _vec(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def ref_flux(gamma, u):
+    u = _vec(u)
+    fx, fy = np.zeros(4), np.zeros(4)
+    lib().orc_ref_flux(gamma, _p(u), _p(fx), _p(fy))
+    return fx, fy
+
+
+def ref_llflux(gamma, ul, ur, fl, fr):
+    out = np.zeros(4)
+    ul, ur, fl, fr = map(_vec, (ul, ur, fl, fr))
+    lib().orc_ref_llflux(gamma, _p(ul), _p(ur), _p(fl), _p(fr), _p(out))
+    return out
+
+
+def ref_hllflux(gamma, ul, ur, fl, fr):
+    out = np.zeros(4)
+    ul, ur, fl, fr = map(_vec, (ul, ur, fl, fr))
+    lib().orc_ref_hllflux(gamma, _p(ul), _p(ur), _p(fl), _p(fr), _p(out))
+    return out
+
+
+# ---- problem set-ups shared by tests, smoke and bench ---------------------------
+def equilibrium_primitive(p: Params):
+    """Closed-form equilibrium at cell centres, primitive AoS(4,nx,ny)."""
+    nx, ny = p.nx, p.ny
+    dx, dy = p.boxlen_x / nx, p.boxlen_y / ny
+    x = (np.arange(nx) + 0.5) * dx
+    y = (np.arange(ny) + 0.5) * dy
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    k = p.rho0 * p.g / p.p0
+    e = np.exp(-(k * (X + Y)))
+    w = np.zeros((4, nx, ny), order="F")
+    w[0] = p.rho0 * e
+    w[3] = p.p0 * e
+    return w
+
+
+def conservative(gamma, w):
+    u = np.zeros_like(w, order="F")
+    u[0] = w[0]
+    u[1] = w[0] * w[1]
+    u[2] = w[0] * w[2]
+    u[3] = w[3] / (gamma - 1.0) + 0.5 * (w[0] * (w[1] * w[1] + w[2] * w[2]))
+    return u
+
+
+def hash_noise(nx, ny, salt):
+    """Deterministic noise in [-1,1) from an integer hash of (i,j,salt);
+    identical on every platform (no RNG stream involved)."""
+    i = np.arange(nx, dtype=np.uint64)[:, None]
+    j = np.arange(ny, dtype=np.uint64)[None, :]
+    h = (i * np.uint64(73856093)) ^ (j * np.uint64(19349663)) ^ np.uint64(salt * 83492791 + 17)
+    h = (h * np.uint64(0x9E3779B97F4A7C15)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    h ^= h >> np.uint64(29)
+    h = (h * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    h ^= h >> np.uint64(32)
+    return (h >> np.uint64(11)).astype(np.float64) / float(1 << 53) * 2.0 - 1.0
+
+
+def perturbed_state(p: Params, amp_bump=1e-3, amp_noise=1e-4, x0=0.3, y0=0.3, width=100.0,
+                    salt=1):
+    """u = U_eq + (0,0,0,dp/(gamma-1)) for a Gaussian pressure bump plus hash noise
+    of relative size amp_noise on every conservative variable."""
+    w = equilibrium_primitive(p)
+    ueq = conservative(p.gamma, w)
+    nx, ny = p.nx, p.ny
+    dx, dy = p.boxlen_x / nx, p.boxlen_y / ny
+    X, Y = np.meshgrid((np.arange(nx) + 0.5) * dx, (np.arange(ny) + 0.5) * dy, indexing="ij")
+    u = ueq.copy(order="F")
+    u[3] += amp_bump * np.exp(-width * ((X - x0) ** 2 + (Y - y0) ** 2)) / (p.gamma - 1.0)
+    if amp_noise:
+        for v in range(4):
+            scale = ueq[0] if v < 3 else ueq[3]
+            u[v] += amp_noise * scale * hash_noise(nx, ny, salt * 4 + v)
+    return u, w

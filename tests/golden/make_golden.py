@@ -1,0 +1,122 @@
+"""Write the golden fixtures in tests/golden/ from the flang-built reference.
+
+Run in the development container (needs oracle/_ref, built by
+``make -C oracle ref`` from the reference's own sources):
+
+    python tests/golden/make_golden.py
+
+Each fixture holds the inputs (or the recipe that regenerates them exactly) and
+the reference's outputs.  Large 300x300 outputs are stored as 4096 sampled
+entries plus per-variable sums, absolute sums and maxima.  Inputs are rounded
+through FP32 so that regenerating them on another CPU (whose exp may differ in
+the last FP64 bit) gives the same values.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
+
+import pyoracle as po  # noqa: E402
+import ref_runner as rr  # noqa: E402
+
+NSAMPLE = 4096
+
+
+def bench2d_inputs(salt=1, amp_bump=1e-3, amp_noise=1e-4):
+    """benchmark_2d set-up (nequilibrium=2, 300x300) plus a pressure bump and
+    hash noise; inputs rounded through FP32."""
+    p = po.make_params(300, 300)
+    u, w = po.perturbed_state(p, amp_bump=amp_bump, amp_noise=amp_noise, salt=salt)
+    u = np.asfortranarray(u.astype(np.float32).astype(np.float64))
+    w = np.asfortranarray(w.astype(np.float32).astype(np.float64))
+    return p, u, w
+
+
+def sample_index(shape, n=NSAMPLE, salt=7):
+    size = int(np.prod(shape))
+    h = po.hash_noise(n, 1, salt)[:, 0]
+    return np.unique(((h + 1.0) * 0.5 * size).astype(np.int64) % size)
+
+
+def summarize(a):
+    a = np.asarray(a)
+    flat = a.reshape(4, -1, order="F")
+    idx = sample_index(a.shape)
+    return {
+        "idx": idx,
+        "sample": a.reshape(-1, order="F")[idx],
+        "sum": flat.sum(axis=1),
+        "abssum": np.abs(flat).sum(axis=1),
+        "maxabs": np.abs(flat).max(axis=1),
+    }
+
+
+def main():
+    assert rr.available("bench2d") and rr.available("dg2d"), "build oracle/_ref first"
+    out = {}
+
+    # A1 compute_update_exact on a perturbed state and at equilibrium
+    p, u, w = bench2d_inputs()
+    d = rr.run("bench2d_update_exact", u=u, w_eq=w)["dudt"]
+    out["bench2d_update_exact_300"] = dict(summarize(d), salt=1, amp_bump=1e-3, amp_noise=1e-4)
+    ueq = po.conservative(p.gamma, w)
+    d0 = rr.run("bench2d_update_exact", u=ueq, w_eq=w)["dudt"]
+    out["bench2d_update_exact_eq_300"] = {"maxabs": np.abs(d0).max(axis=(1, 2)),
+                                          "nonzero": np.array(np.count_nonzero(d0))}
+
+    # A7 compute_max_speed
+    c = rr.run("bench2d_max_speed", u=u)["cmax"]
+    out["bench2d_max_speed_300"] = {"cmax": c, "salt": 1}
+
+    # A8 evolve: tend = 2.5 * first dt -> three Heun steps
+    dt0 = 0.5 * (1.0 / 300) / float(c[0]) * 0.5
+    tend = 2.5 * dt0
+    ue = rr.run("bench2d_evolve", u=u, w_eq=w, tend=np.array(tend))["u"]
+    out["bench2d_evolve_300"] = dict(summarize(ue - ueq), tend=np.array(tend), salt=1)
+
+    # A10 minmod / TVB generalized_minmod (2d/limiters.f90:18-54; dx = 1/8)
+    rng = np.random.default_rng(20261202)
+    xyz = rng.normal(size=(512, 3))
+    xyz[::7, 1] = 0.0
+    xyz[::11, 0] = 0.0
+    xyz[::13, 2] = -0.0
+    r = rr.run("dg2d_minmod", xyz=xyz, M=np.array(50.0))
+    out["minmod"] = {"xyz": xyz, "M": np.array(50.0), "dx": np.array(1.0 / 8),
+                     "minmod": r["minmod"], "generalized_minmod": r["generalized_minmod"]}
+
+    # A11 compute_hllflux (as written) and states for flux consistency
+    n = 256
+    ul = np.zeros((n, 4)); ur = np.zeros((n, 4))
+    for a in (ul, ur):
+        a[:, 0] = rng.uniform(0.2, 2.0, n)
+        a[:, 1] = rng.uniform(-2, 2, n) * a[:, 0]
+        a[:, 2] = rng.uniform(-2, 2, n) * a[:, 0]
+        a[:, 3] = rng.uniform(0.2, 3.0, n) / (p.gamma - 1) + 0.5 * (a[:, 1] ** 2 + a[:, 2] ** 2) / a[:, 0]
+    flag = rng.integers(1, 3, n).astype(np.int32)
+    fl = np.array([po.ref_flux(p.gamma, x)[f - 1] for x, f in zip(ul, flag)])
+    fr = np.array([po.ref_flux(p.gamma, x)[f - 1] for x, f in zip(ur, flag)])
+    f = rr.run("dg2d_hllflux", ul=ul, ur=ur, fl=fl, fr=fr, flag=flag)["f"]
+    out["dg2d_hllflux"] = {"ul": ul, "ur": ur, "fl": fl, "fr": fr, "flag": flag, "f": f,
+                           "gamma": np.array(p.gamma)}
+
+    # A18 Legendre basis and GL quadrature (2d/legendre.f90)
+    x = np.concatenate([np.linspace(-1, 1, 41), [-1.5, 1.5]])
+    r = rr.run("dg2d_legendre", x=x)
+    out["dg2d_legendre"] = {"x": x, "P": r["P"], "dP": r["dP"]}
+    out["dg2d_gl_quadrature"] = rr.run("dg2d_gl_quadrature")
+
+    for name, arrs in out.items():
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrs.items()})
+        print(f"wrote {path} ({os.path.getsize(path)} bytes)")
+
+
+if __name__ == "__main__":
+    main()

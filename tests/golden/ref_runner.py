@@ -1,0 +1,163 @@
+"""Run subroutines of the flang-built reference (oracle/_ref/libref_*.so).
+
+Used only as a checker: by make_golden.py to write the committed fixtures and by
+the CPU tests when oracle/_ref is present.  Every call runs in a child process
+(``run(op, **arrays)``) because the reference prints whole arrays to stdout on
+every call (benchmark_2d.f90:610) and keeps ~30 full-size automatic arrays on the
+stack (benchmark_2d.f90:468-483): the child sends stdout to /dev/null and runs the
+call on a thread with a 2 GiB stack.
+
+The reference's grid sizes are compile-time parameters of its own modules:
+benchmark_2d is 300x300 (parameters_2d.f90:3-4), benchmark_1d has nx=128
+(parameters.f90:3), 2d DG is 8x8 cells with mx=my=2 (2d/parameters_dg_2d.f90:3-6).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import sys
+import tempfile
+import threading
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+LIBS = {
+    "bench2d": "libref_bench2d.so",
+    "bench1d": "libref_bench1d.so",
+    "dg2d": "libref_dg2d.so",
+    "dg1d": "libref_dg1d.so",
+}
+
+
+def available(name="bench2d") -> bool:
+    return os.path.exists(os.path.join(REF_DIR, LIBS[name]))
+
+
+def run(op: str, **arrays):
+    """Run `op` in a child process; returns a dict of numpy arrays."""
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.npz"), os.path.join(td, "out.npz")
+        np.savez(fin, **{k: np.asarray(v) for k, v in arrays.items()})
+        with open(os.devnull, "w") as dn:
+            subprocess.run([sys.executable, os.path.abspath(__file__), op, fin, fout],
+                           stdout=dn, check=True)
+        with np.load(fout) as z:
+            return {k: z[k] for k in z.files}
+
+
+# ---- child side ------------------------------------------------------------------
+def _dp(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _child(op, fin, fout):
+    z = dict(np.load(fin))
+    out = {}
+    if op.startswith("bench2d_"):
+        L = ctypes.CDLL(os.path.join(REF_DIR, LIBS["bench2d"]))
+        if op == "bench2d_update_exact":
+            u = np.asfortranarray(z["u"]); w = np.asfortranarray(z["w_eq"])
+            d = np.zeros_like(u, order="F")
+            L.compute_update_exact_(_dp(u), _dp(w), _dp(d))
+            out["dudt"] = d
+        elif op == "bench2d_update_plain":
+            u = np.asfortranarray(z["u"]); w = np.asfortranarray(z["w_eq"])
+            d = np.zeros_like(u, order="F")
+            L.compute_update_(_dp(u), _dp(w), _dp(d))
+            out["dudt"] = d
+        elif op == "bench2d_max_speed":
+            u = np.asfortranarray(z["u"])
+            c = np.zeros(1)
+            L.compute_max_speed_(_dp(u), _dp(c))
+            out["cmax"] = c
+        elif op == "bench2d_evolve":
+            u = np.asfortranarray(z["u"]).copy(order="F"); w = np.asfortranarray(z["w_eq"])
+            tend = ctypes.c_double.in_dll(L, "_QMparameters_2dEtend")
+            tend.value = float(z["tend"])
+            L.evolve_(_dp(u), _dp(w))
+            out["u"] = u
+        elif op == "bench2d_equilibrium":
+            x = np.asfortranarray(z["x"]); y = np.asfortranarray(z["y"])
+            n1, n2 = x.shape
+            w = np.zeros((4, n1, n2), order="F")
+            a, b = ctypes.c_int(n1), ctypes.c_int(n2)
+            L.get_equilibrium_solution_(_dp(x), _dp(y), _dp(w), ctypes.byref(a), ctypes.byref(b))
+            out["w"] = w
+        else:
+            raise ValueError(op)
+    elif op.startswith("dg2d_"):
+        L = ctypes.CDLL(os.path.join(REF_DIR, LIBS["dg2d"]))
+        d = ctypes.c_double
+        if op == "dg2d_minmod":
+            L.minmod_.restype = d
+            L.generalized_minmod_.restype = d
+            xs = z["xyz"]
+            mm = np.zeros(len(xs)); gm = np.zeros(len(xs))
+            Mv = d.in_dll(L, "_QMparameters_dg_2dEm")
+            Mv.value = float(z["M"])
+            for q, (a, b, c) in enumerate(xs):
+                aa, bb, cc = d(a), d(b), d(c)
+                mm[q] = L.minmod_(ctypes.byref(aa), ctypes.byref(bb), ctypes.byref(cc))
+                aa, bb, cc = d(a), d(b), d(c)
+                gm[q] = L.generalized_minmod_(ctypes.byref(aa), ctypes.byref(bb), ctypes.byref(cc))
+            out["minmod"] = mm
+            out["generalized_minmod"] = gm
+        elif op == "dg2d_hllflux":
+            ul, ur, fl, fr = (np.ascontiguousarray(z[k]) for k in ("ul", "ur", "fl", "fr"))
+            n = ul.shape[0]
+            res = np.zeros((n, 4))
+            for q in range(n):
+                o = np.zeros(4)
+                flag = ctypes.c_int(int(z["flag"][q]))
+                a, b, c, e = (np.ascontiguousarray(v[q]) for v in (ul, ur, fl, fr))
+                L.compute_hllflux_(_dp(a), _dp(b), _dp(c), _dp(e), _dp(o), ctypes.byref(flag))
+                res[q] = o
+            out["f"] = res
+        elif op == "dg2d_legendre":
+            L.legendre_.restype = d
+            L.legendre_prime_.restype = d
+            xs = z["x"]
+            P = np.zeros((len(xs), 7)); dP = np.zeros((len(xs), 7))
+            for q, xv in enumerate(xs):
+                for n in range(7):
+                    a = d(xv); nn = ctypes.c_int(n)
+                    P[q, n] = L.legendre_(ctypes.byref(a), ctypes.byref(nn))
+                    a = d(xv)
+                    dP[q, n] = L.legendre_prime_(ctypes.byref(a), ctypes.byref(nn))
+            out["P"] = P
+            out["dP"] = dP
+        elif op == "dg2d_gl_quadrature":
+            res = {}
+            for n in range(1, 7):
+                xq = np.zeros(n); wq = np.zeros(n)
+                nn = ctypes.c_int(n)
+                L.gl_quadrature_(_dp(xq), _dp(wq), ctypes.byref(nn))
+                out[f"x{n}"] = xq
+                out[f"w{n}"] = wq
+        else:
+            raise ValueError(op)
+    else:
+        raise ValueError(op)
+    np.savez(fout, **out)
+
+
+if __name__ == "__main__":
+    threading.stack_size(1 << 31)
+    err = []
+
+    def _target():
+        try:
+            _child(*sys.argv[1:4])
+        except BaseException as e:  # noqa: BLE001 - reported to the parent below
+            err.append(e)
+
+    th = threading.Thread(target=_target)
+    th.start()
+    th.join()
+    if err:
+        sys.stderr.write(f"ref_runner {sys.argv[1]} failed: {err[0]!r}\n")
+        sys.exit(1)

@@ -1,0 +1,48 @@
+"""The C ABI library loads and exports every symbol include/fvwb.h declares,
+and argument errors are reported without touching the GPU."""
+import ctypes
+
+import pytest
+
+import fvwb
+
+
+def test_library_exports_every_declared_symbol():
+    L = fvwb.lib()
+    declared = fvwb.declared_symbols()
+    assert len(declared) >= 14
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_reference_params_match_parameters_2d():
+    p = fvwb.reference_params(300, 300, 2)
+    assert (p.nvar, p.nx, p.ny) == (4, 300, 300)
+    assert p.gamma == 1.399999976158142 and p.rho0 == 1.2100000381469727
+    assert (p.bc, p.recon, p.flux, p.integrator, p.precision) == (
+        fvwb.BC_FROZEN_RING, fvwb.RECON_FIRST_ORDER, fvwb.FLUX_RUSANOV, fvwb.RK2_HEUN, fvwb.FP64)
+    assert p.cfl == 0.5 and p.boxlen_x == 1.0 and p.boxlen_y == 1.0
+    q = fvwb.reference_params(64, 64, 1)
+    assert q.rho0 == 1.0 and q.p0 == 1.0 and q.g == 1.0
+    with pytest.raises(fvwb.FvwbError):
+        fvwb.reference_params(8, 8, 4)
+
+
+@pytest.mark.parametrize("field,value", [("nvar", 3), ("nx", 2), ("ny", 3), ("bc", 7), ("recon", 5),
+                                         ("flux", 9), ("integrator", 4), ("precision", 3),
+                                         ("gamma", 1.0), ("cfl", 0.0), ("rho0", -1.0)])
+def test_create_rejects_bad_params_before_any_device_call(field, value):
+    p = fvwb.make_params(16, 16)
+    setattr(p, field, value)
+    h = ctypes.c_void_p()
+    rc = fvwb.lib().fvwb_create(ctypes.byref(h), ctypes.byref(p))
+    assert rc == fvwb.ERR_ARG and not h.value
+    assert fvwb.lib().fvwb_last_error(None)
+
+
+def test_null_handle_calls_are_argument_errors():
+    L = fvwb.lib()
+    assert L.fvwb_advance(None, 1, 1.0, None, None) == fvwb.ERR_ARG
+    assert L.fvwb_download(None, None) == fvwb.ERR_ARG
+    assert L.fvwb_compute_update_exact(None, None, None, None) == fvwb.ERR_ARG
+    assert L.fvwb_destroy(None) == fvwb.OK
