@@ -60,6 +60,14 @@ __device__ __forceinline__ T minmod2(T a, T b) {
   return sa == sb ? (sa ? -m : m) : T(0);
 }
 
+// a*b rounded to T before any later use: the product may not be fused into a
+// following add (an FMA would cancel against an already-rounded value).
+template <typename T>
+__device__ __forceinline__ T mul_rounded(T a, T b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
 template <typename T>
 struct Vec4 { T r, mx, my, e; };
 
@@ -115,7 +123,8 @@ __device__ __forceinline__ Side<T> side_plain(const Phys<T>& P, const FaceEq<T>&
   T w0 = U.r;
   T w1 = U.mx / w0;
   T w2 = U.my / w0;
-  T w3 = P.gm1 * (U.e - P.half * w0 * (w1 * w1 + w2 * w2));
+  // the pressure is rounded once and shared by every use (see mul_rounded)
+  T w3 = mul_rounded(P.gm1, U.e - P.half * w0 * (w1 * w1 + w2 * w2));
   Side<T> s;
   if (DIR == 0) {
     s.f = {w1 * U.r, w1 * U.mx + w3, w0 * w1 * w2, w1 * U.e + w1 * w3};
@@ -167,8 +176,9 @@ __device__ __forceinline__ Vec4<T> face_flux(const Phys<T>& P, const FaceEq<T>& 
     R = side_plain<T, DIR>(P, q, dr, jr);
     // equilibrium flux through compute_flux of the resting face state: its
     // pressure is (gamma-1)*(E_eq - 0), equal to the one-sided flux at dU == 0
-    // bit for bit (also under FMA contraction, since every dropped term is 0).
-    T pe = P.gm1 * q.e;
+    // bit for bit.  It must be rounded before the final subtraction: fused
+    // into fh - fe it would leave the product's rounding error behind.
+    T pe = mul_rounded(P.gm1, q.e);
     if (DIR == 0) fe.mx = pe; else fe.my = pe;
   } else {
     L = side_dev<T, DIR>(P, q, dl);

@@ -102,7 +102,11 @@ int fvwb_compute_max_speed(fvwb_handle h, const double* u, double* cmax);
 /* ---- device-resident stepping (replaces the loop of `evolve`,
  *      benchmark_2d.f90:221-260) --------------------------------------------
  * upload: dU = u - conservative(w_eq) is formed on the device in FP64 and
- * stored in the handle's precision; t is reset to 0. */
+ * stored in the handle's precision; t is reset to 0.  The face equilibria and
+ * the centre equilibrium entering the wave speed for dt are the closed form of
+ * the params (as in every reference set-up, where w_eq is that closed form);
+ * w_eq itself is kept only to form dU and to rebuild u on download, so a
+ * state equal to conservative(w_eq) stays exactly at rest. */
 int fvwb_upload(fvwb_handle h, const double* u, const double* w_eq);
 /* Initialise on the device (no host arrays): dU = 0 except a pressure bump
  * dp = amp*exp(-width*((x-x0)^2+(y-y0)^2)), i.e. dE = dp/(gamma-1).  w_eq is
