@@ -150,6 +150,8 @@ def main():
     # a launch moves on average cells*128/3 B in ms_per_step/3.
     bytes_per_step = cells * 128.0
     achieved = bytes_per_step / sec32 * K / 1e9
+    copy_gbs = fvwb.copy_bandwidth(1 << 30, 10)
+    print(f"[bench] device copy bandwidth {copy_gbs:.1f} GB/s", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
 
     out = {
@@ -171,7 +173,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "fv2d_stage_kernel<float,MUSCL,HLL,REFLECT,DEV>",
-                     "bytes_per_cell_update": 128},
+                     "bytes_per_cell_update": 128, "copy_gbs": copy_gbs,
+                     "frac_of_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
         "fp64_plain": {"value": rate64, "size": n, "bytes_per_cell_update": 256,
                        "roofline_frac": rate64 * 256 / 8e12,

@@ -415,3 +415,35 @@ extern "C" int fvwb_get_stream(fvwb_handle h, void** stream) {
   *stream = (void*)h->stream;
   return FVWB_OK;
 }
+
+namespace fvwb {
+hipError_t launch_copy_probe(const void* src, void* dst, long long bytes, hipStream_t s);
+}
+
+extern "C" int fvwb_copy_bandwidth(uint64_t bytes, int32_t reps, double* gbps) {
+  if (!gbps || bytes < 16 || reps < 1) return fail(nullptr, FVWB_ERR_ARG, "bad copy probe arguments");
+  bytes &= ~(uint64_t)15;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = FVWB_OK;
+  float ms = 0.f;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+      hipMemset(a, 1, bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(nullptr, FVWB_ERR_OOM, "copy probe allocation failed");
+  } else {
+    (void)launch_copy_probe(a, b, (long long)bytes, nullptr);
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) (void)launch_copy_probe(a, b, (long long)bytes, nullptr);
+    (void)hipEventRecord(e1, nullptr);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = fail(nullptr, FVWB_ERR_HIP, "copy probe timing failed");
+    else
+      *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}

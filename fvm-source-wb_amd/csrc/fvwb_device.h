@@ -17,47 +17,70 @@
 namespace fvwb {
 
 // ---- cross-lane moves (wave64) -------------------------------------------------
-// lane l receives the value of lane l-1 (lane 0 receives its own value).
+// lane l receives the value of lane l-1; lane 0 receives 0 (bound_ctrl), which
+// lets the compiler fold the move into the consuming VALU op as a DPP operand.
+// Lanes at the wave edges are halo lanes whose results are never stored.
 __device__ __forceinline__ float lane_from_left(float x) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-      __builtin_bit_cast(int, x), __builtin_bit_cast(int, x), 0x138 /*wave_shr:1*/, 0xF, 0xF, false));
+      0, __builtin_bit_cast(int, x), 0x138 /*wave_shr:1*/, 0xF, 0xF, true));
 }
-// lane l receives the value of lane l+1 (lane 63 receives its own value).
+// lane l receives the value of lane l+1; lane 63 receives 0.
 __device__ __forceinline__ float lane_from_right(float x) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-      __builtin_bit_cast(int, x), __builtin_bit_cast(int, x), 0x130 /*wave_shl:1*/, 0xF, 0xF, false));
+      0, __builtin_bit_cast(int, x), 0x130 /*wave_shl:1*/, 0xF, 0xF, true));
 }
 __device__ __forceinline__ double lane_from_left(double x) {
   int2 v = __builtin_bit_cast(int2, x);
-  v.x = __builtin_amdgcn_update_dpp(v.x, v.x, 0x138, 0xF, 0xF, false);
-  v.y = __builtin_amdgcn_update_dpp(v.y, v.y, 0x138, 0xF, 0xF, false);
+  v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x138, 0xF, 0xF, true);
+  v.y = __builtin_amdgcn_update_dpp(0, v.y, 0x138, 0xF, 0xF, true);
   return __builtin_bit_cast(double, v);
 }
 __device__ __forceinline__ double lane_from_right(double x) {
   int2 v = __builtin_bit_cast(int2, x);
-  v.x = __builtin_amdgcn_update_dpp(v.x, v.x, 0x130, 0xF, 0xF, false);
-  v.y = __builtin_amdgcn_update_dpp(v.y, v.y, 0x130, 0xF, 0xF, false);
+  v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x130, 0xF, 0xF, true);
+  v.y = __builtin_amdgcn_update_dpp(0, v.y, 0x130, 0xF, 0xF, true);
   return __builtin_bit_cast(double, v);
 }
 
 // ---- small math helpers --------------------------------------------------------
-template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
-template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
-__device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
-__device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float tmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float tmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double tmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double tmin(double a, double b) { return __builtin_fmin(a, b); }
+// Square roots enter only the wave-speed bounds.  FP32: hardware v_sqrt_f32
+// (1 ulp).  FP64: v_rsq_f64 is only an approximation, so it seeds a Goldschmidt
+// iteration on (g, h) -> (sqrt(x), 1/(2 sqrt(x))) plus one residual correction,
+// accurate to about an ulp for x > 0; x == 0 (|v| of a resting state) gives 0.
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double tsqrt(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  double g = x * r, h = 0.5 * r;
+  double e = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  e = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  double d = __builtin_fma(-g, g, x);
+  return x > 0.0 ? __builtin_fma(d, h, g) : 0.0;
+}
 __device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
 __device__ __forceinline__ double tabs(double x) { return __builtin_fabs(x); }
-// reciprocal: IEEE division in FP64, hardware v_rcp_f32 (1 ulp) in FP32
-__device__ __forceinline__ double trcp(double x) { return 1.0 / x; }
+// reciprocal: hardware v_rcp_f32 (1 ulp) in FP32; in FP64 v_rcp_f64 refined by
+// two Newton steps r <- r + r(1 - x r) (about 1 ulp, half the cost of a division)
+__device__ __forceinline__ double trcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+  return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+}
 __device__ __forceinline__ float trcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // minmod of limiters.f90:1-21 with the third argument equal to the second:
-// s*min(|a|,|b|) if a and b carry the same sign (sign(1,0) = +1), else 0.
-template <typename T>
-__device__ __forceinline__ T minmod2(T a, T b) {
-  bool sa = __builtin_signbit(a), sb = __builtin_signbit(b);
-  T m = tmin(tabs(a), tabs(b));
-  return sa == sb ? (sa ? -m : m) : T(0);
+// s*min(|a|,|b|) if a and b carry the same sign, else 0 -- which is the median
+// of (a, b, 0): one v_med3_f32 in FP32.
+__device__ __forceinline__ float minmod2(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 0.0f); }
+__device__ __forceinline__ double minmod2(double a, double b) {
+  return __builtin_fmax(__builtin_fmin(a, b), __builtin_fmin(__builtin_fmax(a, b), 0.0));
 }
 
 // a*b rounded to T before any later use: the product may not be fused into a
@@ -98,10 +121,11 @@ struct Phys {
   T g;                  // |grad phi| per axis
 };
 
-// Face equilibrium at rest: density, pressure and energy p/(gamma-1) (the
-// reference's compute_conservative of a resting state, benchmark_2d.f90:169).
+// Face equilibrium at rest: density, pressure, energy p/(gamma-1) (the
+// reference's compute_conservative of a resting state, benchmark_2d.f90:169)
+// and E+p.
 template <typename T>
-struct FaceEq { T rho, p, e; };
+struct FaceEq { T rho, p, e, ep; };
 
 enum { FLUX_RUSANOV = 0, FLUX_HLL = 1 };
 enum { FORM_PLAIN = 0, FORM_DEV = 1 };
@@ -115,14 +139,17 @@ struct Side {
   T vmag;         // |v| (Rusanov bound of the reference)
 };
 
-// PLAIN form: U = U_eq + dU in full, primitive and flux exactly as
-// compute_primitive / compute_flux (benchmark_2d.f90:153-156, 315-323).
+// PLAIN form: U = U_eq + dU in full, primitive and flux as compute_primitive /
+// compute_flux (benchmark_2d.f90:153-156, 315-323).  One reciprocal per side
+// (1/max(rho,1e-10)) serves both velocities and the sound speed; for physical
+// states this differs from u(2)/w(1) by about one rounding.
 template <typename T, int DIR>
 __device__ __forceinline__ Side<T> side_plain(const Phys<T>& P, const FaceEq<T>& q, Vec4<T> d, Vec4<T>& U) {
   U = {q.rho + d.r, d.mx, d.my, q.e + d.e};
   T w0 = U.r;
-  T w1 = U.mx / w0;
-  T w2 = U.my / w0;
+  T inv = trcp(tmax(w0, T(1e-10)));
+  T w1 = U.mx * inv;
+  T w2 = U.my * inv;
   // the pressure is rounded once and shared by every use (see mul_rounded)
   T w3 = mul_rounded(P.gm1, U.e - P.half * w0 * (w1 * w1 + w2 * w2));
   Side<T> s;
@@ -133,7 +160,7 @@ __device__ __forceinline__ Side<T> side_plain(const Phys<T>& P, const FaceEq<T>&
     s.f = {U.r * w2, U.mx * w2, U.my * w2 + w3, w2 * U.e + w2 * w3};
     s.vn = w2;
   }
-  s.c = tsqrt(P.gamma * tmax(w3, T(1e-10)) / tmax(w0, T(1e-10)));
+  s.c = tsqrt(P.gamma * tmax(w3, T(1e-10)) * inv);
   s.vmag = tsqrt(w1 * w1 + w2 * w2);
   return s;
 }
@@ -144,11 +171,10 @@ __device__ __forceinline__ Side<T> side_plain(const Phys<T>& P, const FaceEq<T>&
 template <typename T, int DIR>
 __device__ __forceinline__ Side<T> side_dev(const Phys<T>& P, const FaceEq<T>& q, Vec4<T> d) {
   T rho = q.rho + d.r;
-  T inv = trcp(rho);
+  T inv = trcp(tmax(rho, T(1e-10)));
   T u = d.mx * inv, v = d.my * inv;
   T dp = P.gm1 * (d.e - P.half * (d.mx * u + d.my * v));
-  T p = q.p + dp;
-  T ep = (q.e + d.e) + p;
+  T ep = (q.ep + d.e) + dp;      // E + p
   Side<T> s;
   if (DIR == 0) {
     s.f = {d.mx, d.mx * u + dp, d.my * u, ep * u};
@@ -157,7 +183,7 @@ __device__ __forceinline__ Side<T> side_dev(const Phys<T>& P, const FaceEq<T>& q
     s.f = {d.my, d.mx * v, d.my * v + dp, ep * v};
     s.vn = v;
   }
-  s.c = tsqrt(P.gamma * tmax(p, T(1e-10)) * trcp(tmax(rho, T(1e-10))));
+  s.c = tsqrt(P.gamma * tmax(q.p + dp, T(1e-10)) * inv);
   s.vmag = tsqrt(u * u + v * v);
   return s;
 }
@@ -215,11 +241,12 @@ __device__ __forceinline__ Vec4<T> face_flux(const Phys<T>& P, const FaceEq<T>& 
 template <typename T>
 __device__ __forceinline__ T cell_speed(const Phys<T>& P, T rho_eq, T e_eq, Vec4<T> d, bool& ok) {
   T rho = rho_eq + d.r;
-  T inv = T(1) / rho;
+  T inv = trcp(tmax(rho, T(1e-10)));
   T u = d.mx * inv, v = d.my * inv;
   T p = P.gm1 * ((e_eq + d.e) - P.half * rho * (u * u + v * v));
-  ok = (rho > T(0)) && (p > T(0)) && (p < T(3.0e38)) && (rho < T(3.0e38));
-  return tsqrt(u * u + v * v) + tsqrt(P.gamma * tmax(p, T(1e-10)) / tmax(rho, T(1e-10)));
+  T sp = tsqrt(u * u + v * v) + tsqrt(P.gamma * tmax(p, T(1e-10)) * inv);
+  ok = (rho > T(0)) && (p > T(0)) && (sp < T(1e30));
+  return sp;
 }
 
 }  // namespace fvwb

@@ -31,12 +31,38 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return xcd * base + (xcd < rem ? xcd : rem) + idx;
 }
 
+// Row access through a buffer descriptor built on the scalar unit: base = the
+// row's first element (uniform), extent = the row, per-lane 32-bit byte offset.
+// No vector address arithmetic; a lane offset past the row would read 0 / be
+// dropped by the hardware range check.
+template <typename T>
+__device__ __forceinline__ T row_load(const T* row, unsigned boff, int row_bytes) {
+  __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, row_bytes, 0x00020000);
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, boff, 0, 0));
+  }
+}
+template <typename T>
+__device__ __forceinline__ void row_store(T* row, unsigned boff, int row_bytes, T v) {
+  __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, row_bytes, 0x00020000);
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, boff, 0, 0);
+  } else {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, boff, 0, 0);
+  }
+}
+
 template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE>
 __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
   constexpr int H = RECON ? 2 : 1;
   constexpr int COLS = 64 - 2 * H;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // wave index made explicitly uniform so that strip, row and row pointers
+  // live in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
   // ---- step control (dt from the previous step's reduction) -----------------
   double dt = 0.0;
@@ -70,33 +96,45 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
   const int y1 = min(y0 + a.rows, ny);
   const bool own = lane >= H && lane < 64 - H && i < nx;
 
-  // column index with the boundary treatment (mirror for walls, clamp otherwise)
-  int ic;
+  // column with the boundary treatment (mirror for walls, clamp otherwise)
+  int icol;
   T sx = T(1);
   if (BC == BC_REFLECT) {
-    ic = i < 0 ? -1 - i : (i >= nx ? 2 * nx - 1 - i : i);
+    icol = i < 0 ? -1 - i : (i >= nx ? 2 * nx - 1 - i : i);
     if (i < 0 || i >= nx) sx = T(-1);
-    ic = clampi(ic, 0, nx - 1);
+    icol = clampi(icol, 0, nx - 1);
   } else {
-    ic = clampi(i, 0, nx - 1);
+    icol = clampi(i, 0, nx - 1);
   }
+  const unsigned boff = (unsigned)icol * (unsigned)sizeof(T);
+  const int rowb = nx * (int)sizeof(T);
   const long long plane = a.plane;
   const int pitch = a.pitch;
 
   auto load_row = [&](const T* base, int j) -> Vec4<T> {
     int jc;
-    T sy = T(1);
+    bool flip = false;
     if (BC == BC_REFLECT) {
       jc = j < 0 ? -1 - j : (j >= ny ? 2 * ny - 1 - j : j);
-      if (j < 0 || j >= ny) sy = T(-1);
+      flip = j < 0 || j >= ny;
       jc = clampi(jc, 0, ny - 1);
     } else {
       jc = clampi(j, 0, ny - 1);
     }
-    const T* s = base + (long long)jc * pitch + ic;
-    Vec4<T> v = {s[0], s[plane], s[2 * plane], s[3 * plane]};
-    if (BC == BC_REFLECT) { v.mx *= sx; v.my *= sy; }
+    const T* r = base + (long long)jc * pitch;
+    Vec4<T> v = {row_load(r, boff, rowb), row_load(r + plane, boff, rowb),
+                 row_load(r + 2 * plane, boff, rowb), row_load(r + 3 * plane, boff, rowb)};
+    if (BC == BC_REFLECT) {
+      v.mx *= sx;
+      if (flip) v.my = -v.my;
+    }
     return v;
+  };
+  // cell-centred rows (clamped into the grid; rows past the strip are masked)
+  auto load_cell = [&](const T* base, int j) -> Vec4<T> {
+    const T* r = base + (long long)min(j, ny - 1) * pitch;
+    return Vec4<T>{row_load(r, boff, rowb), row_load(r + plane, boff, rowb),
+                   row_load(r + 2 * plane, boff, rowb), row_load(r + 3 * plane, boff, rowb)};
   };
 
   Phys<T> P;
@@ -105,66 +143,83 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
   P.half = T(0.5);
   P.g = T(a.g);
   const T odx = T(a.odx), ody = T(a.ody);
-  const T tdt = T(dt);
-  const T alpha = T(a.alpha), beta = T(a.beta);
+  // stage combination out = alpha*U^n + beta*D + (beta*dt)*L
+  const T alpha = T(a.alpha), beta = T(a.beta), bdt = T(a.beta * dt);
 
-  // separable equilibrium: rho = rho0 e^{-kx} e^{-ky}, p = rho * p0/rho0
+  // separable FP64 equilibrium: rho = (rho0 e^{-kx}) e^{-ky} is formed in FP64
+  // and rounded once; p, E and E+p scale with rho by constant ratios.
   const double rxf = a.rxf[clampi(i, 0, nx)];       // left x face of this column
   const double rxc = a.rxc[clampi(i, 0, nx - 1)];
+  const T por = T(a.p_over_rho), eor = T(a.p_over_rho * a.inv_gm1);
+  const T epor = T(a.p_over_rho * a.gamma * a.inv_gm1);
   auto face_eq = [&](double rx, double ey) -> FaceEq<T> {
-    double r = rx * ey;
-    double pr = r * a.p_over_rho;
-    return FaceEq<T>{T(r), T(pr), T(pr * a.inv_gm1)};
+    FaceEq<T> f;
+    f.rho = T(rx * ey);
+    f.p = f.rho * por;
+    f.e = f.rho * eor;
+    f.ep = f.rho * epor;
+    return f;
   };
 
   double smax = 0.0;
   bool bad = false;
 
-  // ---- prologue: window rows and the y face below the first row -------------
-  Vec4<T> D0, Dp1, Dp2, sy0, Gp;
-  if (RECON) {
-    Vec4<T> Dm2 = load_row(a.in, y0 - 2), Dm1 = load_row(a.in, y0 - 1);
-    D0 = load_row(a.in, y0);
-    Dp1 = load_row(a.in, y0 + 1);
-    Vec4<T> sym1 = minmod2(Dm1 - Dm2, D0 - Dm1);
-    sy0 = minmod2(D0 - Dm1, Dp1 - D0);
+  // ---- prologue -----------------------------------------------------------------
+  // A ring of RING rows holds rows j..j+RING-2 while row j is computed and
+  // receives row j+RING-1 (so RING-3 rows are in flight ahead of use); U^n uses
+  // a ring of URING rows the same way.  The body is unrolled RING times with
+  // rotated roles, so the rings never move data between registers.
+  constexpr int RING = sizeof(T) == 4 ? 6 : 4;
+  constexpr int URING = sizeof(T) == 4 ? 3 : 2;
+  Vec4<T> R[RING], Ur[URING], sy, Gp, dfw;
+  {
     FaceEq<T> qy = face_eq(rxc, a.eyf[clampi(y0, 0, ny)]);
-    Gp = face_flux<T, FLUX, 1, FORM>(P, qy, Dm1 + scale(P.half, sym1), D0 - scale(P.half, sy0));
-    Dp2 = load_row(a.in, y0 + 2);
-  } else {
     Vec4<T> Dm1 = load_row(a.in, y0 - 1);
-    D0 = load_row(a.in, y0);
-    FaceEq<T> qy = face_eq(rxc, a.eyf[clampi(y0, 0, ny)]);
-    Gp = face_flux<T, FLUX, 1, FORM>(P, qy, Dm1, D0);
-    Dp1 = load_row(a.in, y0 + 1);
+#pragma unroll
+    for (int k = 0; k < RING - 1; ++k) R[k] = load_row(a.in, y0 + k);
+    if (RECON) {
+      Vec4<T> Dm2 = load_row(a.in, y0 - 2);
+      Vec4<T> sym1 = minmod2(Dm1 - Dm2, R[0] - Dm1);
+      sy = minmod2(R[0] - Dm1, R[1] - R[0]);
+      dfw = R[1] - R[0];   // B - A of the first row
+      Gp = face_flux<T, FLUX, 1, FORM>(P, qy, Dm1 + scale(P.half, sym1), R[0] - scale(P.half, sy));
+    } else {
+      Gp = face_flux<T, FLUX, 1, FORM>(P, qy, Dm1, R[0]);
+    }
+#pragma unroll
+    for (int k = 0; k < URING; ++k) Ur[k] = Vec4<T>{T(0), T(0), T(0), T(0)};
+    if (MODE == MODE_STAGE) {
+#pragma unroll
+      for (int k = 0; k < URING - 1; ++k) Ur[k] = load_cell(a.un, y0 + k);
+    }
   }
 
-  for (int j = y0; j < y1; ++j) {
-    // prefetch the row entering the window next iteration
-    Vec4<T> Dnext = load_row(a.in, j + (RECON ? 3 : 2));
-    Vec4<T> Un = {T(0), T(0), T(0), T(0)};
-    if (MODE == MODE_STAGE && own) {
-      const T* s = a.un + (long long)j * pitch + i;
-      Un = {s[0], s[plane], s[2 * plane], s[3 * plane]};
-    }
-    const double eyc = a.eyc[j];
-    const double eyf = a.eyf[j + 1];
+  // one row: A = row j, B = row j+1, C = row j+2; S receives row j+RING-1; UA
+  // holds U^n(j) and US receives U^n(j+URING-1).
+  auto row_step = [&](int j, const Vec4<T>& A, const Vec4<T>& B, const Vec4<T>& C, Vec4<T>& S,
+                      const Vec4<T>& UA, Vec4<T>& US) {
+    S = load_row(a.in, j + RING - 1);
+    if (MODE == MODE_STAGE) US = load_cell(a.un, j + URING - 1);
+    const int jt = min(j, ny - 1);
+    const double eyc = a.eyc[jt];
+    const double eyf = a.eyf[jt + 1];
 
     // y face above row j
     Vec4<T> Gn, xl, xr;
     if (RECON) {
-      Vec4<T> syp1 = minmod2(Dp1 - D0, Dp2 - Dp1);
-      Gn = face_flux<T, FLUX, 1, FORM>(P, face_eq(rxc, eyf), D0 + scale(P.half, sy0),
-                                       Dp1 - scale(P.half, syp1));
-      sy0 = syp1;
+      Vec4<T> dnext = C - B;
+      Vec4<T> syn = minmod2(dfw, dnext);   // dfw = B - A, carried from the previous row
+      dfw = dnext;
+      Gn = face_flux<T, FLUX, 1, FORM>(P, face_eq(rxc, eyf), A + scale(P.half, sy), B - scale(P.half, syn));
+      sy = syn;
       // x slopes on row j from the neighbouring lanes
-      Vec4<T> sxl = minmod2(D0 - from_left(D0), from_right(D0) - D0);
-      xr = D0 + scale(P.half, sxl);   // state at this cell's right face
-      xl = D0 - scale(P.half, sxl);   // state at this cell's left face
+      Vec4<T> sxl = minmod2(A - from_left(A), from_right(A) - A);
+      xr = A + scale(P.half, sxl);   // state at this cell's right face
+      xl = A - scale(P.half, sxl);   // state at this cell's left face
     } else {
-      Gn = face_flux<T, FLUX, 1, FORM>(P, face_eq(rxc, eyf), D0, Dp1);
-      xr = D0;
-      xl = D0;
+      Gn = face_flux<T, FLUX, 1, FORM>(P, face_eq(rxc, eyf), A, B);
+      xr = A;
+      xl = A;
     }
     // x face on this lane's left: left state from the neighbour's right face
     Vec4<T> Fl = face_flux<T, FLUX, 0, FORM>(P, face_eq(rxf, eyc), from_left(xr), xl);
@@ -173,36 +228,42 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
     // residual: -dF/dx - dG/dy + (s(U) - s(U_eq)), s = (0, -rho g, -rho g, -g (m_x+m_y))
     Vec4<T> L;
     L.r = -(Fr.r - Fl.r) * odx - (Gn.r - Gp.r) * ody;
-    L.mx = (-(Fr.mx - Fl.mx) * odx - (Gn.mx - Gp.mx) * ody) + (-P.g * D0.r);
-    L.my = (-(Fr.my - Fl.my) * odx - (Gn.my - Gp.my) * ody) + (-P.g * D0.r);
-    L.e = (-(Fr.e - Fl.e) * odx - (Gn.e - Gp.e) * ody) + (-P.g * (D0.mx + D0.my));
+    L.mx = (-(Fr.mx - Fl.mx) * odx - (Gn.mx - Gp.mx) * ody) - P.g * A.r;
+    L.my = (-(Fr.my - Fl.my) * odx - (Gn.my - Gp.my) * ody) - P.g * A.r;
+    L.e = (-(Fr.e - Fl.e) * odx - (Gn.e - Gp.e) * ody) - P.g * (A.mx + A.my);
+    Gp = Gn;
     if (BC == BC_FROZEN && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) L = {T(0), T(0), T(0), T(0)};
 
-    if (own) {
-      Vec4<T> o;
-      if (MODE == MODE_RESIDUAL) {
-        o = L;
-      } else if (MODE == MODE_STAGE1) {
-        o = D0 + scale(tdt, L);
-      } else {
-        o = scale(alpha, Un) + scale(beta, D0 + scale(tdt, L));
-      }
-      T* d = a.out + (long long)j * pitch + i;
-      d[0] = o.r;
-      d[plane] = o.mx;
-      d[2 * plane] = o.my;
-      d[3 * plane] = o.e;
+    Vec4<T> o;
+    if (MODE == MODE_RESIDUAL) {
+      o = L;
+    } else if (MODE == MODE_STAGE1) {
+      o = A + scale(bdt, L);
+    } else {
+      o = scale(alpha, UA) + (scale(beta, A) + scale(bdt, L));
+    }
+    if (own && j < y1) {
+      T* d = a.out + (long long)j * pitch;
+      row_store(d, boff, rowb, o.r);
+      row_store(d + plane, boff, rowb, o.mx);
+      row_store(d + 2 * plane, boff, rowb, o.my);
+      row_store(d + 3 * plane, boff, rowb, o.e);
       if (MODE != MODE_RESIDUAL && a.last) {
         bool ok;
         double rc = rxc * eyc;
-        T sp = cell_speed(P, T(rc), T(rc * a.p_over_rho * a.inv_gm1), o, ok);
+        T rct = T(rc);
+        T sp = cell_speed(P, rct, rct * eor, o, ok);
         smax = fmax(smax, (double)sp);
         bad |= !ok;
       }
     }
-    Gp = Gn;
-    D0 = Dp1;
-    if (RECON) { Dp1 = Dp2; Dp2 = Dnext; } else { Dp1 = Dnext; }
+  };
+
+  for (int j = y0; j < y1; j += RING) {
+#pragma unroll
+    for (int k = 0; k < RING; ++k)
+      row_step(j + k, R[k % RING], R[(k + 1) % RING], R[(k + 2) % RING], R[(k + RING - 1) % RING],
+               Ur[k % URING], Ur[(k + URING - 1) % URING]);
   }
 
   if (MODE != MODE_RESIDUAL && a.last) {
@@ -413,5 +474,33 @@ hipError_t launch_init_gaussian(T* out, double* w, int nx, int ny, int pitch, lo
 }
 template hipError_t launch_init_gaussian<double>(double*, double*, int, int, int, long long, double, double, double, double, double, double, double, double, double, double, hipStream_t);
 template hipError_t launch_init_gaussian<float>(float*, double*, int, int, int, long long, double, double, double, double, double, double, double, double, double, double, hipStream_t);
+
+}  // namespace fvwb
+
+namespace fvwb {
+
+// Plain streaming copy (16 B per lane per iteration): the calibration point for
+// the achieved-bandwidth and HBM-counter figures of the solver kernels.
+__global__ __launch_bounds__(256) void copy_probe_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                         long long n4) {
+  // each thread moves 4 float4 per iteration, all four loads issued before the
+  // stores, so a wave keeps 4 KiB of reads in flight
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; k + 3 * stride < n4; k += 4 * stride) {
+    float4 a = src[k], b = src[k + stride], c = src[k + 2 * stride], d = src[k + 3 * stride];
+    dst[k] = a;
+    dst[k + stride] = b;
+    dst[k + 2 * stride] = c;
+    dst[k + 3 * stride] = d;
+  }
+  for (; k < n4; k += stride) dst[k] = src[k];
+}
+
+hipError_t launch_copy_probe(const void* src, void* dst, long long bytes, hipStream_t s) {
+  long long n4 = bytes / 16;
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(256 * 16), dim3(256), 0, s, (const float4*)src, (float4*)dst, n4);
+  return hipGetLastError();
+}
 
 }  // namespace fvwb

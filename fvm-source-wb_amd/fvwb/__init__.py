@@ -95,6 +95,7 @@ def lib():
         "fvwb_download_deviation": [H, dp],
         "fvwb_get_time": [H, dp, dp, ip],
         "fvwb_get_stream": [H, ctypes.POINTER(ctypes.c_void_p)],
+        "fvwb_copy_bandwidth": [ctypes.c_uint64, ctypes.c_int32, dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -210,6 +211,13 @@ class Solver2D:
         s = ctypes.c_void_p()
         _check(lib().fvwb_get_stream(self._h, ctypes.byref(s)), self._h)
         return s.value
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Device-to-device streaming copy bandwidth in GB/s (read + write)."""
+    g = ctypes.c_double()
+    _check(lib().fvwb_copy_bandwidth(nbytes, reps, ctypes.byref(g)))
+    return g.value
 
 
 def evolve(u, u_eq, params: Params, t_end=math.inf, max_steps=1 << 30):

@@ -129,6 +129,13 @@ int fvwb_get_time(fvwb_handle h, double* t, double* dt_last, int32_t* steps_done
 /* The handle's stream (a hipStream_t), for event timing by the caller. */
 int fvwb_get_stream(fvwb_handle h, void** stream);
 
+/* ---- measurement helper (no counterpart in the reference) ------------------
+ * Device-to-device streaming copy of `bytes` (rounded down to 16), repeated
+ * `reps` times on the null stream; *gbps = 2*bytes*reps / elapsed.  The
+ * calibration point for achieved bandwidth and HBM counters.  Allocates and
+ * frees its own two buffers. */
+int fvwb_copy_bandwidth(uint64_t bytes, int32_t reps, double* gbps);
+
 #ifdef __cplusplus
 }
 #endif
