@@ -1,22 +1,25 @@
 """Benchmark: cell-updates/s of the well-balanced Euler-with-gravity solver on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--drift-steps D]
 
-Prints ONE JSON line.  A step is one full SSP-RK3 time step (three fused stage
-kernels, dt from the on-device CFL reduction) over the whole workload; inputs are
-resident in HBM before the timed region, which is bracketed by HIP events on the
-solver's stream and a device synchronisation on both sides.
+Prints ONE JSON line.  A step advances every instance by one full SSP-RK3 time
+step (three persistent stage launches, dt per instance from the on-device CFL
+reduction); inputs are resident in HBM before the timed region, which is
+bracketed by HIP events on the solver's stream and a device synchronisation on
+both sides.  A cell-update is one cell advanced through one full step.
 
-Workload (FP32-bulk deviation form, MUSCL-minmod, HLL, reflecting walls,
-SSP-RK3): the 8192x8192 gravity problem (rho = p = exp(-(x+y)), 1e-3 Gaussian
-pressure bump).  Beside it, the plain FP64 kernel on the same grid and on
-2048x2048 (the FP64 drop-in), the FP32-vs-FP64 L1 density error, the roofline of
-the stage kernel and the FP64 CPU oracle timed on a bounded sample.
+Workload (`value`): a batch of 512 independent 2D gravity problems, sides drawn
+log-uniform in [64, 1024] (seed 20261202), g ~ U[0.5, 2] (phi = g(x+y),
+rho = p = exp(-g(x+y))), Gaussian pressure bump of amplitude ~ U[1e-4, 1e-2];
+FP32-bulk deviation form, MUSCL-minmod, HLL, reflecting walls, SSP-RK3; launch
+shape chosen by the library's autotuner.  Beside it: the plain FP64 kernel on
+the same batch (speedup and L1 density error of FP32 vs FP64), the 8192x8192
+single problem in both precisions, a pure-equilibrium drift run, the roofline
+of the stage kernel and the FP64 CPU oracle timed on a bounded sample.
 """
 import argparse
 import ctypes
 import json
-import math
 import os
 import sys
 import time
@@ -30,6 +33,7 @@ import fvwb  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s on 1 MI355X: batched FP32-bulk vs FP64 plain at stated L1 err"
+BYTES_PER_CELL_UPDATE = {fvwb.FP32: 128, fvwb.FP64: 256}
 
 
 class HipEvents:
@@ -63,34 +67,64 @@ class HipEvents:
         assert self.hip.hipDeviceSynchronize() == 0
 
 
-def solver_params(n, precision):
-    return fvwb.make_params(n, n, recon=fvwb.RECON_MUSCL_MINMOD, flux=fvwb.FLUX_HLL, bc=fvwb.BC_REFLECT,
-                            integrator=fvwb.SSPRK3, precision=precision, gamma=1.4, rho0=1.0, p0=1.0, g=1.0)
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def time_solver(ev, n, precision, steps, warmup):
-    """Returns (seconds for `steps` steps, solver) with the solver left open."""
-    s = fvwb.Solver2D(solver_params(n, precision))
+def batch_workload(n=512, seed=20261202):
+    rng = np.random.default_rng(seed)
+    sides = np.rint(np.exp(rng.uniform(np.log(64), np.log(1024), n))).astype(int)
+    gs = rng.uniform(0.5, 2.0, n)
+    amps = rng.uniform(1e-4, 1e-2, n)
+    return sides, gs, amps
+
+
+def scheme_params(nx, ny, precision, g=1.0):
+    return fvwb.make_params(nx, ny, recon=fvwb.RECON_MUSCL_MINMOD, flux=fvwb.FLUX_HLL, bc=fvwb.BC_REFLECT,
+                            integrator=fvwb.SSPRK3, precision=precision, gamma=1.4, rho0=1.0, p0=1.0, g=g)
+
+
+def time_batch(ev, sides, gs, amps, precision, steps, warmup, tune=True):
+    b = fvwb.Batch2D([scheme_params(int(s), int(s), precision, float(g)) for s, g in zip(sides, gs)])
+    b.init_gaussian(amps)
+    shape = b.autotune(3, None) if tune else b.get_launch()[:2]
+    if warmup:
+        b.advance(warmup)
+    b.init_gaussian(amps)
+    st = b.stream()
+    e0, e1 = ev.event(), ev.event()
+    ev.sync()
+    ev.record(e0, st)
+    b.advance_async(steps)
+    ev.record(e1, st)
+    ms = ev.elapsed_ms(e0, e1)
+    ev.sync()
+    for k in (0, len(sides) - 1):
+        assert b.time(k)[2] == steps
+    return ms / 1e3, b, shape
+
+
+def time_single(ev, n, precision, steps, warmup):
+    s = fvwb.Solver2D(scheme_params(n, n, precision))
     s.init_gaussian(1e-3, 0.3, 0.3, 100.0)
     if warmup:
         s.advance(warmup)
     s.init_gaussian(1e-3, 0.3, 0.3, 100.0)
-    stream = s.stream()
+    st = s.stream()
     e0, e1 = ev.event(), ev.event()
     ev.sync()
-    ev.record(e0, stream)
+    ev.record(e0, st)
     s.advance_async(steps)
-    ev.record(e1, stream)
+    ev.record(e1, st)
     ms = ev.elapsed_ms(e0, e1)
     ev.sync()
-    t, dt, done = s.time()
-    assert done == steps, (done, steps)
+    assert s.time()[2] == steps
     return ms / 1e3, s
 
 
 def cpu_baseline():
-    """FP64 CPU oracle (plain C port, one core) on a bounded sample: the 2048x2048
-    grid of the FP64 configuration, 2 SSP-RK3 steps."""
+    """FP64 CPU oracle (plain C port, one core) on a bounded sample: one
+    2048x2048 problem of the same scheme, 2 SSP-RK3 steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
@@ -102,56 +136,75 @@ def cpu_baseline():
     po.fv2d_advance(p, u, w, max_steps=steps)
     sec = time.perf_counter() - t0
     return {"value": n * n * steps / sec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle/fvwb_oracle_fv2d.c FP64 MUSCL-minmod+HLL+SSP-RK3, {n}x{n} grid, "
+            "sample": f"oracle/fvwb_oracle_fv2d.c FP64 MUSCL-minmod+HLL+SSP-RK3, one {n}x{n} problem, "
                       f"{steps} steps ({sec:.1f} s), 1 thread"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--size", type=int, default=8192)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-
     ev = HipEvents()
-    n, K, W = args.size, args.steps, args.warmup
-    cells = n * n
+    K, W = args.steps, args.warmup
 
-    sec32, s32 = time_solver(ev, n, fvwb.FP32, K, W)
+    # ---- headline: the 512-instance batch -------------------------------------------
+    sides, gs, amps = batch_workload()
+    cells = int(np.sum(sides.astype(np.int64) ** 2))
+    sec32, b32, shape32 = time_batch(ev, sides, gs, amps, fvwb.FP32, K, W)
     rate32 = cells * K / sec32
-    print(f"[bench] fp32 bulk {n}^2: {K} steps in {sec32 * 1e3:.2f} ms -> {rate32:.4e} cell-updates/s",
-          file=sys.stderr, flush=True)
-    sec64, s64 = time_solver(ev, n, fvwb.FP64, K, W)
+    log(f"batch fp32: {len(sides)} instances, {cells} cells, shape {shape32}, {K} steps in "
+        f"{sec32 * 1e3:.2f} ms -> {rate32:.4e} cell-updates/s")
+    sec64, b64, shape64 = time_batch(ev, sides, gs, amps, fvwb.FP64, K, W)
     rate64 = cells * K / sec64
-    print(f"[bench] fp64 plain {n}^2: {K} steps in {sec64 * 1e3:.2f} ms -> {rate64:.4e} cell-updates/s",
-          file=sys.stderr, flush=True)
-    d32 = s32.download_deviation()[0]
-    d64 = s64.download_deviation()[0]
-    l1 = float(np.abs(d32 - d64).sum()) / cells
-    linf = float(np.abs(d32 - d64).max())
+    log(f"batch fp64: shape {shape64}, {K} steps in {sec64 * 1e3:.2f} ms -> {rate64:.4e} cell-updates/s")
+    l1_max, linf_max = 0.0, 0.0
+    for k, n in enumerate(sides):
+        d32 = b32.download_deviation(k)[0]
+        d64 = b64.download_deviation(k)[0]
+        diff = np.abs(d32 - d64)
+        l1_max = max(l1_max, float(diff.sum()) / (n * n))
+        linf_max = max(linf_max, float(diff.max()))
+    b32.close()
+    b64.close()
+    log(f"fp32 vs fp64 density: max L1 {l1_max:.3e}, max Linf {linf_max:.3e}")
+
+    # ---- the single 8192^2 problem ---------------------------------------------------------
+    n1 = 8192
+    s32sec, s32 = time_single(ev, n1, fvwb.FP32, K, W)
+    s64sec, s64 = time_single(ev, n1, fvwb.FP64, K, W)
+    single32, single64 = n1 * n1 * K / s32sec, n1 * n1 * K / s64sec
+    log(f"single {n1}^2: fp32 {single32:.4e}, fp64 {single64:.4e} cell-updates/s")
     s32.close()
     s64.close()
-    del d32, d64
 
-    n2 = 2048
-    sec2, s2 = time_solver(ev, n2, fvwb.FP64, K, W)
-    rate2 = n2 * n2 * K / sec2
-    s2.close()
-    print(f"[bench] fp64 plain {n2}^2: {rate2:.4e} cell-updates/s", file=sys.stderr, flush=True)
+    # ---- hydrostatic drift: resting atmosphere, FP32 bulk, 8192^2 --------------------------
+    drift = None
+    if args.drift_steps > 0:
+        s = fvwb.Solver2D(scheme_params(n1, n1, fvwb.FP32))
+        s.init_gaussian(0.0, 0.3, 0.3, 100.0)
+        t0 = time.perf_counter()
+        s.advance(args.drift_steps)
+        dev = s.download_deviation()
+        drift = {"steps": args.drift_steps, "size": n1, "max_abs_drho": float(np.abs(dev[0]).max()),
+                 "max_abs_dE": float(np.abs(dev[3]).max()), "seconds": time.perf_counter() - t0}
+        s.close()
+        del dev
+        log(f"drift after {args.drift_steps} steps: {drift}")
 
-    # roofline of the fused stage kernel: algorithmic bytes per cell-update
-    # 8 state copies x 4 variables x 4 B (FP32) over the three SSP-RK3 stages;
-    # a launch moves on average cells*128/3 B in ms_per_step/3.
-    bytes_per_step = cells * 128.0
-    achieved = bytes_per_step / sec32 * K / 1e9
+    # ---- roofline of the fused stage kernel -----------------------------------------------
+    # algorithmic bytes per cell-update: 8 state copies x 4 variables x 4 B (FP32)
+    # over the three SSP-RK3 stages; a launch moves on average cells*128/3 B.
+    achieved = cells * 128.0 * K / sec32 / 1e9
     copy_gbs = fvwb.copy_bandwidth(1 << 30, 10)
-    print(f"[bench] device copy bandwidth {copy_gbs:.1f} GB/s", file=sys.stderr, flush=True)
+    log(f"device copy bandwidth {copy_gbs:.1f} GB/s")
     cpu = None if args.no_cpu else cpu_baseline()
 
     out = {
@@ -167,21 +220,24 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"2D Euler+gravity {n}x{n} FP32-bulk deviation, MUSCL-minmod, HLL, "
-                               "reflecting walls, SSP-RK3, 1e-3 Gaussian pressure bump",
-                   "cells": cells, "parallelism": "single GPU"},
+        "config": {"workload": "batch of 512 independent 2D Euler+gravity problems, sides log-uniform "
+                               "64..1024 (seed 20261202), g~U[0.5,2], bump amplitude~U[1e-4,1e-2]; FP32-bulk "
+                               "deviation, MUSCL-minmod, HLL, reflecting walls, SSP-RK3, autotuned launch",
+                   "instances": len(sides), "cells": cells, "launch_shape": list(shape32),
+                   "parallelism": "single GPU, one persistent launch per RK stage, hipGraph replay"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fv2d_stage_kernel<float,MUSCL,HLL,REFLECT,DEV>",
+                     "kernel": "fv2d_batch_stage_kernel<float,MUSCL,HLL,REFLECT,DEV>",
                      "bytes_per_cell_update": 128, "copy_gbs": copy_gbs,
                      "frac_of_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
-        "fp64_plain": {"value": rate64, "size": n, "bytes_per_cell_update": 256,
-                       "roofline_frac": rate64 * 256 / 8e12,
-                       "value_2048": rate2},
+        "fp64_plain": {"value": rate64, "workload": "same batch", "bytes_per_cell_update": 256,
+                       "roofline_frac": rate64 * 256 / 8e12},
         "speedup_fp32_vs_fp64": rate32 / rate64,
-        "l1_density_err_fp32_vs_fp64": l1,
-        "linf_density_err_fp32_vs_fp64": linf,
+        "l1_density_err_fp32_vs_fp64": l1_max,
+        "linf_density_err_fp32_vs_fp64": linf_max,
+        "single_8192": {"fp32": single32, "fp64": single64, "speedup": single32 / single64},
+        "hydrostatic_drift": drift,
     }
     print(json.dumps(out), flush=True)
 

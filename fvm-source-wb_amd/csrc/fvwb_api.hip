@@ -16,7 +16,8 @@
 
 namespace fvwb {
 hipError_t launch_fv2d_stage_f64(const StageArgs<double>& a, int recon, int flux, int bc, int mode, hipStream_t s);
-hipError_t launch_fv2d_stage_f32(const StageArgs<float>& a, int recon, int flux, int bc, int mode, hipStream_t s);
+hipError_t launch_fv2d_stage_f32(const StageArgs<float>& a, int recon, int flux, int bc, int mode, int cpl,
+                                 hipStream_t s);
 template <typename T>
 hipError_t launch_aos_to_dev(const double* u, const double* w, T* out, int nx, int ny, int pitch, long long plane,
                              double gamma, hipStream_t s);
@@ -55,7 +56,7 @@ struct fvwb_ctx {
   unsigned long long* red = nullptr;
   int parity = 0;
   bool uploaded = false;
-  Tune tune{32};
+  Tune tune{16, 2};
   std::string err;
 };
 
@@ -162,7 +163,10 @@ extern "C" int fvwb_create(fvwb_handle* out, const fvwb_params* p) {
   }
   // launch shape: rows per wave strip so that a 2048^2 grid yields >= 4k waves
   long long cells = (long long)nx * ny;
-  h->tune.rows = cells >= (1ll << 24) ? 32 : (cells >= (1ll << 20) ? 16 : 8);
+  // launch shape: short strips keep >= ~16 waves per CU on mid-size grids;
+  // FP32 lanes own 2 columns (8-byte loads leave registers for a deeper ring)
+  h->tune.rows = cells >= (1ll << 24) ? 16 : (cells >= (1ll << 20) ? 8 : 4);
+  h->tune.cpl = 2;
   *out = h;
   return FVWB_OK;
 }
@@ -192,7 +196,8 @@ static StageArgs<T> base_args(fvwb_ctx* h) {
   a.ctrl = h->ctrl;
   a.plane = h->plane; a.pitch = h->pitch; a.nx = p.nx; a.ny = p.ny;
   a.rows = h->tune.rows;
-  const int cols = 64 - 2 * (p.recon == FVWB_RECON_MUSCL_MINMOD ? 2 : 1);
+  const int cpl = sizeof(T) == 8 ? 2 : h->tune.cpl;
+  const int cols = 62 * cpl;   // columns per wave strip
   a.ntx = (p.nx + cols - 1) / cols;
   a.ntiles = a.ntx * ((p.ny + a.rows - 1) / a.rows);
   const double dx = p.boxlen_x / p.nx, dy = p.boxlen_y / p.ny;
@@ -309,7 +314,7 @@ static int enqueue_steps(fvwb_ctx* h, int nsteps, double t_end) {
     if constexpr (sizeof(T) == 8)
       return launch_fv2d_stage_f64(s, p.recon, p.flux, p.bc, mode, h->stream);
     else
-      return launch_fv2d_stage_f32(s, p.recon, p.flux, p.bc, mode, h->stream);
+      return launch_fv2d_stage_f32(s, p.recon, p.flux, p.bc, mode, h->tune.cpl, h->stream);
   };
   for (int n = 0; n < nsteps; ++n) {
     a.parity = h->parity;
@@ -446,4 +451,22 @@ extern "C" int fvwb_copy_bandwidth(uint64_t bytes, int32_t reps, double* gbps) {
   if (a) (void)hipFree(a);
   if (b) (void)hipFree(b);
   return rc;
+}
+
+extern "C" int fvwb_set_launch(fvwb_handle h, int32_t rows_per_strip, int32_t columns_per_lane) {
+  if (!h) return fail(h, FVWB_ERR_ARG, "NULL handle");
+  if (rows_per_strip < 4 || rows_per_strip > 1024 || rows_per_strip % 4 != 0)
+    return fail(h, FVWB_ERR_ARG, "rows_per_strip must be a multiple of 4 in [4, 1024]");
+  if (columns_per_lane != 2 && columns_per_lane != 4)
+    return fail(h, FVWB_ERR_ARG, "columns_per_lane must be 2 or 4");
+  h->tune.rows = rows_per_strip;
+  if (h->p.precision == FVWB_FP32) h->tune.cpl = columns_per_lane;
+  return FVWB_OK;
+}
+
+extern "C" int fvwb_get_launch(fvwb_handle h, int32_t* rows_per_strip, int32_t* columns_per_lane) {
+  if (!h || !rows_per_strip || !columns_per_lane) return fail(h, FVWB_ERR_ARG, "NULL argument");
+  *rows_per_strip = h->tune.rows;
+  *columns_per_lane = h->p.precision == FVWB_FP32 ? h->tune.cpl : 2;
+  return FVWB_OK;
 }

@@ -32,37 +32,53 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 
 // Row access through a buffer descriptor built on the scalar unit: base = the
-// row's first element (uniform), extent = the row, per-lane 32-bit byte offset.
-// No vector address arithmetic; a lane offset past the row would read 0 / be
-// dropped by the hardware range check.
-template <typename T>
-__device__ __forceinline__ T row_load(const T* row, unsigned boff, int row_bytes) {
-  __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, row_bytes, 0x00020000);
-  if constexpr (sizeof(T) == 4) {
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
-  } else {
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, boff, 0, 0));
-  }
+// row's first element (uniform), extent = the allocated row (pitch), per-lane
+// 32-bit byte offset.  No vector address arithmetic, and the hardware range
+// check bounds every access to the row.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* row, int row_bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, row_bytes, 0x00020000);
 }
 template <typename T>
-__device__ __forceinline__ void row_store(T* row, unsigned boff, int row_bytes, T v) {
-  __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, row_bytes, 0x00020000);
-  if constexpr (sizeof(T) == 4) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, boff, 0, 0);
-  } else {
-    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, boff, 0, 0);
-  }
+__device__ __forceinline__ T ld1(const T* row, unsigned boff, int row_bytes) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(row_rsrc(row, row_bytes), boff, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(row, row_bytes), boff, 0, 0));
 }
 
-template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE>
-__global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
-  constexpr int H = RECON ? 2 : 1;
-  constexpr int COLS = 64 - 2 * H;
-  const int lane = threadIdx.x & 63;
-  // wave index made explicitly uniform so that strip, row and row pointers
-  // live in scalar registers
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+// A lane's block of C consecutive columns of one variable: one 8- or 16-byte load.
+template <typename T, int C>
+struct Lanes { T v[C]; };
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+template <typename T, int C>
+__device__ __forceinline__ Lanes<T, C> ldv(const T* row, unsigned boff, int row_bytes) {
+  static_assert(C * sizeof(T) == 8 || C * sizeof(T) == 16, "block of 8 or 16 bytes");
+  if constexpr (C * sizeof(T) == 16)
+    return __builtin_bit_cast(Lanes<T, C>, __builtin_amdgcn_raw_buffer_load_b128(row_rsrc(row, row_bytes), boff, 0, 0));
+  else
+    return __builtin_bit_cast(Lanes<T, C>, __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(row, row_bytes), boff, 0, 0));
+}
+template <typename T, int C>
+__device__ __forceinline__ void stv(T* row, unsigned boff, int row_bytes, Lanes<T, C> x) {
+  if constexpr (C * sizeof(T) == 16)
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), row_rsrc(row, row_bytes), boff, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), row_rsrc(row, row_bytes), boff, 0, 0);
+}
+
+// Deviation state of a lane's C columns on one row.
+template <typename T, int C>
+struct Blk { Vec4<T> c[C]; };
+
+// One wave computes one strip ("tile") of one instance.  The wave that owns
+// tile 0 of an instance also does that instance's step bookkeeping: the first
+// stage clears the next step's speed slots, the last stage advances t.
+template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL>
+__device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile, const int lane, const int slot) {
+  // each lane owns CPL adjacent columns (8 or 16 bytes per variable plane);
+  // lanes 0 and 63 are halo, so a wave produces 62*CPL columns
+  constexpr int COLS = 62 * CPL;
 
   // ---- step control (dt from the previous step's reduction) -----------------
   double dt = 0.0;
@@ -76,10 +92,10 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
     for (int k = 0; k < 8; ++k) m = c->cmax[p][k] > m ? c->cmax[p][k] : m;
     // dt = 0.5*dx/cmax*cfl (benchmark_2d.f90:242)
     dt = 0.5 * a.h / __longlong_as_double((long long)m) * a.cfl;
-    if (blockIdx.x == 0 && threadIdx.x < 8) {
+    if (tile == 0 && lane < 8) {
       Ctrl* cw = a.ctrl;
-      if (a.first) cw->cmax[q][threadIdx.x] = 0ull;
-      if (a.last && threadIdx.x == 0) {
+      if (a.first) cw->cmax[q][lane] = 0ull;
+      if (a.last && lane == 0) {
         cw->t[q] = active ? t + dt : t;
         if (active) { cw->steps_done += 1; cw->dt_last = dt; }
       }
@@ -87,31 +103,22 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
     if (!active) return;
   }
 
-  const int tile = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
-  if (tile >= a.ntiles) return;
   const int tx = tile % a.ntx, ty = tile / a.ntx;
   const int nx = a.nx, ny = a.ny;
-  const int i = tx * COLS + lane - H;          // this lane's column
+  const int xw = tx * COLS - CPL;              // first column of lane 0 (uniform)
+  const int x0 = xw + CPL * lane;              // first column of this lane's block
   const int y0 = ty * a.rows;
   const int y1 = min(y0 + a.rows, ny);
-  const bool own = lane >= H && lane < 64 - H && i < nx;
-
-  // column with the boundary treatment (mirror for walls, clamp otherwise)
-  int icol;
-  T sx = T(1);
-  if (BC == BC_REFLECT) {
-    icol = i < 0 ? -1 - i : (i >= nx ? 2 * nx - 1 - i : i);
-    if (i < 0 || i >= nx) sx = T(-1);
-    icol = clampi(icol, 0, nx - 1);
-  } else {
-    icol = clampi(i, 0, nx - 1);
-  }
-  const unsigned boff = (unsigned)icol * (unsigned)sizeof(T);
-  const int rowb = nx * (int)sizeof(T);
+  const bool own = lane >= 1 && lane <= 62 && x0 < nx;
+  // interior strips read whole blocks; strips touching a side wall build their
+  // halo column by column (mirror for walls, clamp for the frozen ring)
+  const bool fast = xw >= 0 && xw + 64 * CPL <= nx;
   const long long plane = a.plane;
   const int pitch = a.pitch;
+  const int rowb = pitch * (int)sizeof(T);
+  const unsigned vboff = (unsigned)clampi(x0, 0, pitch - CPL) * (unsigned)sizeof(T);
 
-  auto load_row = [&](const T* base, int j) -> Vec4<T> {
+  auto load_row = [&](const T* base, int j) -> Blk<T, CPL> {
     int jc;
     bool flip = false;
     if (BC == BC_REFLECT) {
@@ -122,19 +129,44 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
       jc = clampi(j, 0, ny - 1);
     }
     const T* r = base + (long long)jc * pitch;
-    Vec4<T> v = {row_load(r, boff, rowb), row_load(r + plane, boff, rowb),
-                 row_load(r + 2 * plane, boff, rowb), row_load(r + 3 * plane, boff, rowb)};
-    if (BC == BC_REFLECT) {
-      v.mx *= sx;
-      if (flip) v.my = -v.my;
+    Blk<T, CPL> b;
+    if (fast) {
+      Lanes<T, CPL> v0 = ldv<T, CPL>(r, vboff, rowb), v1 = ldv<T, CPL>(r + plane, vboff, rowb),
+                    v2 = ldv<T, CPL>(r + 2 * plane, vboff, rowb), v3 = ldv<T, CPL>(r + 3 * plane, vboff, rowb);
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) b.c[k] = Vec4<T>{v0.v[k], v1.v[k], v2.v[k], v3.v[k]};
+    } else {
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        int col = x0 + k, mc;
+        T sxk = T(1);
+        if (BC == BC_REFLECT) {
+          mc = col < 0 ? -1 - col : (col >= nx ? 2 * nx - 1 - col : col);
+          if (col < 0 || col >= nx) sxk = T(-1);
+          mc = clampi(mc, 0, nx - 1);
+        } else {
+          mc = clampi(col, 0, nx - 1);
+        }
+        const unsigned o = (unsigned)mc * (unsigned)sizeof(T);
+        b.c[k] = Vec4<T>{ld1(r, o, rowb), sxk * ld1(r + plane, o, rowb), ld1(r + 2 * plane, o, rowb),
+                         ld1(r + 3 * plane, o, rowb)};
+      }
     }
-    return v;
+    if (BC == BC_REFLECT && flip) {
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) b.c[k].my = -b.c[k].my;
+    }
+    return b;
   };
   // cell-centred rows (clamped into the grid; rows past the strip are masked)
-  auto load_cell = [&](const T* base, int j) -> Vec4<T> {
+  auto load_cell = [&](const T* base, int j) -> Blk<T, CPL> {
     const T* r = base + (long long)min(j, ny - 1) * pitch;
-    return Vec4<T>{row_load(r, boff, rowb), row_load(r + plane, boff, rowb),
-                   row_load(r + 2 * plane, boff, rowb), row_load(r + 3 * plane, boff, rowb)};
+    Lanes<T, CPL> v0 = ldv<T, CPL>(r, vboff, rowb), v1 = ldv<T, CPL>(r + plane, vboff, rowb),
+                  v2 = ldv<T, CPL>(r + 2 * plane, vboff, rowb), v3 = ldv<T, CPL>(r + 3 * plane, vboff, rowb);
+    Blk<T, CPL> b;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) b.c[k] = Vec4<T>{v0.v[k], v1.v[k], v2.v[k], v3.v[k]};
+    return b;
   };
 
   Phys<T> P;
@@ -148,8 +180,12 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
 
   // separable FP64 equilibrium: rho = (rho0 e^{-kx}) e^{-ky} is formed in FP64
   // and rounded once; p, E and E+p scale with rho by constant ratios.
-  const double rxf = a.rxf[clampi(i, 0, nx)];       // left x face of this column
-  const double rxc = a.rxc[clampi(i, 0, nx - 1)];
+  double rxf[CPL], rxc[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    rxf[k] = a.rxf[clampi(x0 + k, 0, nx)];        // left x face of column x0+k
+    rxc[k] = a.rxc[clampi(x0 + k, 0, nx - 1)];
+  }
   const T por = T(a.p_over_rho), eor = T(a.p_over_rho * a.inv_gm1);
   const T epor = T(a.p_over_rho * a.gamma * a.inv_gm1);
   auto face_eq = [&](double rx, double ey) -> FaceEq<T> {
@@ -165,29 +201,35 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
   bool bad = false;
 
   // ---- prologue -----------------------------------------------------------------
-  // A ring of RING rows holds rows j..j+RING-2 while row j is computed and
-  // receives row j+RING-1 (so RING-3 rows are in flight ahead of use); U^n uses
-  // a ring of URING rows the same way.  The body is unrolled RING times with
-  // rotated roles, so the rings never move data between registers.
-  constexpr int RING = sizeof(T) == 4 ? 6 : 4;
-  constexpr int URING = sizeof(T) == 4 ? 3 : 2;
-  Vec4<T> R[RING], Ur[URING], sy, Gp, dfw;
+  // A ring of RING rows holds rows j..j+2 while row j is computed and receives
+  // row j+RING-1 (one row in flight ahead of use); U^n uses a ring of URING.
+  // The body is unrolled RING times with rotated roles, so the rings never move
+  // data.  (A deeper ring measured no faster: the strip sweep is limited by its
+  // memory access pattern, not by loads in flight per wave.)
+  constexpr int RING = 4;
+  constexpr int URING = 2;
+  Blk<T, CPL> R[RING], Ur[URING], sy, Gp;
   {
-    FaceEq<T> qy = face_eq(rxc, a.eyf[clampi(y0, 0, ny)]);
-    Vec4<T> Dm1 = load_row(a.in, y0 - 1);
+    const double eyf0 = a.eyf[clampi(y0, 0, ny)];
+    Blk<T, CPL> Dm1 = load_row(a.in, y0 - 1);
 #pragma unroll
     for (int k = 0; k < RING - 1; ++k) R[k] = load_row(a.in, y0 + k);
-    if (RECON) {
-      Vec4<T> Dm2 = load_row(a.in, y0 - 2);
-      Vec4<T> sym1 = minmod2(Dm1 - Dm2, R[0] - Dm1);
-      sy = minmod2(R[0] - Dm1, R[1] - R[0]);
-      dfw = R[1] - R[0];   // B - A of the first row
-      Gp = face_flux<T, FLUX, 1, FORM>(P, qy, Dm1 + scale(P.half, sym1), R[0] - scale(P.half, sy));
-    } else {
-      Gp = face_flux<T, FLUX, 1, FORM>(P, qy, Dm1, R[0]);
+    Blk<T, CPL> Dm2;
+    if (RECON) Dm2 = load_row(a.in, y0 - 2);
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      FaceEq<T> qy = face_eq(rxc[k], eyf0);
+      if (RECON) {
+        Vec4<T> sym1 = minmod2(Dm1.c[k] - Dm2.c[k], R[0].c[k] - Dm1.c[k]);
+        sy.c[k] = minmod2(R[0].c[k] - Dm1.c[k], R[1].c[k] - R[0].c[k]);
+        Gp.c[k] = face_flux<T, FLUX, 1, FORM>(P, qy, Dm1.c[k] + scale(P.half, sym1),
+                                              R[0].c[k] - scale(P.half, sy.c[k]));
+      } else {
+        Gp.c[k] = face_flux<T, FLUX, 1, FORM>(P, qy, Dm1.c[k], R[0].c[k]);
+      }
     }
 #pragma unroll
-    for (int k = 0; k < URING; ++k) Ur[k] = Vec4<T>{T(0), T(0), T(0), T(0)};
+    for (int k = 0; k < URING; ++k) Ur[k] = Blk<T, CPL>{};
     if (MODE == MODE_STAGE) {
 #pragma unroll
       for (int k = 0; k < URING - 1; ++k) Ur[k] = load_cell(a.un, y0 + k);
@@ -196,65 +238,98 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
 
   // one row: A = row j, B = row j+1, C = row j+2; S receives row j+RING-1; UA
   // holds U^n(j) and US receives U^n(j+URING-1).
-  auto row_step = [&](int j, const Vec4<T>& A, const Vec4<T>& B, const Vec4<T>& C, Vec4<T>& S,
-                      const Vec4<T>& UA, Vec4<T>& US) {
+  auto row_step = [&](int j, const Blk<T, CPL>& A, const Blk<T, CPL>& B, const Blk<T, CPL>& C,
+                      Blk<T, CPL>& S, const Blk<T, CPL>& UA, Blk<T, CPL>& US) {
     S = load_row(a.in, j + RING - 1);
     if (MODE == MODE_STAGE) US = load_cell(a.un, j + URING - 1);
     const int jt = min(j, ny - 1);
     const double eyc = a.eyc[jt];
     const double eyf = a.eyf[jt + 1];
 
-    // y face above row j
-    Vec4<T> Gn, xl, xr;
+    // y faces above row j
+    Blk<T, CPL> Gn;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      FaceEq<T> qy = face_eq(rxc[k], eyf);
+      if (RECON) {
+        Vec4<T> syn = minmod2(B.c[k] - A.c[k], C.c[k] - B.c[k]);
+        Gn.c[k] = face_flux<T, FLUX, 1, FORM>(P, qy, A.c[k] + scale(P.half, sy.c[k]), B.c[k] - scale(P.half, syn));
+        sy.c[k] = syn;
+      } else {
+        Gn.c[k] = face_flux<T, FLUX, 1, FORM>(P, qy, A.c[k], B.c[k]);
+      }
+    }
+    // x faces on the left of each column; the first takes its left state from
+    // the neighbouring lane's last column, the last column's right face is the
+    // right neighbour's first
+    Vec4<T> xr[CPL], xl[CPL];
     if (RECON) {
-      Vec4<T> dnext = C - B;
-      Vec4<T> syn = minmod2(dfw, dnext);   // dfw = B - A, carried from the previous row
-      dfw = dnext;
-      Gn = face_flux<T, FLUX, 1, FORM>(P, face_eq(rxc, eyf), A + scale(P.half, sy), B - scale(P.half, syn));
-      sy = syn;
-      // x slopes on row j from the neighbouring lanes
-      Vec4<T> sxl = minmod2(A - from_left(A), from_right(A) - A);
-      xr = A + scale(P.half, sxl);   // state at this cell's right face
-      xl = A - scale(P.half, sxl);   // state at this cell's left face
+      Vec4<T> lm = from_left(A.c[CPL - 1]), rp = from_right(A.c[0]);
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        Vec4<T> am = k == 0 ? lm : A.c[k - 1];
+        Vec4<T> ap = k == CPL - 1 ? rp : A.c[k + 1];
+        Vec4<T> sx = minmod2(A.c[k] - am, ap - A.c[k]);
+        xr[k] = A.c[k] + scale(P.half, sx);
+        xl[k] = A.c[k] - scale(P.half, sx);
+      }
     } else {
-      Gn = face_flux<T, FLUX, 1, FORM>(P, face_eq(rxc, eyf), A, B);
-      xr = A;
-      xl = A;
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) xr[k] = xl[k] = A.c[k];
     }
-    // x face on this lane's left: left state from the neighbour's right face
-    Vec4<T> Fl = face_flux<T, FLUX, 0, FORM>(P, face_eq(rxf, eyc), from_left(xr), xl);
-    Vec4<T> Fr = from_right(Fl);
+    Vec4<T> F[CPL + 1];
+    const Vec4<T> xr_left = from_left(xr[CPL - 1]);
+#pragma unroll
+    for (int k = 0; k < CPL; ++k)
+      F[k] = face_flux<T, FLUX, 0, FORM>(P, face_eq(rxf[k], eyc), k == 0 ? xr_left : xr[k - 1], xl[k]);
+    F[CPL] = from_right(F[0]);
 
-    // residual: -dF/dx - dG/dy + (s(U) - s(U_eq)), s = (0, -rho g, -rho g, -g (m_x+m_y))
-    Vec4<T> L;
-    L.r = -(Fr.r - Fl.r) * odx - (Gn.r - Gp.r) * ody;
-    L.mx = (-(Fr.mx - Fl.mx) * odx - (Gn.mx - Gp.mx) * ody) - P.g * A.r;
-    L.my = (-(Fr.my - Fl.my) * odx - (Gn.my - Gp.my) * ody) - P.g * A.r;
-    L.e = (-(Fr.e - Fl.e) * odx - (Gn.e - Gp.e) * ody) - P.g * (A.mx + A.my);
+    Blk<T, CPL> O;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const Vec4<T>& D = A.c[k];
+      // residual: -dF/dx - dG/dy + (s(U) - s(U_eq)), s = (0, -rho g, -rho g, -g (m_x+m_y))
+      Vec4<T> L;
+      L.r = -(F[k + 1].r - F[k].r) * odx - (Gn.c[k].r - Gp.c[k].r) * ody;
+      L.mx = (-(F[k + 1].mx - F[k].mx) * odx - (Gn.c[k].mx - Gp.c[k].mx) * ody) - P.g * D.r;
+      L.my = (-(F[k + 1].my - F[k].my) * odx - (Gn.c[k].my - Gp.c[k].my) * ody) - P.g * D.r;
+      L.e = (-(F[k + 1].e - F[k].e) * odx - (Gn.c[k].e - Gp.c[k].e) * ody) - P.g * (D.mx + D.my);
+      if (BC == BC_FROZEN) {
+        const int col = x0 + k;
+        if (col == 0 || col == nx - 1 || j == 0 || j == ny - 1) L = {T(0), T(0), T(0), T(0)};
+      }
+      if (MODE == MODE_RESIDUAL) {
+        O.c[k] = L;
+      } else if (MODE == MODE_STAGE1) {
+        O.c[k] = D + scale(bdt, L);
+      } else {
+        O.c[k] = scale(alpha, UA.c[k]) + (scale(beta, D) + scale(bdt, L));
+      }
+    }
     Gp = Gn;
-    if (BC == BC_FROZEN && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) L = {T(0), T(0), T(0), T(0)};
-
-    Vec4<T> o;
-    if (MODE == MODE_RESIDUAL) {
-      o = L;
-    } else if (MODE == MODE_STAGE1) {
-      o = A + scale(bdt, L);
-    } else {
-      o = scale(alpha, UA) + (scale(beta, A) + scale(bdt, L));
-    }
     if (own && j < y1) {
       T* d = a.out + (long long)j * pitch;
-      row_store(d, boff, rowb, o.r);
-      row_store(d + plane, boff, rowb, o.mx);
-      row_store(d + 2 * plane, boff, rowb, o.my);
-      row_store(d + 3 * plane, boff, rowb, o.e);
+      Lanes<T, CPL> o0, o1, o2, o3;
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        o0.v[k] = O.c[k].r; o1.v[k] = O.c[k].mx; o2.v[k] = O.c[k].my; o3.v[k] = O.c[k].e;
+      }
+      // a block may run past nx into the row padding, which is never read as data
+      stv<T, CPL>(d, vboff, rowb, o0);
+      stv<T, CPL>(d + plane, vboff, rowb, o1);
+      stv<T, CPL>(d + 2 * plane, vboff, rowb, o2);
+      stv<T, CPL>(d + 3 * plane, vboff, rowb, o3);
       if (MODE != MODE_RESIDUAL && a.last) {
-        bool ok;
-        double rc = rxc * eyc;
-        T rct = T(rc);
-        T sp = cell_speed(P, rct, rct * eor, o, ok);
-        smax = fmax(smax, (double)sp);
-        bad |= !ok;
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+          if (x0 + k < nx) {
+            bool ok;
+            T rct = T(rxc[k] * eyc);
+            T sp = cell_speed(P, rct, rct * eor, O.c[k], ok);
+            smax = fmax(smax, (double)sp);
+            bad |= !ok;
+          }
+        }
       }
     }
   };
@@ -270,7 +345,54 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
     if (bad) atomicOr(&a.ctrl->err, 1u);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
-    if (lane == 0) atomicMax(&a.ctrl->cmax[q][blockIdx.x & 7], (unsigned long long)__double_as_longlong(smax));
+    if (lane == 0) atomicMax(&a.ctrl->cmax[q][slot], (unsigned long long)__double_as_longlong(smax));
+  }
+}
+
+template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL>
+__global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
+  const int lane = threadIdx.x & 63;
+  // wave index made explicitly uniform so that strip, row and row pointers
+  // live in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tile = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
+  if (tile >= a.ntiles) return;
+  stage_tile<T, RECON, FLUX, BC, FORM, MODE, CPL>(a, tile, lane, blockIdx.x & 7);
+}
+
+// Persistent batched stage: every wave walks the global tile list of all
+// instances (static stride, each XCD taking contiguous runs) and finds each
+// tile's instance by binary search over the tile prefix sums.
+template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL>
+__global__ __launch_bounds__(256) void fv2d_batch_stage_kernel(BatchArgs<T> b) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int stride = gridDim.x * 4;
+  for (int t = xcd_block(blockIdx.x, gridDim.x) * 4 + wave; t < b.ntiles; t += stride) {
+    int lo = 0, hi = b.ninst - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (b.tile_start[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const InstDesc& d = b.inst[lo];
+    StageArgs<T> a;
+    a.in = b.in + d.off;
+    a.un = b.un ? b.un + d.off : nullptr;
+    a.out = b.out + d.off;
+    a.rxf = b.tabx + d.tx_off;
+    a.rxc = b.tabx + d.tx_off + d.nx + 1;
+    a.eyc = b.taby + d.ty_off;
+    a.eyf = b.taby + d.ty_off + d.ny;
+    a.ctrl = b.ctrl + lo;
+    a.plane = d.plane;
+    a.pitch = d.pitch; a.nx = d.nx; a.ny = d.ny;
+    a.rows = d.rows; a.ntx = d.ntx; a.ntiles = d.ntiles;
+    a.parity = b.parity; a.first = b.first; a.last = b.last;
+    a.t_end = b.t_end; a.h = d.h; a.cfl = d.cfl;
+    a.gamma = d.gamma; a.g = d.g; a.p_over_rho = d.p_over_rho; a.inv_gm1 = d.inv_gm1;
+    a.odx = d.odx; a.ody = d.ody;
+    a.alpha = b.alpha; a.beta = b.beta;
+    stage_tile<T, RECON, FLUX, BC, FORM, MODE, CPL>(a, t - b.tile_start[lo], lane, blockIdx.x & 7);
   }
 }
 
@@ -382,35 +504,76 @@ __global__ void init_gaussian_kernel(T* __restrict__ out, double* __restrict__ w
 
 // ---- host-side launchers ----------------------------------------------------------
 
-template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE>
+template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL>
 static hipError_t launch_stage_t(const StageArgs<T>& a, hipStream_t s) {
   int blocks = (a.ntiles + 3) / 4;
-  hipLaunchKernelGGL((fv2d_stage_kernel<T, RECON, FLUX, BC, FORM, MODE>), dim3(blocks), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((fv2d_stage_kernel<T, RECON, FLUX, BC, FORM, MODE, CPL>), dim3(blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
-template <typename T, int FORM, int MODE>
+template <typename T, int FORM, int MODE, int CPL>
 static hipError_t dispatch_scheme(const StageArgs<T>& a, int recon, int flux, int bc, hipStream_t s) {
 #define FVWB_CASE(R, F, B)                                              \
-  if (recon == R && flux == F && bc == B) return launch_stage_t<T, R, F, B, FORM, MODE>(a, s);
+  if (recon == R && flux == F && bc == B) return launch_stage_t<T, R, F, B, FORM, MODE, CPL>(a, s);
   FVWB_CASE(0, 0, 0) FVWB_CASE(0, 0, 1) FVWB_CASE(0, 1, 0) FVWB_CASE(0, 1, 1)
   FVWB_CASE(1, 0, 0) FVWB_CASE(1, 0, 1) FVWB_CASE(1, 1, 0) FVWB_CASE(1, 1, 1)
 #undef FVWB_CASE
   return hipErrorInvalidValue;
 }
 
-// Stage modes: 0 residual (FP64 plain only), 1 first stage, 2 later stages.
-hipError_t launch_fv2d_stage_f64(const StageArgs<double>& a, int recon, int flux, int bc, int mode,
-                                 hipStream_t s) {
-  if (mode == MODE_RESIDUAL) return dispatch_scheme<double, FORM_PLAIN, MODE_RESIDUAL>(a, recon, flux, bc, s);
-  if (mode == MODE_STAGE1) return dispatch_scheme<double, FORM_PLAIN, MODE_STAGE1>(a, recon, flux, bc, s);
-  return dispatch_scheme<double, FORM_PLAIN, MODE_STAGE>(a, recon, flux, bc, s);
+template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL>
+static hipError_t launch_batch_t(const BatchArgs<T>& b, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL((fv2d_batch_stage_kernel<T, RECON, FLUX, BC, FORM, MODE, CPL>), dim3(blocks), dim3(256), 0, s, b);
+  return hipGetLastError();
 }
 
-hipError_t launch_fv2d_stage_f32(const StageArgs<float>& a, int recon, int flux, int bc, int mode,
+template <typename T, int FORM, int MODE, int CPL>
+static hipError_t dispatch_batch(const BatchArgs<T>& b, int blocks, int recon, int flux, int bc, hipStream_t s) {
+#define FVWB_CASE(R, F, B)                                              \
+  if (recon == R && flux == F && bc == B) return launch_batch_t<T, R, F, B, FORM, MODE, CPL>(b, blocks, s);
+  FVWB_CASE(0, 0, 0) FVWB_CASE(0, 0, 1) FVWB_CASE(0, 1, 0) FVWB_CASE(0, 1, 1)
+  FVWB_CASE(1, 0, 0) FVWB_CASE(1, 0, 1) FVWB_CASE(1, 1, 0) FVWB_CASE(1, 1, 1)
+#undef FVWB_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fv2d_batch_f64(const BatchArgs<double>& b, int blocks, int recon, int flux, int bc, int mode,
                                  hipStream_t s) {
-  if (mode == MODE_STAGE1) return dispatch_scheme<float, FORM_DEV, MODE_STAGE1>(a, recon, flux, bc, s);
-  if (mode == MODE_STAGE) return dispatch_scheme<float, FORM_DEV, MODE_STAGE>(a, recon, flux, bc, s);
+  if (mode == MODE_STAGE1) return dispatch_batch<double, FORM_PLAIN, MODE_STAGE1, 2>(b, blocks, recon, flux, bc, s);
+  if (mode == MODE_STAGE) return dispatch_batch<double, FORM_PLAIN, MODE_STAGE, 2>(b, blocks, recon, flux, bc, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fv2d_batch_f32(const BatchArgs<float>& b, int blocks, int recon, int flux, int bc, int mode,
+                                 int cpl, hipStream_t s) {
+  if (cpl == 4) {
+    if (mode == MODE_STAGE1) return dispatch_batch<float, FORM_DEV, MODE_STAGE1, 4>(b, blocks, recon, flux, bc, s);
+    if (mode == MODE_STAGE) return dispatch_batch<float, FORM_DEV, MODE_STAGE, 4>(b, blocks, recon, flux, bc, s);
+  } else if (cpl == 2) {
+    if (mode == MODE_STAGE1) return dispatch_batch<float, FORM_DEV, MODE_STAGE1, 2>(b, blocks, recon, flux, bc, s);
+    if (mode == MODE_STAGE) return dispatch_batch<float, FORM_DEV, MODE_STAGE, 2>(b, blocks, recon, flux, bc, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// Stage modes: 0 residual (FP64 plain only), 1 first stage, 2 later stages.
+// FP64 lanes own 2 columns (16-byte loads); FP32 lanes own `cpl` = 2 or 4.
+hipError_t launch_fv2d_stage_f64(const StageArgs<double>& a, int recon, int flux, int bc, int mode,
+                                 hipStream_t s) {
+  if (mode == MODE_RESIDUAL) return dispatch_scheme<double, FORM_PLAIN, MODE_RESIDUAL, 2>(a, recon, flux, bc, s);
+  if (mode == MODE_STAGE1) return dispatch_scheme<double, FORM_PLAIN, MODE_STAGE1, 2>(a, recon, flux, bc, s);
+  return dispatch_scheme<double, FORM_PLAIN, MODE_STAGE, 2>(a, recon, flux, bc, s);
+}
+
+hipError_t launch_fv2d_stage_f32(const StageArgs<float>& a, int recon, int flux, int bc, int mode, int cpl,
+                                 hipStream_t s) {
+  if (cpl == 4) {
+    if (mode == MODE_STAGE1) return dispatch_scheme<float, FORM_DEV, MODE_STAGE1, 4>(a, recon, flux, bc, s);
+    if (mode == MODE_STAGE) return dispatch_scheme<float, FORM_DEV, MODE_STAGE, 4>(a, recon, flux, bc, s);
+  } else if (cpl == 2) {
+    if (mode == MODE_STAGE1) return dispatch_scheme<float, FORM_DEV, MODE_STAGE1, 2>(a, recon, flux, bc, s);
+    if (mode == MODE_STAGE) return dispatch_scheme<float, FORM_DEV, MODE_STAGE, 2>(a, recon, flux, bc, s);
+  }
   return hipErrorInvalidValue;
 }
 

@@ -38,9 +38,36 @@ struct StageArgs {
   double alpha, beta;
 };
 
+// Per-instance description of a batch member (device array).
+struct InstDesc {
+  long long off;      // element offset of the instance's 4-plane block in a pool buffer
+  long long plane;    // elements per plane (pitch * ny)
+  int pitch, nx, ny;
+  int rows, ntx, ntiles;
+  int tx_off, ty_off; // offsets of the x tables (rxf[nx+1], rxc[nx]) and y tables (eyc[ny], eyf[ny+1])
+  double h, cfl, gamma, g, p_over_rho, inv_gm1, odx, ody;
+};
+
+// Kernel arguments of one batched stage.
+template <typename T>
+struct BatchArgs {
+  const InstDesc* inst;
+  const int* tile_start;  // prefix sums of tiles per instance, [ninst]
+  int ninst, ntiles;
+  const T* in;
+  const T* un;
+  T* out;
+  const double* tabx;
+  const double* taby;
+  Ctrl* ctrl;              // one per instance
+  int parity, first, last;
+  double t_end, alpha, beta;
+};
+
 // Launch configuration chosen by the tuner.
 struct Tune {
   int rows;  // rows per wave strip
+  int cpl;   // FP32 columns per lane (2 or 4)
 };
 
 }  // namespace fvwb

@@ -96,6 +96,23 @@ def lib():
         "fvwb_get_time": [H, dp, dp, ip],
         "fvwb_get_stream": [H, ctypes.POINTER(ctypes.c_void_p)],
         "fvwb_copy_bandwidth": [ctypes.c_uint64, ctypes.c_int32, dp],
+        "fvwb_set_launch": [H, ctypes.c_int32, ctypes.c_int32],
+        "fvwb_batch_create": [ctypes.POINTER(H), ctypes.c_int32, pp],
+        "fvwb_batch_destroy": [H],
+        "fvwb_batch_init_gaussian": [H, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+        "fvwb_batch_upload": [H, ctypes.c_int32, dp, dp],
+        "fvwb_batch_reset_time": [H],
+        "fvwb_batch_advance": [H, ctypes.c_int32, ctypes.c_double],
+        "fvwb_batch_advance_async": [H, ctypes.c_int32, ctypes.c_double],
+        "fvwb_batch_download": [H, ctypes.c_int32, dp],
+        "fvwb_batch_download_deviation": [H, ctypes.c_int32, dp],
+        "fvwb_batch_get_time": [H, ctypes.c_int32, dp, dp, ip],
+        "fvwb_batch_get_stream": [H, ctypes.POINTER(ctypes.c_void_p)],
+        "fvwb_batch_use_graph": [H, ctypes.c_int32],
+        "fvwb_batch_set_launch": [H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
+        "fvwb_batch_get_launch": [H, ip, ip, ip, ip],
+        "fvwb_batch_autotune": [H, ctypes.c_int32, ctypes.c_char_p, ip, ip],
+        "fvwb_get_launch": [H, ip, ip],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -103,6 +120,8 @@ def lib():
         f.restype = ctypes.c_int
     L.fvwb_last_error.argtypes = [H]
     L.fvwb_last_error.restype = ctypes.c_char_p
+    L.fvwb_batch_last_error.argtypes = [H]
+    L.fvwb_batch_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -207,10 +226,116 @@ class Solver2D:
         _check(lib().fvwb_download_deviation(self._h, _dp(out)), self._h)
         return out
 
+    def set_launch(self, rows_per_strip, columns_per_lane=4):
+        _check(lib().fvwb_set_launch(self._h, rows_per_strip, columns_per_lane), self._h)
+
+    def get_launch(self):
+        r, c = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().fvwb_get_launch(self._h, ctypes.byref(r), ctypes.byref(c)), self._h)
+        return r.value, c.value
+
     def stream(self):
         s = ctypes.c_void_p()
         _check(lib().fvwb_get_stream(self._h, ctypes.byref(s)), self._h)
         return s.value
+
+
+def _bcheck(rc, h=None):
+    if rc != OK:
+        msg = lib().fvwb_batch_last_error(h)
+        raise FvwbError(rc, msg.decode() if msg else "")
+
+
+class Batch2D:
+    """Many independent 2D instances advanced together (one persistent launch
+    per stage); each instance has its own size, gravity, dt and clock."""
+
+    def __init__(self, params_list):
+        self.params = list(params_list)
+        self.n = len(self.params)
+        arr = (Params * self.n)(*self.params)
+        h = ctypes.c_void_p()
+        _bcheck(lib().fvwb_batch_create(ctypes.byref(h), self.n, arr))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fvwb_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def shape(self, b):
+        if not 0 <= b < self.n:
+            raise FvwbError(ERR_ARG, f"instance index {b} out of range [0, {self.n})")
+        return (4, self.params[b].nx, self.params[b].ny)
+
+    def init_gaussian(self, amps, x0=0.3, y0=0.3, width=100.0):
+        a = np.ascontiguousarray(amps, dtype=np.float64)
+        assert a.shape == (self.n,)
+        _bcheck(lib().fvwb_batch_init_gaussian(self._h, _dp(a), x0, y0, width), self._h)
+
+    def upload(self, b, u, w_eq):
+        u, w_eq = _fortran(u, self.shape(b)), _fortran(w_eq, self.shape(b))
+        _bcheck(lib().fvwb_batch_upload(self._h, b, _dp(u), _dp(w_eq)), self._h)
+
+    def reset_time(self):
+        _bcheck(lib().fvwb_batch_reset_time(self._h), self._h)
+
+    def advance(self, nsteps, t_end=math.inf):
+        _bcheck(lib().fvwb_batch_advance(self._h, nsteps, t_end), self._h)
+
+    def advance_async(self, nsteps, t_end=math.inf):
+        _bcheck(lib().fvwb_batch_advance_async(self._h, nsteps, t_end), self._h)
+
+    def download(self, b):
+        out = np.zeros(self.shape(b), order="F")
+        _bcheck(lib().fvwb_batch_download(self._h, b, _dp(out)), self._h)
+        return out
+
+    def download_deviation(self, b):
+        out = np.zeros(self.shape(b), order="F")
+        _bcheck(lib().fvwb_batch_download_deviation(self._h, b, _dp(out)), self._h)
+        return out
+
+    def time(self, b):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int32()
+        _bcheck(lib().fvwb_batch_get_time(self._h, b, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), self._h)
+        return t.value, dt.value, n.value
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        _bcheck(lib().fvwb_batch_get_stream(self._h, ctypes.byref(s)), self._h)
+        return s.value
+
+    def use_graph(self, enable=True):
+        _bcheck(lib().fvwb_batch_use_graph(self._h, 1 if enable else 0), self._h)
+
+    def set_launch(self, rows, cpl=2, blocks=0):
+        _bcheck(lib().fvwb_batch_set_launch(self._h, rows, cpl, blocks), self._h)
+
+    def get_launch(self):
+        r, c, b, t = (ctypes.c_int32() for _ in range(4))
+        _bcheck(lib().fvwb_batch_get_launch(self._h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(b),
+                                            ctypes.byref(t)), self._h)
+        return r.value, c.value, b.value, t.value
+
+    def autotune(self, trial_steps=3, cache_path=None):
+        r, c = ctypes.c_int32(), ctypes.c_int32()
+        path = cache_path.encode() if cache_path else None
+        _bcheck(lib().fvwb_batch_autotune(self._h, trial_steps, path, ctypes.byref(r), ctypes.byref(c)), self._h)
+        return r.value, c.value
 
 
 def copy_bandwidth(nbytes=1 << 30, reps=10):

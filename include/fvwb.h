@@ -129,12 +129,59 @@ int fvwb_get_time(fvwb_handle h, double* t, double* dt_last, int32_t* steps_done
 /* The handle's stream (a hipStream_t), for event timing by the caller. */
 int fvwb_get_stream(fvwb_handle h, void** stream);
 
+/* ---- launch shape (no counterpart in the reference) -------------------------
+ * A wave owns a strip of 62*columns_per_lane columns and rows_per_strip rows.
+ * columns_per_lane: 2 or 4 for FP32 (8- or 16-byte loads per lane and variable),
+ * always 2 for FP64.  rows_per_strip: multiple of 4 in [4, 1024].  Results do not
+ * depend on the shape; speed does. */
+int fvwb_set_launch(fvwb_handle h, int32_t rows_per_strip, int32_t columns_per_lane);
+int fvwb_get_launch(fvwb_handle h, int32_t* rows_per_strip, int32_t* columns_per_lane);
+
 /* ---- measurement helper (no counterpart in the reference) ------------------
  * Device-to-device streaming copy of `bytes` (rounded down to 16), repeated
  * `reps` times on the null stream; *gbps = 2*bytes*reps / elapsed.  The
  * calibration point for achieved bandwidth and HBM counters.  Allocates and
  * frees its own two buffers. */
 int fvwb_copy_bandwidth(uint64_t bytes, int32_t reps, double* gbps);
+
+
+/* ---- batched instances (config "batch of independent problems") -------------
+ * Many independent 2D problems of different sizes advance together: each has
+ * its own nx, ny, box, rho0, p0, g, its own CFL time step and clock; all share
+ * bc, recon, flux, integrator, precision, gamma and cfl (checked at create).
+ * One persistent launch per Runge-Kutta stage covers every strip of every
+ * instance; step pairs are replayed from a HIP graph.  No counterpart in the
+ * reference (it runs one problem per executable); each instance computes what
+ * fvwb_create/fvwb_advance would compute for it alone. */
+typedef struct fvwb_batch* fvwb_batch_handle;
+int fvwb_batch_create(fvwb_batch_handle* h, int32_t ninst, const fvwb_params* per_inst);
+int fvwb_batch_destroy(fvwb_batch_handle h);
+const char* fvwb_batch_last_error(fvwb_batch_handle h);
+/* Device initial condition for every instance: closed-form equilibrium plus a
+ * Gaussian pressure bump of amplitude amp[b] (as fvwb_init_gaussian); resets
+ * every clock. */
+int fvwb_batch_init_gaussian(fvwb_batch_handle h, const double* amp, double x0, double y0, double width);
+/* Host state of one instance (as fvwb_upload); clocks restart at the next
+ * fvwb_batch_reset_time. */
+int fvwb_batch_upload(fvwb_batch_handle h, int32_t inst, const double* u, const double* w_eq);
+int fvwb_batch_reset_time(fvwb_batch_handle h);
+int fvwb_batch_advance(fvwb_batch_handle h, int32_t nsteps, double t_end);
+int fvwb_batch_advance_async(fvwb_batch_handle h, int32_t nsteps, double t_end);
+int fvwb_batch_download(fvwb_batch_handle h, int32_t inst, double* u);
+int fvwb_batch_download_deviation(fvwb_batch_handle h, int32_t inst, double* du);
+int fvwb_batch_get_time(fvwb_batch_handle h, int32_t inst, double* t, double* dt_last, int32_t* steps_done);
+int fvwb_batch_get_stream(fvwb_batch_handle h, void** stream);
+/* Replay step pairs from a HIP graph (default on). */
+int fvwb_batch_use_graph(fvwb_batch_handle h, int32_t enable);
+/* Launch shape: rows per strip (multiple of 4), FP32 columns per lane (2/4),
+ * persistent blocks (0 = one per four strips). */
+int fvwb_batch_set_launch(fvwb_batch_handle h, int32_t rows, int32_t cpl, int32_t blocks);
+int fvwb_batch_get_launch(fvwb_batch_handle h, int32_t* rows, int32_t* cpl, int32_t* blocks, int32_t* ntiles);
+/* Time `trial_steps` steps for each candidate shape on a saved copy of the
+ * state (restored afterwards), keep the fastest and append it to the text file
+ * cache_path (NULL: no cache); a cached entry for the same batch is reused. */
+int fvwb_batch_autotune(fvwb_batch_handle h, int32_t trial_steps, const char* cache_path, int32_t* rows,
+                        int32_t* cpl);
 
 #ifdef __cplusplus
 }
