@@ -84,10 +84,13 @@ def scheme_params(nx, ny, precision, g=1.0):
                             integrator=fvwb.SSPRK3, precision=precision, gamma=1.4, rho0=1.0, p0=1.0, g=g)
 
 
-def time_batch(ev, sides, gs, amps, precision, steps, warmup, tune=True):
+def time_batch(ev, sides, gs, amps, precision, steps, warmup, shape=None):
     b = fvwb.Batch2D([scheme_params(int(s), int(s), precision, float(g)) for s, g in zip(sides, gs)])
     b.init_gaussian(amps)
-    shape = b.autotune(3, None) if tune else b.get_launch()[:2]
+    if shape is None:
+        shape = b.autotune(3, None)
+    else:
+        b.set_launch(shape[0], shape[1], 0)
     if warmup:
         b.advance(warmup)
     b.init_gaussian(amps)
@@ -122,6 +125,21 @@ def time_single(ev, n, precision, steps, warmup):
     return ms / 1e3, s
 
 
+def measured_traffic(cells, shape):
+    """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (scaled by the copy-kernel calibration), valid for the
+    workload and launch shape they were taken on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "fv2d_batch512_fp32_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except OSError:
+        return None, "no profile"
+    if d.get("cells") != cells or tuple(d.get("launch_shape", ())) != tuple(shape):
+        return None, f"profile taken on shape {d.get('launch_shape')}, not {list(shape)}"
+    return d["hbm_bytes_per_launch"], "profiles/fv2d_batch512_fp32_traffic.json"
+
+
 def cpu_baseline():
     """FP64 CPU oracle (plain C port, one core) on a bounded sample: one
     2048x2048 problem of the same scheme, 2 SSP-RK3 steps."""
@@ -147,6 +165,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=10000)
+    ap.add_argument("--shape", default=None,
+                    help="ROWS,CPL: fixed launch shape for the batch instead of the autotuner")
+    ap.add_argument("--skip-single", action="store_true", help="skip the 8192^2 single-problem runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -158,11 +179,12 @@ def main():
     # ---- headline: the 512-instance batch -------------------------------------------
     sides, gs, amps = batch_workload()
     cells = int(np.sum(sides.astype(np.int64) ** 2))
-    sec32, b32, shape32 = time_batch(ev, sides, gs, amps, fvwb.FP32, K, W)
+    fixed = tuple(int(v) for v in args.shape.split(",")) if args.shape else None
+    sec32, b32, shape32 = time_batch(ev, sides, gs, amps, fvwb.FP32, K, W, fixed)
     rate32 = cells * K / sec32
     log(f"batch fp32: {len(sides)} instances, {cells} cells, shape {shape32}, {K} steps in "
         f"{sec32 * 1e3:.2f} ms -> {rate32:.4e} cell-updates/s")
-    sec64, b64, shape64 = time_batch(ev, sides, gs, amps, fvwb.FP64, K, W)
+    sec64, b64, shape64 = time_batch(ev, sides, gs, amps, fvwb.FP64, K, W, fixed)
     rate64 = cells * K / sec64
     log(f"batch fp64: shape {shape64}, {K} steps in {sec64 * 1e3:.2f} ms -> {rate64:.4e} cell-updates/s")
     l1_max, linf_max = 0.0, 0.0
@@ -178,12 +200,15 @@ def main():
 
     # ---- the single 8192^2 problem ---------------------------------------------------------
     n1 = 8192
-    s32sec, s32 = time_single(ev, n1, fvwb.FP32, K, W)
-    s64sec, s64 = time_single(ev, n1, fvwb.FP64, K, W)
-    single32, single64 = n1 * n1 * K / s32sec, n1 * n1 * K / s64sec
-    log(f"single {n1}^2: fp32 {single32:.4e}, fp64 {single64:.4e} cell-updates/s")
-    s32.close()
-    s64.close()
+    single = None
+    if not args.skip_single:
+        s32sec, s32 = time_single(ev, n1, fvwb.FP32, K, W)
+        s64sec, s64 = time_single(ev, n1, fvwb.FP64, K, W)
+        single32, single64 = n1 * n1 * K / s32sec, n1 * n1 * K / s64sec
+        single = {"fp32": single32, "fp64": single64, "speedup": single32 / single64}
+        log(f"single {n1}^2: fp32 {single32:.4e}, fp64 {single64:.4e} cell-updates/s")
+        s32.close()
+        s64.close()
 
     # ---- hydrostatic drift: resting atmosphere, FP32 bulk, 8192^2 --------------------------
     drift = None
@@ -202,9 +227,13 @@ def main():
     # ---- roofline of the fused stage kernel -----------------------------------------------
     # algorithmic bytes per cell-update: 8 state copies x 4 variables x 4 B (FP32)
     # over the three SSP-RK3 stages; a launch moves on average cells*128/3 B.
-    achieved = cells * 128.0 * K / sec32 / 1e9
+    # The timed region holds exactly 3K stage launches back to back, so the
+    # average launch duration is sec32 / (3K).
+    launch_us = sec32 / (3 * K) * 1e6
+    achieved = cells * 128.0 / 3 / (launch_us * 1e-6) / 1e9
     copy_gbs = fvwb.copy_bandwidth(1 << 30, 10)
     log(f"device copy bandwidth {copy_gbs:.1f} GB/s")
+    traffic, traffic_note = measured_traffic(cells, shape32)
     cpu = None if args.no_cpu else cpu_baseline()
 
     out = {
@@ -226,8 +255,9 @@ def main():
                    "instances": len(sides), "cells": cells, "launch_shape": list(shape32),
                    "parallelism": "single GPU, one persistent launch per RK stage, hipGraph replay"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "kernel": "fv2d_batch_stage_kernel<float,MUSCL,HLL,REFLECT,DEV>",
+                     "avg_launch_us": launch_us, "algorithmic_bytes_per_launch": cells * 128.0 / 3,
                      "bytes_per_cell_update": 128, "copy_gbs": copy_gbs,
                      "frac_of_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
@@ -236,7 +266,7 @@ def main():
         "speedup_fp32_vs_fp64": rate32 / rate64,
         "l1_density_err_fp32_vs_fp64": l1_max,
         "linf_density_err_fp32_vs_fp64": linf_max,
-        "single_8192": {"fp32": single32, "fp64": single64, "speedup": single32 / single64},
+        "single_8192": single,
         "hydrostatic_drift": drift,
     }
     print(json.dumps(out), flush=True)
