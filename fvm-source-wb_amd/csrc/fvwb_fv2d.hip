@@ -642,27 +642,21 @@ template hipError_t launch_init_gaussian<float>(float*, double*, int, int, int, 
 
 namespace fvwb {
 
-// Plain streaming copy (16 B per lane per iteration): the calibration point for
+// Plain streaming copy (16 B per lane): the calibration point for
 // the achieved-bandwidth and HBM-counter figures of the solver kernels.
 __global__ __launch_bounds__(256) void copy_probe_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
                                                          long long n4) {
-  // each thread moves 4 float4 per iteration, all four loads issued before the
-  // stores, so a wave keeps 4 KiB of reads in flight
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; k + 3 * stride < n4; k += 4 * stride) {
-    float4 a = src[k], b = src[k + stride], c = src[k + 2 * stride], d = src[k + 3 * stride];
-    dst[k] = a;
-    dst[k + stride] = b;
-    dst[k + 2 * stride] = c;
-    dst[k + 3 * stride] = d;
-  }
-  for (; k < n4; k += stride) dst[k] = src[k];
+  // one float4 per thread over a grid that covers the buffer: the fastest plain
+  // device copy pattern on MI355X (measured against grid-stride variants)
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n4) dst[k] = src[k];
 }
 
 hipError_t launch_copy_probe(const void* src, void* dst, long long bytes, hipStream_t s) {
-  long long n4 = bytes / 16;
-  hipLaunchKernelGGL(copy_probe_kernel, dim3(256 * 16), dim3(256), 0, s, (const float4*)src, (float4*)dst, n4);
+  const long long n4 = bytes / 16;
+  const long long blocks = (n4 + 255) / 256;
+  if (blocks < 1 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(copy_probe_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const float4*)src, (float4*)dst, n4);
   return hipGetLastError();
 }
 

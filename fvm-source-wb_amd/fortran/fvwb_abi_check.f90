@@ -1,0 +1,21 @@
+! fvwb_abi_check — checks the Fortran view of the C ABI without touching a GPU:
+! the size of the bind(C) parameter struct, the reference set-up values filled
+! by fvwb_params_reference_2d, and that fvwb_create rejects an invalid set-up
+! with FVWB_ERR_ARG and a message.  Prints one line per field.
+program fvwb_abi_check
+  use iso_c_binding
+  use fvwb_iface
+  implicit none
+  type(fvwb_params) :: p
+  type(c_ptr) :: h
+  integer(c_int) :: rc
+  write(*, '(a,i0)') 'sizeof ', c_sizeof(p)
+  rc = fvwb_params_reference_2d(p, 300_c_int32_t, 200_c_int32_t, 2_c_int32_t)
+  write(*, '(a,i0)') 'status ', rc
+  write(*, '(a,8(1x,i0))') 'ints', p%nvar, p%nx, p%ny, p%bc, p%recon, p%flux, p%integrator, p%precision
+  write(*, '(a,7(1x,z16.16))') 'doubles', p%gamma, p%boxlen_x, p%boxlen_y, p%cfl, p%rho0, p%p0, p%g
+  p%nx = 2
+  rc = fvwb_create(h, p)
+  write(*, '(a,i0)') 'create_bad ', rc
+  write(*, '(a,a)') 'message ', fvwb_message(fvwb_last_error(c_null_ptr))
+end program fvwb_abi_check

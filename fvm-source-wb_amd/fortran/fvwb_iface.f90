@@ -1,0 +1,237 @@
+! fvwb_iface — Fortran 2008 iso_c_binding interfaces to libfvwb (include/fvwb.h).
+!
+! Every procedure below binds one extern "C" entry point of the C ABI with the
+! same name and argument order; arrays are passed as assumed-size real(c_double)
+! in the Fortran layout u(1:nvar,1:nx,1:ny), which is exactly the C layout the
+! library documents (variable fastest, then x, then y).  Status codes are the
+! FVWB_* constants; fvwb_check turns a nonzero status into `error stop` with the
+! library's message, so a caller written like the reference (no status
+! handling) still stops loudly on a device error.
+module fvwb_iface
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: FVWB_OK = 0, FVWB_ERR_ARG = -1, FVWB_ERR_HIP = -2, &
+       FVWB_ERR_OOM = -3, FVWB_ERR_NONPHYSICAL = -4, FVWB_ERR_STATE = -5
+  integer(c_int32_t), parameter, public :: FVWB_BC_FROZEN_RING = 0, FVWB_BC_REFLECT = 1
+  integer(c_int32_t), parameter, public :: FVWB_RECON_FIRST_ORDER = 0, FVWB_RECON_MUSCL_MINMOD = 1
+  integer(c_int32_t), parameter, public :: FVWB_FLUX_RUSANOV = 0, FVWB_FLUX_HLL = 1
+  integer(c_int32_t), parameter, public :: FVWB_RK2_HEUN = 0, FVWB_SSPRK3 = 1
+  integer(c_int32_t), parameter, public :: FVWB_FP64 = 0, FVWB_FP32 = 1
+
+  ! struct fvwb_params: eight int32 then seven doubles (no padding: 32 + 56 bytes)
+  type, bind(C), public :: fvwb_params
+     integer(c_int32_t) :: nvar, nx, ny, bc, recon, flux, integrator, precision
+     real(c_double) :: gamma, boxlen_x, boxlen_y, cfl, rho0, p0, g
+  end type fvwb_params
+
+  public :: fvwb_params_reference_2d, fvwb_create, fvwb_destroy, fvwb_last_error
+  public :: fvwb_compute_update_exact, fvwb_compute_max_speed
+  public :: fvwb_upload, fvwb_init_gaussian, fvwb_advance, fvwb_download, fvwb_download_deviation
+  public :: fvwb_get_time
+  public :: fvwb_batch_create, fvwb_batch_destroy, fvwb_batch_last_error, fvwb_batch_init_gaussian
+  public :: fvwb_batch_upload, fvwb_batch_reset_time, fvwb_batch_advance, fvwb_batch_download
+  public :: fvwb_batch_download_deviation, fvwb_batch_get_time
+  public :: fvwb_check, fvwb_batch_check, fvwb_message
+
+  interface
+     integer(c_int) function fvwb_params_reference_2d(p, nx, ny, nequilibrium) &
+          bind(C, name='fvwb_params_reference_2d')
+       import :: c_int, c_int32_t, fvwb_params
+       type(fvwb_params), intent(out) :: p
+       integer(c_int32_t), value :: nx, ny, nequilibrium
+     end function fvwb_params_reference_2d
+
+     integer(c_int) function fvwb_create(h, p) bind(C, name='fvwb_create')
+       import :: c_int, c_ptr, fvwb_params
+       type(c_ptr), intent(out) :: h
+       type(fvwb_params), intent(in) :: p
+     end function fvwb_create
+
+     integer(c_int) function fvwb_destroy(h) bind(C, name='fvwb_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_destroy
+
+     type(c_ptr) function fvwb_last_error(h) bind(C, name='fvwb_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_last_error
+
+     ! drop-in for benchmark_2d.f90:465-618 (compute_update_exact(u,w_eq,dudt))
+     integer(c_int) function fvwb_compute_update_exact(h, u, w_eq, dudt) &
+          bind(C, name='fvwb_compute_update_exact')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*), w_eq(*)
+       real(c_double), intent(out) :: dudt(*)
+     end function fvwb_compute_update_exact
+
+     ! drop-in for benchmark_2d.f90:264-279 (compute_max_speed(u,cmax))
+     integer(c_int) function fvwb_compute_max_speed(h, u, cmax) bind(C, name='fvwb_compute_max_speed')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*)
+       real(c_double), intent(out) :: cmax
+     end function fvwb_compute_max_speed
+
+     integer(c_int) function fvwb_upload(h, u, w_eq) bind(C, name='fvwb_upload')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*), w_eq(*)
+     end function fvwb_upload
+
+     integer(c_int) function fvwb_init_gaussian(h, amp, x0, y0, width) bind(C, name='fvwb_init_gaussian')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: amp, x0, y0, width
+     end function fvwb_init_gaussian
+
+     integer(c_int) function fvwb_advance(h, nsteps, t_end, t_out, steps_done) bind(C, name='fvwb_advance')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nsteps
+       real(c_double), value :: t_end
+       real(c_double), intent(out) :: t_out
+       integer(c_int32_t), intent(out) :: steps_done
+     end function fvwb_advance
+
+     integer(c_int) function fvwb_download(h, u) bind(C, name='fvwb_download')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: u(*)
+     end function fvwb_download
+
+     integer(c_int) function fvwb_download_deviation(h, du) bind(C, name='fvwb_download_deviation')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: du(*)
+     end function fvwb_download_deviation
+
+     integer(c_int) function fvwb_get_time(h, t, dt_last, steps_done) bind(C, name='fvwb_get_time')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: t, dt_last
+       integer(c_int32_t), intent(out) :: steps_done
+     end function fvwb_get_time
+
+     integer(c_int) function fvwb_batch_create(h, ninst, per_inst) bind(C, name='fvwb_batch_create')
+       import :: c_int, c_int32_t, c_ptr, fvwb_params
+       type(c_ptr), intent(out) :: h
+       integer(c_int32_t), value :: ninst
+       type(fvwb_params), intent(in) :: per_inst(*)
+     end function fvwb_batch_create
+
+     integer(c_int) function fvwb_batch_destroy(h) bind(C, name='fvwb_batch_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_batch_destroy
+
+     type(c_ptr) function fvwb_batch_last_error(h) bind(C, name='fvwb_batch_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_batch_last_error
+
+     integer(c_int) function fvwb_batch_init_gaussian(h, amp, x0, y0, width) &
+          bind(C, name='fvwb_batch_init_gaussian')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: amp(*)
+       real(c_double), value :: x0, y0, width
+     end function fvwb_batch_init_gaussian
+
+     integer(c_int) function fvwb_batch_upload(h, inst, u, w_eq) bind(C, name='fvwb_batch_upload')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: inst
+       real(c_double), intent(in) :: u(*), w_eq(*)
+     end function fvwb_batch_upload
+
+     integer(c_int) function fvwb_batch_reset_time(h) bind(C, name='fvwb_batch_reset_time')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_batch_reset_time
+
+     integer(c_int) function fvwb_batch_advance(h, nsteps, t_end) bind(C, name='fvwb_batch_advance')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nsteps
+       real(c_double), value :: t_end
+     end function fvwb_batch_advance
+
+     integer(c_int) function fvwb_batch_download(h, inst, u) bind(C, name='fvwb_batch_download')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: inst
+       real(c_double), intent(out) :: u(*)
+     end function fvwb_batch_download
+
+     integer(c_int) function fvwb_batch_download_deviation(h, inst, du) &
+          bind(C, name='fvwb_batch_download_deviation')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: inst
+       real(c_double), intent(out) :: du(*)
+     end function fvwb_batch_download_deviation
+
+     integer(c_int) function fvwb_batch_get_time(h, inst, t, dt_last, steps_done) &
+          bind(C, name='fvwb_batch_get_time')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: inst
+       real(c_double), intent(out) :: t, dt_last
+       integer(c_int32_t), intent(out) :: steps_done
+     end function fvwb_batch_get_time
+
+     integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
+       import :: c_size_t, c_ptr
+       type(c_ptr), value :: s
+     end function c_strlen
+  end interface
+
+contains
+
+  ! Copy a NUL-terminated C string into a Fortran string.
+  function fvwb_message(cstr) result(msg)
+    type(c_ptr), intent(in) :: cstr
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, i
+    if (.not. c_associated(cstr)) then
+       msg = '(no message)'
+       return
+    end if
+    n = int(c_strlen(cstr))
+    call c_f_pointer(cstr, chars, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+       msg(i:i) = chars(i)
+    end do
+  end function fvwb_message
+
+  ! Stop with the library's message when a call on a single-problem handle failed.
+  subroutine fvwb_check(status, h, what)
+    integer(c_int), intent(in) :: status
+    type(c_ptr), intent(in) :: h
+    character(len=*), intent(in) :: what
+    if (status /= FVWB_OK) then
+       write(0, '(a,a,a,i0,a,a)') 'fvwb: ', what, ' failed with status ', status, ': ', &
+            fvwb_message(fvwb_last_error(h))
+       error stop 1
+    end if
+  end subroutine fvwb_check
+
+  ! Same for a batch handle.
+  subroutine fvwb_batch_check(status, h, what)
+    integer(c_int), intent(in) :: status
+    type(c_ptr), intent(in) :: h
+    character(len=*), intent(in) :: what
+    if (status /= FVWB_OK) then
+       write(0, '(a,a,a,i0,a,a)') 'fvwb: ', what, ' failed with status ', status, ': ', &
+            fvwb_message(fvwb_batch_last_error(h))
+       error stop 1
+    end if
+  end subroutine fvwb_batch_check
+
+end module fvwb_iface
