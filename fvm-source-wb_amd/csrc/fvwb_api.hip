@@ -52,6 +52,7 @@ struct fvwb_ctx {
   double* rxc = nullptr;
   double* eyc = nullptr;
   double* eyf = nullptr;
+  double* eyf_ref = nullptr;   // y-face table at y = j*dx (the reference's face positions)
   Ctrl* ctrl = nullptr;
   unsigned long long* red = nullptr;
   int parity = 0;
@@ -134,6 +135,7 @@ extern "C" int fvwb_create(fvwb_handle* out, const fvwb_params* p) {
   check(alloc_dev(h, (void**)&h->rxc, nx * sizeof(double)));
   check(alloc_dev(h, (void**)&h->eyc, ny * sizeof(double)));
   check(alloc_dev(h, (void**)&h->eyf, (ny + 1) * sizeof(double)));
+  check(alloc_dev(h, (void**)&h->eyf_ref, (ny + 1) * sizeof(double)));
   check(alloc_dev(h, (void**)&h->ctrl, sizeof(Ctrl)));
   check(alloc_dev(h, (void**)&h->red, sizeof(unsigned long long)));
   if (rc != FVWB_OK) {
@@ -145,15 +147,19 @@ extern "C" int fvwb_create(fvwb_handle* out, const fvwb_params* p) {
   const double dx = p->boxlen_x / nx, dy = p->boxlen_y / ny, k = p->rho0 * p->g / p->p0;
   std::string tmp;
   double* hx = new double[2 * (size_t)nx + 1];
-  double* hy = new double[2 * (size_t)ny + 1];
+  double* hy = new double[3 * (size_t)ny + 2];
   for (int i = 0; i <= nx; ++i) hx[i] = p->rho0 * std::exp(-(k * ((double)i * dx)));
   for (int i = 0; i < nx; ++i) hx[nx + 1 + i] = p->rho0 * std::exp(-(k * (((double)i + 0.5) * dx)));
   for (int j = 0; j < ny; ++j) hy[j] = std::exp(-(k * (((double)j + 0.5) * dy)));
   for (int j = 0; j <= ny; ++j) hy[ny + j] = std::exp(-(k * ((double)j * dy)));
+  // benchmark_2d.f90:513 places the y faces at (j-1)*dx, not (j-1)*dy
+  for (int j = 0; j <= ny; ++j) hy[2 * ny + 1 + j] = std::exp(-(k * ((double)j * dx)));
   hipError_t e1 = hipMemcpy(h->rxf, hx, (nx + 1) * sizeof(double), hipMemcpyHostToDevice);
   hipError_t e2 = hipMemcpy(h->rxc, hx + nx + 1, nx * sizeof(double), hipMemcpyHostToDevice);
   hipError_t e3 = hipMemcpy(h->eyc, hy, ny * sizeof(double), hipMemcpyHostToDevice);
   hipError_t e4 = hipMemcpy(h->eyf, hy + ny, (ny + 1) * sizeof(double), hipMemcpyHostToDevice);
+  if (e4 == hipSuccess)
+    e4 = hipMemcpy(h->eyf_ref, hy + 2 * ny + 1, (ny + 1) * sizeof(double), hipMemcpyHostToDevice);
   delete[] hx;
   delete[] hy;
   if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
@@ -175,7 +181,8 @@ extern "C" int fvwb_destroy(fvwb_handle h) {
   if (!h) return FVWB_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (int b = 0; b < 3; ++b) if (h->buf[b]) (void)hipFree(h->buf[b]);
-  double* ds[] = {h->w_eq, h->stage_u, h->stage_w, h->res_in, h->res_out, h->rxf, h->rxc, h->eyc, h->eyf};
+  double* ds[] = {h->w_eq, h->stage_u, h->stage_w, h->res_in, h->res_out, h->rxf, h->rxc, h->eyc, h->eyf,
+                   h->eyf_ref};
   for (double* d : ds) if (d) (void)hipFree(d);
   if (h->ctrl) (void)hipFree(h->ctrl);
   if (h->red) (void)hipFree(h->red);
@@ -218,7 +225,9 @@ static int ensure_residual_buffers(fvwb_ctx* h) {
   return FVWB_OK;
 }
 
-extern "C" int fvwb_compute_update_exact(fvwb_handle h, const double* u, const double* w_eq, double* dudt) {
+// Residual of the handle's scheme on host arrays; reference_faces selects the
+// reference's y-face positions (j*dx) for the face equilibrium.
+static int residual_host(fvwb_handle h, const double* u, const double* w_eq, double* dudt, bool reference_faces) {
   if (!h || !u || !w_eq || !dudt) return fail(h, FVWB_ERR_ARG, "NULL argument");
   int rc = ensure_residual_buffers(h);
   if (rc) return rc;
@@ -230,12 +239,21 @@ extern "C" int fvwb_compute_update_exact(fvwb_handle h, const double* u, const d
                                         h->stream));
   StageArgs<double> a = base_args<double>(h);
   a.in = h->res_in; a.un = nullptr; a.out = h->res_out;
+  if (reference_faces) a.eyf = h->eyf_ref;
   FVWB_HIP(h, launch_fv2d_stage_f64(a, p.recon, p.flux, p.bc, 0, h->stream));
   FVWB_HIP(h, launch_dev_to_aos<double>(h->res_out, nullptr, h->stage_u, p.nx, p.ny, h->pitch, h->plane, p.gamma,
                                         false, h->stream));
   FVWB_HIP(h, hipMemcpyAsync(dudt, h->stage_u, bytes, hipMemcpyDeviceToHost, h->stream));
   FVWB_HIP(h, hipStreamSynchronize(h->stream));
   return FVWB_OK;
+}
+
+extern "C" int fvwb_compute_update_exact(fvwb_handle h, const double* u, const double* w_eq, double* dudt) {
+  return residual_host(h, u, w_eq, dudt, true);
+}
+
+extern "C" int fvwb_compute_residual(fvwb_handle h, const double* u, const double* w_eq, double* dudt) {
+  return residual_host(h, u, w_eq, dudt, false);
 }
 
 extern "C" int fvwb_compute_max_speed(fvwb_handle h, const double* u, double* cmax) {

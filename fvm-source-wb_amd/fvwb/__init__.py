@@ -59,6 +59,26 @@ def make_params(nx, ny, *, bc=BC_FROZEN_RING, recon=RECON_FIRST_ORDER, flux=FLUX
                   cfl, rho0, p0, g)
 
 
+BC1D_PERIODIC, BC1D_ZERO_GRADIENT, BC1D_REFLEXIVE, BC1D_WALL = 1, 2, 3, 4
+
+
+class Params1D(ctypes.Structure):
+    """``fvwb1d_params`` of include/fvwb.h."""
+
+    _fields_ = [
+        ("nvar", ctypes.c_int32), ("nx", ctypes.c_int32), ("bc", ctypes.c_int32),
+        ("recon", ctypes.c_int32), ("integrator", ctypes.c_int32), ("nequilibrium", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("boxlen", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("g", ctypes.c_double),
+    ]
+
+
+def make_params_1d(nx, *, bc=BC1D_WALL, recon=RECON_MUSCL_MINMOD, integrator=RK2_HEUN, nequilibrium=2,
+                   gamma=1.4, boxlen=1.0, cfl=0.5, g=1.0) -> Params1D:
+    """Defaults: the 1D gravity workload (MUSCL-minmod, Rusanov, SSP-RK2, walls, CFL 0.5)."""
+    return Params1D(3, nx, bc, recon, integrator, nequilibrium, gamma, boxlen, cfl, g)
+
+
 _lib = None
 
 
@@ -66,7 +86,7 @@ def declared_symbols(header=HEADER):
     """Names of the functions include/fvwb.h declares."""
     import re
     text = open(header).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fvwb_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fvwb(?:1d)?_\w+)\s*\(", text, re.M)))
 
 
 def lib():
@@ -86,6 +106,7 @@ def lib():
         "fvwb_create": [ctypes.POINTER(H), pp],
         "fvwb_destroy": [H],
         "fvwb_compute_update_exact": [H, dp, dp, dp],
+        "fvwb_compute_residual": [H, dp, dp, dp],
         "fvwb_compute_max_speed": [H, dp, dp],
         "fvwb_upload": [H, dp, dp],
         "fvwb_init_gaussian": [H, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double],
@@ -114,6 +135,21 @@ def lib():
         "fvwb_batch_autotune": [H, ctypes.c_int32, ctypes.c_char_p, ip, ip],
         "fvwb_get_launch": [H, ip, ip],
     }
+    p1 = ctypes.POINTER(Params1D)
+    sig.update({
+        "fvwb1d_params_reference": [p1, ctypes.c_int32, ctypes.c_int32],
+        "fvwb1d_create": [ctypes.POINTER(H), ctypes.c_int32, p1],
+        "fvwb1d_destroy": [H],
+        "fvwb1d_compute_update": [H, dp, dp, dp],
+        "fvwb1d_compute_max_speed": [H, dp, dp],
+        "fvwb1d_compute_residual": [H, ctypes.c_int32, dp, dp, dp],
+        "fvwb1d_upload": [H, ctypes.c_int32, dp, dp],
+        "fvwb1d_init_gaussian": [H, dp, ctypes.c_double, ctypes.c_double],
+        "fvwb1d_advance": [H, ctypes.c_int32, ctypes.c_double],
+        "fvwb1d_download": [H, ctypes.c_int32, dp],
+        "fvwb1d_get_time": [H, ctypes.c_int32, dp, dp, ip],
+        "fvwb1d_get_stream": [H, ctypes.POINTER(ctypes.c_void_p)],
+    })
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -122,6 +158,8 @@ def lib():
     L.fvwb_last_error.restype = ctypes.c_char_p
     L.fvwb_batch_last_error.argtypes = [H]
     L.fvwb_batch_last_error.restype = ctypes.c_char_p
+    L.fvwb1d_last_error.argtypes = [H]
+    L.fvwb1d_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -184,6 +222,13 @@ class Solver2D:
         u, w_eq = _fortran(u, self.shape), _fortran(w_eq, self.shape)
         out = np.zeros(self.shape, order="F")
         _check(lib().fvwb_compute_update_exact(self._h, _dp(u), _dp(w_eq), _dp(out)), self._h)
+        return out
+
+    def compute_residual(self, u, w_eq):
+        """L(u) of the configured scheme, the residual advance() integrates (y faces at (j-1)*dy)."""
+        u, w_eq = _fortran(u, self.shape), _fortran(w_eq, self.shape)
+        out = np.zeros(self.shape, order="F")
+        _check(lib().fvwb_compute_residual(self._h, _dp(u), _dp(w_eq), _dp(out)), self._h)
         return out
 
     def compute_max_speed(self, u):
@@ -353,3 +398,115 @@ def evolve(u, u_eq, params: Params, t_end=math.inf, max_steps=1 << 30):
         s.upload(u, u_eq)
         t, n = s.advance(max_steps, t_end)
         return s.download(), t, n
+
+
+# ---- 1D (benchmark_1d.f90) ---------------------------------------------------------
+def reference_params_1d(nx, nequilibrium=2) -> Params1D:
+    """benchmark_1d.f90's set-up (parameters.f90) for nx cells."""
+    p = Params1D()
+    rc = lib().fvwb1d_params_reference(ctypes.byref(p), nx, nequilibrium)
+    if rc != OK:
+        raise FvwbError(rc, "bad reference parameters")
+    return p
+
+
+def _check1d(rc, h=None):
+    if rc != OK:
+        msg = lib().fvwb1d_last_error(h)
+        raise FvwbError(rc, msg.decode() if msg else "")
+
+
+class Solver1D:
+    """``ninst`` independent 1D problems on the GPU (instance 0 serves the
+    reference's per-stage procedures).  Arrays are float64 AoS(3,nx):
+    numpy shape (3, nx), order="F"."""
+
+    def __init__(self, params):
+        if isinstance(params, Params1D):
+            params = [params]
+        self.params = list(params)
+        arr = (Params1D * len(self.params))(*self.params)
+        self._h = ctypes.c_void_p()
+        _check1d(lib().fvwb1d_create(ctypes.byref(self._h), len(self.params), arr))
+
+    def close(self):
+        if self._h:
+            lib().fvwb1d_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _nx(self, inst):
+        if not 0 <= inst < len(self.params):
+            raise FvwbError(ERR_ARG, f"instance {inst} out of range")
+        return self.params[inst].nx
+
+    def _arr(self, a, inst=0):
+        a = np.asfortranarray(a, dtype=np.float64)
+        if a.shape != (3, self._nx(inst)):
+            raise FvwbError(ERR_ARG, f"expected shape (3, {self._nx(inst)}), got {a.shape}")
+        return a
+
+    def compute_update(self, u, w_eq):
+        """dudt of compute_update(u, w_eq, dudt) (benchmark_1d.f90:263-377), instance 0."""
+        u, w_eq = self._arr(u), self._arr(w_eq)
+        out = np.zeros((3, self._nx(0)), order="F")
+        _check1d(lib().fvwb1d_compute_update(self._h, _dp(u), _dp(w_eq), _dp(out)), self._h)
+        return out
+
+    def compute_max_speed(self, u):
+        """cmax of compute_max_speed(u, cmax) (benchmark_1d.f90:157-169), instance 0."""
+        u = self._arr(u)
+        c = ctypes.c_double()
+        _check1d(lib().fvwb1d_compute_max_speed(self._h, _dp(u), ctypes.byref(c)), self._h)
+        return c.value
+
+    def compute_residual(self, u, w_eq, inst=0):
+        u, w_eq = self._arr(u, inst), self._arr(w_eq, inst)
+        out = np.zeros((3, self._nx(inst)), order="F")
+        _check1d(lib().fvwb1d_compute_residual(self._h, inst, _dp(u), _dp(w_eq), _dp(out)), self._h)
+        return out
+
+    def upload(self, u, w_eq, inst=0):
+        u, w_eq = self._arr(u, inst), self._arr(w_eq, inst)
+        _check1d(lib().fvwb1d_upload(self._h, inst, _dp(u), _dp(w_eq)), self._h)
+
+    def init_gaussian(self, amp, x0=0.5, width=0.05):
+        amp = np.ascontiguousarray(np.broadcast_to(np.asarray(amp, dtype=np.float64), (len(self.params),)))
+        _check1d(lib().fvwb1d_init_gaussian(self._h, _dp(amp), x0, width), self._h)
+
+    def advance(self, nsteps=None, t_end=math.inf):
+        """Advance every instance until t_end (last step clipped) or nsteps more steps."""
+        if nsteps is None:
+            if not math.isfinite(t_end):
+                raise FvwbError(ERR_ARG, "give nsteps or a finite t_end")
+            nsteps = 2**31 - 1
+        _check1d(lib().fvwb1d_advance(self._h, int(min(nsteps, 2**31 - 1)), float(t_end)), self._h)
+
+    def download(self, inst=0):
+        out = np.zeros((3, self._nx(inst)), order="F")
+        _check1d(lib().fvwb1d_download(self._h, inst, _dp(out)), self._h)
+        return out
+
+    def time(self, inst=0):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int32()
+        self._nx(inst)
+        _check1d(lib().fvwb1d_get_time(self._h, inst, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)),
+                 self._h)
+        return t.value, dt.value, n.value
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        _check1d(lib().fvwb1d_get_stream(self._h, ctypes.byref(s)), self._h)
+        return s.value or 0

@@ -92,8 +92,15 @@ const char* fvwb_last_error(fvwb_handle h);
  * (benchmark_2d.f90:465-618).  u: conservative state AoS(4,nx,ny); w_eq:
  * PRIMITIVE equilibrium at cell centres AoS(4,nx,ny), at rest; dudt (out):
  * AoS(4,nx,ny).  Uses the handle's recon/flux/bc (the reference set-up is
- * FIRST_ORDER/RUSANOV/FROZEN_RING).  Always computed in FP64. */
+ * FIRST_ORDER/RUSANOV/FROZEN_RING).  Always computed in FP64.  The face
+ * equilibrium sits where the reference puts it: x faces at (i-1)*dx, y faces at
+ * (j-1)*dx (benchmark_2d.f90:507-514; the same as (j-1)*dy when dx == dy). */
 int fvwb_compute_update_exact(fvwb_handle h, const double* u, const double* w_eq, double* dudt);
+
+/* The residual L(u) that fvwb_advance integrates: as fvwb_compute_update_exact
+ * but with the y faces at (j-1)*dy.  No separate counterpart in the reference
+ * (identical to compute_update_exact on its square grids). */
+int fvwb_compute_residual(fvwb_handle h, const double* u, const double* w_eq, double* dudt);
 
 /* Replaces `subroutine compute_max_speed(u, cmax)` (benchmark_2d.f90:264-279):
  * cmax = max over cells of sqrt(vx^2+vy^2) + sqrt(gamma*max(p,1e-10)/max(rho,1e-10)). */
@@ -182,6 +189,77 @@ int fvwb_batch_get_launch(fvwb_batch_handle h, int32_t* rows, int32_t* cpl, int3
  * cache_path (NULL: no cache); a cached entry for the same batch is reused. */
 int fvwb_batch_autotune(fvwb_batch_handle h, int32_t trial_steps, const char* cache_path, int32_t* rows,
                         int32_t* cpl);
+
+/* ---- 1D: benchmark_1d.f90 ------------------------------------------------
+ * 1D Euler with gravity, gamma-law gas, potential phi = g*x on [0,boxlen],
+ * state u(1:3,1:nx) = (rho, rho*v, E) with the variable index fastest.
+ * Equilibria (nequilibrium, benchmark_1d.f90:134-155 for g = 1):
+ *   1, 2  isothermal  rho = p = exp(-g*x), v = 0
+ *   3     polytropic  rho = (1 - (gamma-1)/gamma*g*x)^(1/(gamma-1)), p = rho^gamma
+ * A handle holds `ninst` independent instances that share recon, integrator and
+ * bc; each has its own nx, boxlen, g, cfl and clock. */
+#define FVWB1D_BC_PERIODIC       1  /* benchmark_1d.f90:318-321 codes; in the    */
+#define FVWB1D_BC_ZERO_GRADIENT  2  /* reference they only change the two        */
+#define FVWB1D_BC_REFLEXIVE      3  /* boundary faces (see fvwb1d_compute_update) */
+#define FVWB1D_BC_WALL           4  /* configured scheme: solid walls, ghost cells
+                                       mirror the deviation, momentum negated     */
+
+typedef struct fvwb1d_params {
+  int32_t nvar;          /* must be 3                                         */
+  int32_t nx;            /* cells, nx >= 4 (fvwb1d_advance: nx <= 2048)       */
+  int32_t bc;            /* FVWB1D_BC_*; the configured scheme takes
+                            ZERO_GRADIENT or WALL                              */
+  int32_t recon;         /* FVWB_RECON_FIRST_ORDER / FVWB_RECON_MUSCL_MINMOD  */
+  int32_t integrator;    /* FVWB_RK2_HEUN (SSP-RK2) / FVWB_SSPRK3              */
+  int32_t nequilibrium;  /* 1, 2 or 3 (above)                                  */
+  double  gamma;         /* reference: 1.4 as an FP32 literal                  */
+  double  boxlen;        /* reference: 1                                       */
+  double  cfl;           /* dt = cfl*dx/max(|v|+c), last step clipped to t_end */
+  double  g;             /* reference: 1                                       */
+} fvwb1d_params;
+
+typedef struct fvwb1d* fvwb1d_handle;
+
+/* Fill `p` with benchmark_1d.f90's set-up (parameters.f90:1-22): nx cells,
+ * gamma = (double)1.4f, boxlen 1, g 1, zero-gradient bc code, first order,
+ * SSP-RK2, cfl 0.5. */
+int fvwb1d_params_reference(fvwb1d_params* p, int32_t nx, int32_t nequilibrium);
+int fvwb1d_create(fvwb1d_handle* h, int32_t ninst, const fvwb1d_params* per_inst);
+int fvwb1d_destroy(fvwb1d_handle h);
+const char* fvwb1d_last_error(fvwb1d_handle h);
+
+/* Replaces `subroutine compute_update(u, w_eq, dudt)` of benchmark_1d.f90:263-377
+ * (the EQL exact-deviation residual) for instance 0: u conservative AoS(3,nx),
+ * w_eq PRIMITIVE equilibrium at cell centres AoS(3,nx), dudt (out) AoS(3,nx).
+ * First order, Rusanov flux (compute_llflux :438-451), face states U_eq(face) +
+ * (u - U_eq(centre)), source -rho*(x(i+1)-x(i-1))/(2/nx) (get_source :379-405,
+ * evaluated on the conservative state as the reference does), and
+ * dudt(1) = dudt(2), dudt(nx) = dudt(nx-1) (:374-375); the bc code changes only
+ * the two boundary faces, whose fluxes reach no other cell, so it does not
+ * change dudt.  Always FP64, rounding statement by statement as the reference. */
+int fvwb1d_compute_update(fvwb1d_handle h, const double* u, const double* w_eq, double* dudt);
+/* Replaces `subroutine compute_max_speed(u, cmax)` (benchmark_1d.f90:157-184)
+ * for instance 0: cmax = max |v| + sqrt(gamma*max(p,1e-10)/max(rho,1e-10)). */
+int fvwb1d_compute_max_speed(fvwb1d_handle h, const double* u, double* cmax);
+/* Residual of the configured scheme for instance `inst` (what fvwb1d_advance
+ * integrates): deviation form with face equilibrium, p.recon, Rusanov, source
+ * (0, -g*drho, -g*dm), ghost cells per p.bc.  No counterpart in the reference. */
+int fvwb1d_compute_residual(fvwb1d_handle h, int32_t inst, const double* u, const double* w_eq, double* dudt);
+
+/* Device-resident stepping.  upload: host state of one instance (w_eq as
+ * above); init_gaussian: equilibrium plus a pressure pulse
+ * amp[b]*exp(-(x-x0)^2/(2*width^2)) in every instance b (fvm.f90:124
+ * convention); clocks restart at 0. */
+int fvwb1d_upload(fvwb1d_handle h, int32_t inst, const double* u, const double* w_eq);
+int fvwb1d_init_gaussian(fvwb1d_handle h, const double* amp, double x0, double width);
+/* Advance every instance until its t reaches t_end (the last step clipped) or
+ * it has taken nsteps more steps: one launch, one workgroup per instance, the
+ * whole run in LDS.  Blocks until done; FVWB_ERR_NONPHYSICAL if any instance
+ * produced rho<=0, p<=0 or a non-finite value (its clock stops there). */
+int fvwb1d_advance(fvwb1d_handle h, int32_t nsteps, double t_end);
+int fvwb1d_download(fvwb1d_handle h, int32_t inst, double* u);
+int fvwb1d_get_time(fvwb1d_handle h, int32_t inst, double* t, double* dt_last, int32_t* steps_done);
+int fvwb1d_get_stream(fvwb1d_handle h, void** stream);
 
 #ifdef __cplusplus
 }

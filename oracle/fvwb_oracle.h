@@ -12,7 +12,11 @@
  *                  flang build of the reference in oracle/_ref;
  *   orc_fv2d_*   — the configured scheme (MUSCL-minmod / HLL / reflecting walls /
  *                  SSP-RK3) in the same plain FP64 form, pinned by invariants
- *                  and closed forms in tests/.
+ *                  and closed forms in tests/;
+ *   orc_ref1d_*  — restatements of benchmark_1d.f90 (the 'EQL' path), pinned
+ *                  like orc_ref2d_*;
+ *   orc_fv1d_*   — the configured 1D scheme (deviation form, MUSCL-minmod,
+ *                  Rusanov, SSP-RK2/RK3, walls), pinned by invariants.
  */
 #ifndef FVWB_ORACLE_H
 #define FVWB_ORACLE_H
@@ -69,6 +73,31 @@ double orc_fv2d_max_speed(const fvwb_params* p, const double* u);
 void   orc_fv2d_advance(const fvwb_params* p, double* u, const double* w_eq,
                         double t_end, int max_steps, double* t_out, int* steps_out,
                         double* dt_last);
+
+/* ---- 1D: benchmark_1d.f90 restatements (nvar = 3, AoS(3,nx)) ------------ */
+void   orc_ref1d_primitive(double gamma, const double u[3], double w[3]);      /* :115-123 */
+void   orc_ref1d_conservative(double gamma, const double w[3], double u[3]);   /* :125-132 */
+double orc_ref1d_speed(double gamma, const double u[3]);                       /* :171-184 */
+void   orc_ref1d_flux(double gamma, const double u[3], double f[3]);           /* :186-198 */
+void   orc_ref1d_llflux(double gamma, const double ul[3], const double ur[3],   /* :438-451 */
+                        const double fl[3], const double fr[3], double out[3]);
+/* get_equilibrium_solution (:134-155) at n points */
+void   orc_ref1d_equilibrium(double gamma, int nequilibrium, int n, const double* x, double* w);
+/* compute_update, solver 'EQL' (:263-377), with p->nx, gamma, boxlen, bc code
+ * (1 periodic, 2 zero gradient, 3 reflexive) and nequilibrium */
+void   orc_ref1d_update(const fvwb1d_params* p, const double* u, const double* w_eq, double* dudt);
+/* compute_max_speed (:157-169) */
+double orc_ref1d_max_speed(const fvwb1d_params* p, const double* u);
+
+/* ---- 1D configured scheme (deviation form, plain FP64) -------------------- */
+void   orc_fv1d_equilibrium(const fvwb1d_params* p, double x, double w[3]);
+void   orc_fv1d_face_flux(const fvwb1d_params* p, const double ueq[3], const double dl[3],
+                          const double dr[3], double out[3]);
+void   orc_fv1d_residual(const fvwb1d_params* p, const double* u, const double* w_eq, double* dudt);
+/* dt = cfl*dx/cmax, last step clipped to t_end; runs while t < t_end and
+ * steps < max_steps */
+void   orc_fv1d_advance(const fvwb1d_params* p, double* u, const double* w_eq, double t_end,
+                        int max_steps, double* t_out, int* steps_out, double* dt_last);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,25 @@ class Params(ctypes.Structure):
     ]
 
 
+BC1D_PERIODIC, BC1D_ZERO_GRADIENT, BC1D_REFLEXIVE, BC1D_WALL = 1, 2, 3, 4
+
+
+class Params1D(ctypes.Structure):
+    """Mirror of ``fvwb1d_params`` (include/fvwb.h)."""
+
+    _fields_ = [
+        ("nvar", ctypes.c_int32), ("nx", ctypes.c_int32), ("bc", ctypes.c_int32),
+        ("recon", ctypes.c_int32), ("integrator", ctypes.c_int32), ("nequilibrium", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("boxlen", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("g", ctypes.c_double),
+    ]
+
+
+def make_params_1d(nx, *, bc=BC1D_ZERO_GRADIENT, recon=RECON_FIRST_ORDER, integrator=RK2_HEUN,
+                   nequilibrium=2, gamma=GAMMA_REF, boxlen=1.0, cfl=0.5, g=1.0) -> Params1D:
+    return Params1D(3, nx, bc, recon, integrator, nequilibrium, gamma, boxlen, cfl, g)
+
+
 def make_params(nx, ny, *, bc=BC_FROZEN_RING, recon=RECON_FIRST_ORDER, flux=FLUX_RUSANOV,
                 integrator=RK2_HEUN, precision=FP64, gamma=GAMMA_REF, boxlen_x=1.0,
                 boxlen_y=1.0, cfl=0.5, rho0=RHO0_REF, p0=1.0, g=1.0) -> Params:
@@ -85,6 +104,18 @@ def lib():
         L.orc_ref_flux.argtypes = [ctypes.c_double, dp, dp, dp]
         L.orc_ref_speed.argtypes = [ctypes.c_double, dp]
         L.orc_ref_speed.restype = ctypes.c_double
+        p1 = ctypes.POINTER(Params1D)
+        L.orc_ref1d_update.argtypes = [p1, dp, dp, dp]
+        L.orc_ref1d_max_speed.argtypes = [p1, dp]
+        L.orc_ref1d_max_speed.restype = ctypes.c_double
+        L.orc_ref1d_equilibrium.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.orc_ref1d_llflux.argtypes = [ctypes.c_double, dp, dp, dp, dp, dp]
+        L.orc_ref1d_flux.argtypes = [ctypes.c_double, dp, dp]
+        L.orc_fv1d_residual.argtypes = [p1, dp, dp, dp]
+        L.orc_fv1d_face_flux.argtypes = [p1, dp, dp, dp, dp]
+        L.orc_fv1d_equilibrium.argtypes = [p1, ctypes.c_double, dp]
+        L.orc_fv1d_advance.argtypes = [p1, dp, dp, ctypes.c_double, ctypes.c_int, dp,
+                                       ctypes.POINTER(ctypes.c_int), dp]
         _lib = L
     return _lib
 
@@ -239,4 +270,89 @@ def perturbed_state(p: Params, amp_bump=1e-3, amp_noise=1e-4, x0=0.3, y0=0.3, wi
         for v in range(4):
             scale = ueq[0] if v < 3 else ueq[3]
             u[v] += amp_noise * scale * hash_noise(nx, ny, salt * 4 + v)
+    return u, w
+
+
+# ---- 1D (benchmark_1d.f90) ---------------------------------------------------------
+def ref1d_update(p: Params1D, u, w_eq):
+    u, w_eq = fortran(u), fortran(w_eq)
+    out = np.zeros_like(u, order="F")
+    lib().orc_ref1d_update(ctypes.byref(p), _p(u), _p(w_eq), _p(out))
+    return out
+
+
+def ref1d_max_speed(p: Params1D, u):
+    return lib().orc_ref1d_max_speed(ctypes.byref(p), _p(fortran(u)))
+
+
+def ref1d_flux(gamma, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    f = np.zeros(3)
+    lib().orc_ref1d_flux(gamma, _p(u), _p(f))
+    return f
+
+
+def ref1d_llflux(gamma, ul, ur, fl, fr):
+    ul, ur, fl, fr = (np.ascontiguousarray(a, dtype=np.float64) for a in (ul, ur, fl, fr))
+    out = np.zeros(3)
+    lib().orc_ref1d_llflux(gamma, _p(ul), _p(ur), _p(fl), _p(fr), _p(out))
+    return out
+
+
+def fv1d_residual(p: Params1D, u, w_eq):
+    u, w_eq = fortran(u), fortran(w_eq)
+    out = np.zeros_like(u, order="F")
+    lib().orc_fv1d_residual(ctypes.byref(p), _p(u), _p(w_eq), _p(out))
+    return out
+
+
+def fv1d_face_flux(p: Params1D, ueq, dl, dr):
+    ueq, dl, dr = (np.ascontiguousarray(a, dtype=np.float64) for a in (ueq, dl, dr))
+    out = np.zeros(3)
+    lib().orc_fv1d_face_flux(ctypes.byref(p), _p(ueq), _p(dl), _p(dr), _p(out))
+    return out
+
+
+def fv1d_advance(p: Params1D, u, w_eq, t_end=float("inf"), max_steps=1 << 30):
+    u = fortran(u).copy(order="F")
+    w_eq = fortran(w_eq)
+    t, dt = ctypes.c_double(), ctypes.c_double()
+    n = ctypes.c_int()
+    lib().orc_fv1d_advance(ctypes.byref(p), _p(u), _p(w_eq), t_end, max_steps, ctypes.byref(t),
+                           ctypes.byref(n), ctypes.byref(dt))
+    return u, t.value, n.value, dt.value
+
+
+def equilibrium_1d(p: Params1D, x):
+    """Closed-form equilibrium of p at points x, primitive AoS(3,n)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    w = np.zeros((3, len(x)), order="F")
+    for i, xv in enumerate(x):
+        wi = np.zeros(3)
+        lib().orc_fv1d_equilibrium(ctypes.byref(p), float(xv), _p(wi))
+        w[:, i] = wi
+    return w
+
+
+def conservative_1d(gamma, w):
+    u = np.zeros_like(w, order="F")
+    u[0] = w[0]
+    u[1] = w[0] * w[1]
+    u[2] = w[2] / (gamma - 1.0) + 0.5 * w[0] * (w[1] * w[1])
+    return u
+
+
+def perturbed_state_1d(p: Params1D, amp_bump=1e-3, x0=0.5, width=0.05, amp_noise=0.0, salt=1):
+    """Equilibrium plus a Gaussian pressure pulse amp*exp(-(x-x0)^2/(2 width^2))
+    (the fvm.f90:124 convention) and hash noise of relative size amp_noise on
+    every conservative variable; returns (u, w_eq) AoS(3,nx)."""
+    nx = p.nx
+    x = (np.arange(nx) + 0.5) * (p.boxlen / nx)
+    w = equilibrium_1d(p, x)
+    u = conservative_1d(p.gamma, w)
+    u[2] += amp_bump * np.exp(-((x - x0) ** 2) / (2.0 * width ** 2)) / (p.gamma - 1.0)
+    if amp_noise:
+        for v in range(3):
+            scale = u[0] if v < 2 else u[2]
+            u[v] += amp_noise * scale * hash_noise(nx, 1, salt * 3 + v)[:, 0]
     return u, w

@@ -39,6 +39,16 @@ def bench2d_inputs(salt=1, amp_bump=1e-3, amp_noise=1e-4):
     return p, u, w
 
 
+def bench1d_inputs(salt=1):
+    """benchmark_1d set-up (nequilibrium=2, nx=128) plus a pressure pulse and
+    hash noise on every variable; inputs rounded through FP32."""
+    p = po.make_params_1d(128)
+    u, w = po.perturbed_state_1d(p, amp_bump=1e-3, amp_noise=1e-4, salt=salt)
+    u = np.asfortranarray(u.astype(np.float32).astype(np.float64))
+    w = np.asfortranarray(w.astype(np.float32).astype(np.float64))
+    return p, u, w
+
+
 def sample_index(shape, n=NSAMPLE, salt=7):
     size = int(np.prod(shape))
     h = po.hash_noise(n, 1, salt)[:, 0]
@@ -111,6 +121,26 @@ def main():
     r = rr.run("dg2d_legendre", x=x)
     out["dg2d_legendre"] = {"x": x, "P": r["P"], "dP": r["dP"]}
     out["dg2d_gl_quadrature"] = rr.run("dg2d_gl_quadrature")
+
+    # A13 benchmark_1d compute_update ('EQL') and compute_max_speed, nx = 128
+    assert rr.available("bench1d"), "build oracle/_ref first"
+    p1, u1, w1 = bench1d_inputs()
+    d1 = rr.run("bench1d_update", u=u1, w_eq=w1)["dudt"]
+    c1 = rr.run("bench1d_max_speed", u=u1)["cmax"]
+    ue1 = po.conservative_1d(p1.gamma, w1)
+    d0 = rr.run("bench1d_update", u=ue1, w_eq=w1)["dudt"]
+    out["bench1d_update_128"] = {"u": u1, "w_eq": w1, "dudt": d1, "cmax": c1, "dudt_eq": d0}
+    # compute_llflux on random state pairs
+    m = 256
+    ul = np.zeros((m, 3)); ur = np.zeros((m, 3))
+    for a in (ul, ur):
+        a[:, 0] = rng.uniform(0.2, 2.0, m)
+        a[:, 1] = rng.uniform(-2, 2, m) * a[:, 0]
+        a[:, 2] = rng.uniform(0.2, 3.0, m) / (p1.gamma - 1) + 0.5 * a[:, 1] ** 2 / a[:, 0]
+    fl1 = np.array([po.ref1d_flux(p1.gamma, x) for x in ul])
+    fr1 = np.array([po.ref1d_flux(p1.gamma, x) for x in ur])
+    f1 = rr.run("bench1d_llflux", ul=ul, ur=ur, fl=fl1, fr=fr1)["f"]
+    out["bench1d_llflux"] = {"ul": ul, "ur": ur, "fl": fl1, "fr": fr1, "f": f1, "gamma": np.array(p1.gamma)}
 
     for name, arrs in out.items():
         path = os.path.join(HERE, name + ".npz")

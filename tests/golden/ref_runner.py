@@ -89,6 +89,29 @@ def _child(op, fin, fout):
             out["w"] = w
         else:
             raise ValueError(op)
+    elif op.startswith("bench1d_"):
+        L = ctypes.CDLL(os.path.join(REF_DIR, LIBS["bench1d"]))
+        if op == "bench1d_update":
+            u = np.asfortranarray(z["u"]); w = np.asfortranarray(z["w_eq"])
+            d = np.zeros_like(u, order="F")
+            L.compute_update_(_dp(u), _dp(w), _dp(d))
+            out["dudt"] = d
+        elif op == "bench1d_max_speed":
+            u = np.asfortranarray(z["u"])
+            c = np.zeros(1)
+            L.compute_max_speed_(_dp(u), _dp(c))
+            out["cmax"] = c
+        elif op == "bench1d_llflux":
+            ul, ur, fl, fr = (np.ascontiguousarray(z[k]) for k in ("ul", "ur", "fl", "fr"))
+            res = np.zeros_like(ul)
+            for q in range(ul.shape[0]):
+                o = np.zeros(3)
+                a, b, c, e = (np.ascontiguousarray(v[q]) for v in (ul, ur, fl, fr))
+                L.compute_llflux_(_dp(a), _dp(b), _dp(c), _dp(e), _dp(o))
+                res[q] = o
+            out["f"] = res
+        else:
+            raise ValueError(op)
     elif op.startswith("dg2d_"):
         L = ctypes.CDLL(os.path.join(REF_DIR, LIBS["dg2d"]))
         d = ctypes.c_double

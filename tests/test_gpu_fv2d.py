@@ -75,6 +75,23 @@ def test_compute_update_exact_equilibrium_is_bitwise_zero():
     assert np.count_nonzero(d) == 0
 
 
+@pytest.mark.parametrize("shape", [(64, 48), (40, 90)])
+def test_compute_update_exact_non_square_keeps_reference_face_positions(shape):
+    """On nx != ny the reference places the y faces at (j-1)*dx
+    (benchmark_2d.f90:507-514); the drop-in follows it, while the configured
+    residual uses (j-1)*dy, so the two differ."""
+    nx, ny = shape
+    say(f"case: compute_update_exact {nx}x{ny} reference set-up")
+    p = po.make_params(nx, ny)
+    u, w = po.perturbed_state(p, amp_bump=1e-3, amp_noise=1e-4, salt=4)
+    with fvwb.Solver2D(fvwb.reference_params(nx, ny, 2)) as s:
+        d = s.compute_update_exact(u, w)
+        r = s.compute_residual(u, w)
+    assert np.max(np.abs(d - po.ref2d_update_exact(p, 2, u, w))) <= res_tol(max(nx, ny))
+    assert np.max(np.abs(r - po.fv2d_residual(p, u, w))) <= res_tol(max(nx, ny))
+    assert np.max(np.abs(d - r)) > 1e3 * res_tol(max(nx, ny))
+
+
 def test_compute_max_speed_matches_reference_golden():
     say("case: compute_max_speed 300x300")
     g = load("bench2d_max_speed_300")
@@ -120,7 +137,7 @@ def test_residual_every_scheme_matches_oracle(scheme):
     op = orc_params(gp)
     u, w = po.perturbed_state(op, amp_bump=1e-2, amp_noise=1e-3, x0=0.6, y0=0.4, salt=3)
     with fvwb.Solver2D(gp) as s:
-        d = s.compute_update_exact(u, w)
+        d = s.compute_residual(u, w)
     np.testing.assert_allclose(d, po.fv2d_residual(op, u, w), rtol=0, atol=res_tol(130))
 
 
@@ -193,7 +210,7 @@ def test_edge_shapes_residual(shape, recon):
     op = orc_params(gp)
     u, w = po.perturbed_state(op, amp_bump=1e-2, amp_noise=1e-3, salt=6)
     with fvwb.Solver2D(gp) as s:
-        d = s.compute_update_exact(u, w)
+        d = s.compute_residual(u, w)
     np.testing.assert_allclose(d, po.fv2d_residual(op, u, w), rtol=0, atol=res_tol(max(nx, ny)))
 
 

@@ -1,0 +1,197 @@
+"""The 1D CPU oracle: the benchmark_1d.f90 restatement against the fixtures
+written from the flang-built reference (bit for bit) and live against
+oracle/_ref when present; the configured 1D scheme against what the
+mathematics fixes (exact rest, flux consistency, wall symmetry and mass
+conservation, convergence orders, Sod's exact solution) and against the 2D
+oracle on a y-uniform problem."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import ref_runner as rr
+from make_golden import bench1d_inputs
+from test_oracle_invariants import exact_riemann_density
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with np.load(os.path.join(GOLD, name + ".npz")) as z:
+        return {k: z[k] for k in z.files}
+
+
+# ---- the reference restatement -------------------------------------------------------
+def test_ref1d_update_matches_reference_golden_bitwise():
+    g = load("bench1d_update_128")
+    p, u, w = bench1d_inputs()
+    np.testing.assert_array_equal(u, g["u"])
+    np.testing.assert_array_equal(w, g["w_eq"])
+    np.testing.assert_array_equal(po.ref1d_update(p, u, w), g["dudt"])
+    assert po.ref1d_max_speed(p, u) == float(g["cmax"][0])
+    ue = po.conservative_1d(p.gamma, w)
+    np.testing.assert_array_equal(po.ref1d_update(p, ue, w), g["dudt_eq"])
+    assert np.count_nonzero(g["dudt_eq"]) == 0
+
+
+def test_ref1d_llflux_matches_reference_golden_bitwise():
+    g = load("bench1d_llflux")
+    gamma = float(g["gamma"])
+    for q in range(len(g["f"])):
+        np.testing.assert_array_equal(po.ref1d_flux(gamma, g["ul"][q]), g["fl"][q])
+        np.testing.assert_array_equal(po.ref1d_llflux(gamma, g["ul"][q], g["ur"][q], g["fl"][q], g["fr"][q]),
+                                      g["f"][q])
+
+
+@pytest.mark.skipif(not rr.available("bench1d"), reason="oracle/_ref not built")
+def test_ref1d_update_matches_live_reference_on_other_states():
+    for salt in (2, 3):
+        p, u, w = bench1d_inputs(salt=salt)
+        u[1] *= 50.0          # stronger flow: wave speeds differ across faces
+        d = rr.run("bench1d_update", u=u, w_eq=w)["dudt"]
+        np.testing.assert_array_equal(po.ref1d_update(p, u, w), d)
+
+
+def test_ref1d_bc_code_does_not_change_dudt():
+    """benchmark_1d.f90:374-375 overwrite both boundary cells, so the bc code
+    (which only changes the two boundary faces) cannot reach dudt."""
+    p, u, w = bench1d_inputs()
+    ds = []
+    for bc in (po.BC1D_PERIODIC, po.BC1D_ZERO_GRADIENT, po.BC1D_REFLEXIVE):
+        p.bc = bc
+        ds.append(po.ref1d_update(p, u, w))
+    np.testing.assert_array_equal(ds[0], ds[1])
+    np.testing.assert_array_equal(ds[0], ds[2])
+    np.testing.assert_array_equal(ds[0][:, 0], ds[0][:, 1])
+
+
+@pytest.mark.parametrize("neq", (1, 2, 3))
+def test_ref1d_equilibria_are_exact_fixed_points(neq):
+    p = po.make_params_1d(96, nequilibrium=neq)
+    x = (np.arange(96) + 0.5) / 96
+    w = np.zeros((3, 96), order="F")
+    lib = po.lib()
+    lib.orc_ref1d_equilibrium(p.gamma, neq, 96, po._p(np.ascontiguousarray(x)), po._p(w))
+    if neq == 3:   # polytropic: p = rho^gamma exactly in the closed form
+        np.testing.assert_allclose(w[2], w[0] ** p.gamma, rtol=1e-14)
+    else:
+        np.testing.assert_allclose(w[0], np.exp(-x), rtol=4e-16)   # numpy's exp: within 1 ulp of libm
+        np.testing.assert_array_equal(w[0], w[2])
+    assert np.count_nonzero(po.ref1d_update(p, po.conservative_1d(p.gamma, w), w)) == 0
+
+
+# ---- the configured scheme -------------------------------------------------------------
+SCHEMES_1D = [(r, bc) for r in (po.RECON_FIRST_ORDER, po.RECON_MUSCL_MINMOD)
+              for bc in (po.BC1D_ZERO_GRADIENT, po.BC1D_WALL)]
+
+
+@pytest.mark.parametrize("scheme", SCHEMES_1D)
+@pytest.mark.parametrize("neq,g", [(2, 1.0), (3, 1.0), (2, 2.5), (3, 0.7)])
+def test_fv1d_rest_is_exact(scheme, neq, g):
+    r, bc = scheme
+    p = po.make_params_1d(77, recon=r, bc=bc, nequilibrium=neq, g=g, gamma=1.4, integrator=po.SSPRK3)
+    u, w = po.perturbed_state_1d(p, amp_bump=0.0)
+    assert np.count_nonzero(po.fv1d_residual(p, u, w)) == 0
+    u2, t, n, _ = po.fv1d_advance(p, u, w, t_end=10.0, max_steps=60)
+    assert n == 60 and np.array_equal(u2, u)
+
+
+def test_fv1d_face_flux_is_consistent():
+    p = po.make_params_1d(16, gamma=1.4)
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        ueq = np.array([rng.uniform(0.5, 2), 0.0, rng.uniform(1, 4)])
+        d = np.array([rng.normal() * 0.1, rng.normal() * 0.2, rng.normal() * 0.1])
+        f = po.fv1d_face_flux(p, ueq, d, d)
+        np.testing.assert_array_equal(f, po.ref1d_flux(p.gamma, ueq + d) - po.ref1d_flux(p.gamma, ueq))
+
+
+def test_fv1d_walls_mirror_and_conserve_mass():
+    """Without gravity a pulse centred in the box stays mirror symmetric
+    (rho(x) = rho(1-x), m(x) = -m(1-x)), and walls keep the mass constant."""
+    p = po.make_params_1d(120, recon=po.RECON_MUSCL_MINMOD, bc=po.BC1D_WALL, g=0.0, gamma=1.4,
+                          integrator=po.SSPRK3)
+    u, w = po.perturbed_state_1d(p, amp_bump=0.3, x0=0.5, width=0.05)
+    u2, t, n, _ = po.fv1d_advance(p, u, w, t_end=0.4)
+    assert t == 0.4
+    np.testing.assert_allclose(u2[0], u2[0][::-1], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(u2[1], -u2[1][::-1], rtol=0, atol=1e-13)
+    assert abs(u2[0].sum() - u[0].sum()) <= 1e-13 * u[0].sum()
+    # with gravity the walls still conserve mass
+    p.g, p.nequilibrium = 1.0, 2
+    u, w = po.perturbed_state_1d(p, amp_bump=1e-2, x0=0.3)
+    u2, _, _, _ = po.fv1d_advance(p, u, w, t_end=0.5)
+    assert abs(u2[0].sum() - u[0].sum()) <= 1e-13 * u[0].sum()
+
+
+def _pulse_error(nx, recon, integ=po.SSPRK3, cfl=0.4):
+    p = po.make_params_1d(nx, recon=recon, bc=po.BC1D_WALL, g=1.0, gamma=1.4, integrator=integ, cfl=cfl)
+    u, w = po.perturbed_state_1d(p, amp_bump=1e-3, x0=0.5, width=0.08)
+    u2, _, _, _ = po.fv1d_advance(p, u, w, t_end=0.15)
+    return u2
+
+
+def _coarsen(a, f):
+    return a.reshape(a.shape[0], -1, f).mean(axis=2)
+
+
+@pytest.mark.parametrize("recon,lo,hi", [(po.RECON_FIRST_ORDER, 0.7, 1.3), (po.RECON_MUSCL_MINMOD, 1.6, 2.6)])
+def test_fv1d_spatial_order(recon, lo, hi):
+    """Smooth small pulse: the cell-average error against a 16x finer run
+    shrinks at the scheme's order."""
+    ref = _pulse_error(1280, recon)
+    errs = []
+    for n in (80, 160, 320):
+        u = _pulse_error(n, recon)
+        errs.append(np.mean(np.abs(u[2] - _coarsen(ref, 1280 // n)[2])))
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all((orders > lo) & (orders < hi)), (errs, orders)
+
+
+@pytest.mark.parametrize("integ,order", [(po.RK2_HEUN, 2), (po.SSPRK3, 3)])
+def test_fv1d_time_order(integ, order):
+    """Same grid, CFL number halved: the change shrinks as cfl^order."""
+    a, b, c = (_pulse_error(200, po.RECON_MUSCL_MINMOD, integ, cfl) for cfl in (0.4, 0.2, 0.1))
+    r = np.log2(np.max(np.abs(a - b)) / np.max(np.abs(b - c)))
+    assert abs(r - order) < 0.45, r
+
+
+def test_fv1d_sod_converges_to_exact_solution():
+    errs = []
+    for nx in (200, 400):
+        p = po.make_params_1d(nx, recon=po.RECON_MUSCL_MINMOD, bc=po.BC1D_WALL, g=0.0, gamma=1.4,
+                              integrator=po.SSPRK3)
+        x = (np.arange(nx) + 0.5) / nx
+        w = po.equilibrium_1d(p, x)
+        ws = np.zeros((3, nx), order="F")
+        ws[0] = np.where(x < 0.5, 1.0, 0.125)
+        ws[2] = np.where(x < 0.5, 1.0, 0.1)
+        u, t, n, _ = po.fv1d_advance(p, po.conservative_1d(p.gamma, ws), w, t_end=0.2)
+        assert t == 0.2
+        exact, _, _ = exact_riemann_density(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4, (x - 0.5) / t)
+        errs.append(np.mean(np.abs(u[0] - exact)))
+    assert errs[0] < 0.02 and errs[1] < 0.7 * errs[0], errs
+
+
+@pytest.mark.parametrize("recon", (po.RECON_FIRST_ORDER, po.RECON_MUSCL_MINMOD))
+def test_fv1d_equals_2d_oracle_on_y_uniform_problem(recon):
+    """Without gravity, a 2D problem uniform in y with reflecting walls is the
+    1D problem in every row: the 1D oracle (Rusanov, SSP-RK2) and the 2D
+    oracle agree to rounding over 25 steps of the same dt."""
+    nx = 90
+    p1 = po.make_params_1d(nx, recon=recon, bc=po.BC1D_WALL, g=0.0, gamma=1.4, cfl=0.3)
+    p2 = po.make_params(nx, 4, recon=recon, flux=po.FLUX_RUSANOV, bc=po.BC_REFLECT, integrator=po.RK2_HEUN,
+                        gamma=1.4, rho0=1.0, p0=1.0, g=0.0, cfl=0.6)
+    u1, w1 = po.perturbed_state_1d(p1, amp_bump=0.2, x0=0.35, width=0.06, amp_noise=1e-3)
+    u2 = np.zeros((4, nx, 4), order="F")
+    w2 = np.zeros((4, nx, 4), order="F")
+    for j in range(4):
+        u2[[0, 1, 3], :, j] = u1
+        w2[[0, 1, 3], :, j] = w1
+    a, _, n1, dt1 = po.fv1d_advance(p1, u1, w1, max_steps=25)
+    b, _, n2, dt2 = po.fv2d_advance(p2, u2, w2, max_steps=25)
+    assert n1 == n2 == 25 and abs(dt1 - dt2) <= 1e-15 * dt1
+    for j in range(4):
+        np.testing.assert_allclose(b[[0, 1, 3], :, j], a, rtol=0, atol=1e-13)
+    assert np.max(np.abs(b[2])) < 1e-15
