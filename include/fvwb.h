@@ -261,6 +261,39 @@ int fvwb1d_download(fvwb1d_handle h, int32_t inst, double* u);
 int fvwb1d_get_time(fvwb1d_handle h, int32_t inst, double* t, double* dt_last, int32_t* steps_done);
 int fvwb1d_get_stream(fvwb1d_handle h, void** stream);
 
+/* ---- 2D modal DG in deviation form (config: P2, 3x3 Gauss-Legendre) -------
+ * Modal discontinuous Galerkin for the same 2D Euler-with-gravity problem as
+ * above (phi = g*(x+y), isothermal atmosphere).  The unknown is the deviation
+ * dU = U - U_eq expanded in the orthonormal Legendre basis of
+ * 2d/legendre.f90:1-50 (phi_n = sqrt(2n+1) P_n, so 1/4 of the integral of
+ * phi_i(xi)phi_j(eta)phi_k(xi)phi_l(eta) over [-1,1]^2 is delta_ik delta_jl),
+ * total degree <= k: modes (0,0) (1,0) (0,1) (2,0) (1,1) (0,2) in that order.
+ * Volume and source integrals on (k+1)^2 Gauss-Legendre points, face integrals
+ * on k+1 points per face (2d/benchmark_2d_dg.f90:1207-1244, 1372-1411,
+ * 1426-1462: dudt = (vol_x/dx + vol_y/dy - edges)/2 + src/4); the fluxes and
+ * source are deviations from the equilibrium's (dg_with_source.f90:1749-2031),
+ * so a resting atmosphere is an exact fixed point.  Host arrays of modes are
+ * the reference's layout delta_u(1:4,1:nx,1:ny,1:nmodes): variable fastest,
+ * mode index slowest. */
+#define FVWB_DG_LIMITER_NONE  0
+#define FVWB_DG_LIMITER_TVB   1   /* 2d/limiters.f90:30-54 generalized_minmod on the
+                                     linear modes of each conservative deviation;
+                                     a limited cell drops its quadratic modes   */
+
+typedef struct fvwb_dg_params {
+  int32_t nvar;        /* must be 4                                             */
+  int32_t nx, ny;      /* cells, >= 2                                           */
+  int32_t degree;      /* polynomial degree k (the GPU path implements k = 2)   */
+  int32_t flux;        /* FVWB_FLUX_RUSANOV / FVWB_FLUX_HLL                     */
+  int32_t limiter;     /* FVWB_DG_LIMITER_*                                     */
+  int32_t bc;          /* FVWB_BC_REFLECT (walls)                               */
+  int32_t reserved;    /* 0                                                     */
+  double  gamma, boxlen_x, boxlen_y;
+  double  cfl;         /* dt = cfl / ((2k+1) max((|u|+c)/dx + (|v|+c)/dy))      */
+  double  rho0, p0, g; /* equilibrium rho0*exp(-k(x+y)), k = rho0*g/p0          */
+  double  tvb_m;       /* TVB constant M: slopes below M*dx^2 are kept          */
+} fvwb_dg_params;
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,11 @@
  *   orc_ref1d_*  — restatements of benchmark_1d.f90 (the 'EQL' path), pinned
  *                  like orc_ref2d_*;
  *   orc_fv1d_*   — the configured 1D scheme (deviation form, MUSCL-minmod,
- *                  Rusanov, SSP-RK2/RK3, walls), pinned by invariants.
+ *                  Rusanov, SSP-RK2/RK3, walls), pinned by invariants;
+ *   orc_dg_*, orc_ref_dg2d_* — the reference's Legendre basis, Gauss-Legendre
+ *                  quadrature and tensor projections (2d/), pinned like above;
+ *   orc_dg2d_*   — the configured modal DG (deviation form, P_k, k <= 2),
+ *                  pinned by invariants.
  */
 #ifndef FVWB_ORACLE_H
 #define FVWB_ORACLE_H
@@ -98,6 +102,25 @@ void   orc_fv1d_residual(const fvwb1d_params* p, const double* u, const double* 
  * steps < max_steps */
 void   orc_fv1d_advance(const fvwb1d_params* p, double* u, const double* w_eq, double t_end,
                         int max_steps, double* t_out, int* steps_out, double* dt_last);
+
+/* ---- 2D modal DG ------------------------------------------------------------ */
+/* 2d/legendre.f90: legendre (:1-25), legendre_prime (:27-50), gl_quadrature (:77-108) */
+double orc_dg_legendre(double x, int n);
+double orc_dg_legendre_prime(double x, int n);
+void   orc_dg_gl_quadrature(int n, double* xq, double* wq);
+/* 2d/benchmark_2d_dg.f90:497-592 tensor-basis projections, AoS(4,nx,ny,order,order) */
+void   orc_ref_dg2d_modes_from_nodes(int nx, int ny, int order, const double* nodes, double* modes);
+void   orc_ref_dg2d_nodes_from_modes(int nx, int ny, int order, const double* modes, double* nodes);
+/* configured scheme; modes AoS(4,nx,ny,nmodes) of the deviation */
+int    orc_dg2d_nmodes(int degree);
+void   orc_dg2d_residual(const fvwb_dg_params* p, const double* d, double* L);
+void   orc_dg2d_limit(const fvwb_dg_params* p, double* d);
+double orc_dg2d_max_rate(const fvwb_dg_params* p, const double* d);
+void   orc_dg2d_advance(const fvwb_dg_params* p, double* d, double t_end, int max_steps, double* t_out,
+                        int* steps_out, double* dt_last);
+/* deviation at the (k+1)^2 volume points, AoS(4,nx,ny,(k+1)^2), point a + (k+1)*b */
+void   orc_dg2d_nodal(const fvwb_dg_params* p, const double* d, double* nodal);
+void   orc_dg2d_project(const fvwb_dg_params* p, const double* nodal, double* d);
 
 #ifdef __cplusplus
 }

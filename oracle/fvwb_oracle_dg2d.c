@@ -1,0 +1,410 @@
+/*
+ * fvwb_oracle_dg2d.c — CPU FP64 oracle, 2D modal DG (TEST INFRASTRUCTURE).
+ *
+ * Restatements of the reference's DG building blocks, pinned against the
+ * flang build of the reference: legendre / legendre_prime / gl_quadrature
+ * (2d/legendre.f90:1-108) and the tensor-basis projections
+ * get_modes_from_nodes / get_nodes_from_modes (2d/benchmark_2d_dg.f90:497-592).
+ *
+ * The configured scheme: modal DG of total degree k <= 2 in deviation form
+ * (the template of dg_with_source.f90:1749-2031 carried to 2D), Gauss-Legendre
+ * volume and face quadrature with the reference's assembly
+ * (2d/benchmark_2d_dg.f90:1455-1462), the configured FV face flux, walls, the
+ * TVB limiter of 2d/limiters.f90:30-54 on the linear modes, SSP-RK3.  Pinned by
+ * invariants in tests/ (exact rest, degree 0 == first-order FV, convergence
+ * orders, mass identity).  Built with -ffp-contract=off.  Not linked by the
+ * product.
+ */
+#include "fvwb_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- 2d/legendre.f90 ------------------------------------------------------- */
+
+/* legendre(x,n): sqrt(2n+1) P_n(x) with x clamped to [-1,1] (:1-25) */
+double orc_dg_legendre(double x, int n)
+{
+  x = fmin(fmax(x, -1.0), 1.0);
+  double x2 = x * x, l = 0.0;
+  switch (n) {
+  case 0: l = 1.0; break;
+  case 1: l = x; break;
+  case 2: l = 0.5 * (3 * x2 - 1); break;
+  case 3: l = 0.5 * (5.0 * (x2 * x) - 3.0 * x); break;
+  case 4: l = 0.125 * (35.0 * (x2 * x2) - 30.0 * x2 + 3.0); break;
+  case 5: l = 0.125 * (63.0 * (x2 * x2 * x) - 70.0 * (x2 * x) + 15.0 * x); break;
+  case 6: l = 1.0 / 16.0 * (231.0 * (x2 * x2 * x2) - 315.0 * (x2 * x2) + 105.0 * x2 - 5.0); break;
+  default: l = 0.0;
+  }
+  return sqrt(2.0 * (double)n + 1.0) * l;
+}
+
+/* legendre_prime(x,n) (:27-50) */
+double orc_dg_legendre_prime(double x, int n)
+{
+  x = fmin(fmax(x, -1.0), 1.0);
+  double x2 = x * x, l = 0.0;
+  switch (n) {
+  case 0: l = 0.0; break;
+  case 1: l = 1.0; break;
+  case 2: l = 3.0 * x; break;
+  case 3: l = 0.5 * (15.0 * x2 - 3.0); break;
+  case 4: l = 0.125 * (140.0 * (x2 * x) - 60.0 * x); break;
+  case 5: l = 0.125 * (315.0 * (x2 * x2) - 210.0 * x2 + 15.0); break;
+  case 6: l = 1.0 / 16.0 * (1386.0 * (x2 * x2 * x) - 1260.0 * (x2 * x) + 210.0 * x); break;
+  default: l = 0.0;
+  }
+  return sqrt(2.0 * (double)n + 1.0) * l;
+}
+
+/* gl_quadrature(x_quad,w_quad,n) (:77-108): Newton from the classical
+ * initial guess (its coefficient evaluated in default REAL), 500 steps,
+ * nodes sign-flipped, weights 2(2n+1)/((1-x^2) L'(x)^2), then mirrored. */
+void orc_dg_gl_quadrature(int n, double* xq, double* wq)
+{
+  const double dpi = acos(-1.0);
+  for (int i = 1; i <= n; ++i) {
+    float nf = (float)n;
+    float c = (1.0f - 0.125f / nf / nf) + 0.125f / nf / nf / nf;
+    double xx = (double)c * cos(dpi * (4.0 * (double)i - 1.0) / (4.0 * (double)n + 2.0));
+    for (int it = 0; it < 500; ++it) xx = xx - orc_dg_legendre(xx, n) / orc_dg_legendre_prime(xx, n);
+    xq[i - 1] = -xx;
+    double lp = orc_dg_legendre_prime(xq[i - 1], n);
+    wq[i - 1] = 2 * (2.0 * (double)n + 1.0) / (1.0 - xq[i - 1] * xq[i - 1]) / (lp * lp);
+  }
+  for (int i = n / 2 + 1; i <= n; ++i) {
+    xq[i - 1] = -xq[n - i];
+    wq[i - 1] = wq[n - i];
+  }
+}
+
+/* ---- 2d/benchmark_2d_dg.f90:497-592, tensor (Q) basis ------------------------ */
+/* layout AoS(4, nx, ny, order, order): element (v,i,j,a,b) */
+static inline size_t q5(int v, int i, int j, int a, int b, int nx, int ny, int o)
+{
+  return (size_t)v + 4u * ((size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * ((size_t)a + (size_t)o * (size_t)b)));
+}
+#define Q5(arr, v, i, j, a, b, nx, ny, o) ((arr)[q5((v), (i), (j), (a), (b), (nx), (ny), (o))])
+
+void orc_ref_dg2d_modes_from_nodes(int nx, int ny, int order, const double* nodes, double* modes)
+{
+  double xq[8], wq[8];
+  orc_dg_gl_quadrature(order, xq, wq);
+  memset(modes, 0, sizeof(double) * 4 * (size_t)nx * ny * order * order);
+  for (int ic = 0; ic < nx; ++ic)
+    for (int jc = 0; jc < ny; ++jc)
+      for (int i = 0; i < order; ++i)
+        for (int j = 0; j < order; ++j)
+          for (int a = 0; a < order; ++a)
+            for (int b = 0; b < order; ++b)
+              for (int v = 0; v < 4; ++v)
+                Q5(modes, v, ic, jc, i, j, nx, ny, order) =
+                    Q5(modes, v, ic, jc, i, j, nx, ny, order) +
+                    0.25 * Q5(nodes, v, ic, jc, a, b, nx, ny, order) * orc_dg_legendre(xq[a], i) *
+                        orc_dg_legendre(xq[b], j) * wq[a] * wq[b];
+}
+
+void orc_ref_dg2d_nodes_from_modes(int nx, int ny, int order, const double* modes, double* nodes)
+{
+  double xq[8], wq[8];
+  orc_dg_gl_quadrature(order, xq, wq);
+  memset(nodes, 0, sizeof(double) * 4 * (size_t)nx * ny * order * order);
+  for (int v = 0; v < 4; ++v)
+    for (int ic = 0; ic < nx; ++ic)
+      for (int jc = 0; jc < ny; ++jc)
+        for (int i = 0; i < order; ++i)
+          for (int j = 0; j < order; ++j)
+            for (int a = 0; a < order; ++a)
+              for (int b = 0; b < order; ++b)
+                Q5(nodes, v, ic, jc, i, j, nx, ny, order) =
+                    Q5(nodes, v, ic, jc, i, j, nx, ny, order) +
+                    Q5(modes, v, ic, jc, a, b, nx, ny, order) * orc_dg_legendre(xq[i], a) *
+                        orc_dg_legendre(xq[j], b);
+}
+
+/* ---- configured scheme ------------------------------------------------------ */
+
+typedef struct {
+  int k, nm, nq;
+  int mi[6], mj[6];
+  double xq[3], wq[3];
+  double phi[3][3];   /* phi[n][a] = L_n(xq[a]) */
+  double dphi[3][3];  /* L_n'(xq[a]) */
+  double phim[3], phip[3];  /* L_n(-1), L_n(+1) */
+} dg_basis;
+
+static const int MODE_I[6] = {0, 1, 0, 2, 1, 0};
+static const int MODE_J[6] = {0, 0, 1, 0, 1, 2};
+
+int orc_dg2d_nmodes(int degree) { return (degree + 1) * (degree + 2) / 2; }
+
+static void make_basis(int k, dg_basis* B)
+{
+  B->k = k;
+  B->nm = orc_dg2d_nmodes(k);
+  B->nq = k + 1;
+  for (int m = 0; m < B->nm; ++m) { B->mi[m] = MODE_I[m]; B->mj[m] = MODE_J[m]; }
+  orc_dg_gl_quadrature(B->nq, B->xq, B->wq);
+  for (int n = 0; n <= k; ++n) {
+    for (int a = 0; a < B->nq; ++a) {
+      B->phi[n][a] = orc_dg_legendre(B->xq[a], n);
+      B->dphi[n][a] = orc_dg_legendre_prime(B->xq[a], n);
+    }
+    B->phim[n] = orc_dg_legendre(-1.0, n);
+    B->phip[n] = orc_dg_legendre(1.0, n);
+  }
+}
+
+/* host layout of modes: AoS(4, nx, ny, nm), element (v,i,j,m) */
+#define DM(arr, v, i, j, m, nx, ny) \
+  ((arr)[(size_t)(v) + 4u * ((size_t)(i) + (size_t)(nx) * ((size_t)(j) + (size_t)(ny) * (size_t)(m)))])
+
+/* the FV parameter block that carries the face flux settings */
+static fvwb_params fv_of(const fvwb_dg_params* p)
+{
+  fvwb_params q;
+  memset(&q, 0, sizeof q);
+  q.nvar = 4; q.nx = p->nx; q.ny = p->ny;
+  q.bc = FVWB_BC_REFLECT; q.recon = FVWB_RECON_FIRST_ORDER; q.flux = p->flux;
+  q.integrator = FVWB_SSPRK3; q.precision = FVWB_FP64;
+  q.gamma = p->gamma; q.boxlen_x = p->boxlen_x; q.boxlen_y = p->boxlen_y; q.cfl = p->cfl;
+  q.rho0 = p->rho0; q.p0 = p->p0; q.g = p->g;
+  return q;
+}
+
+/* conservative equilibrium at rest at (x,y) */
+static void ueq_at(const fvwb_dg_params* p, double x, double y, double u[4])
+{
+  const double kk = p->rho0 * p->g / p->p0;
+  const double e = exp(-(kk * (x + y)));
+  u[0] = p->rho0 * e;
+  u[1] = 0.0;
+  u[2] = 0.0;
+  u[3] = p->p0 * e / (p->gamma - 1.0);
+}
+
+/* deviation of cell (i,j) at (xi, eta) given as basis values per direction */
+static void eval_dev(const double* d, int i, int j, int nx, int ny, const dg_basis* B, const double* lx,
+                     const double* ly, double out[4])
+{
+  for (int v = 0; v < 4; ++v) {
+    double s = 0.0;
+    for (int m = 0; m < B->nm; ++m) s += DM(d, v, i, j, m, nx, ny) * (lx[B->mi[m]] * ly[B->mj[m]]);
+    out[v] = s;
+  }
+}
+
+void orc_dg2d_residual(const fvwb_dg_params* p, const double* d, double* L)
+{
+  const int nx = p->nx, ny = p->ny;
+  dg_basis B;
+  make_basis(p->degree, &B);
+  const int nm = B.nm, nq = B.nq;
+  const double dx = p->boxlen_x / nx, dy = p->boxlen_y / ny;
+  const double odx = 1.0 / dx, ody = 1.0 / dy;
+  const fvwb_params fv = fv_of(p);
+  memset(L, 0, sizeof(double) * 4 * (size_t)nx * ny * nm);
+  double lx[3], ly[3];
+
+  /* volume and source integrals */
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const double xc = (i + 0.5) * dx, yc = (j + 0.5) * dy;
+      for (int b = 0; b < nq; ++b)
+        for (int a = 0; a < nq; ++a) {
+          for (int n = 0; n <= B.k; ++n) { lx[n] = B.phi[n][a]; ly[n] = B.phi[n][b]; }
+          double dv[4], ue[4], uu[4], fx[4], fy[4], fxe[4], fye[4];
+          eval_dev(d, i, j, nx, ny, &B, lx, ly, dv);
+          ueq_at(p, xc + 0.5 * dx * B.xq[a], yc + 0.5 * dy * B.xq[b], ue);
+          for (int v = 0; v < 4; ++v) uu[v] = ue[v] + dv[v];
+          orc_ref_flux(p->gamma, uu, fx, fy);
+          orc_ref_flux(p->gamma, ue, fxe, fye);
+          const double s[4] = {0.0, -p->g * dv[0], -p->g * dv[0], -p->g * (dv[1] + dv[2])};
+          const double w = B.wq[a] * B.wq[b];
+          for (int m = 0; m < nm; ++m) {
+            const int mi = B.mi[m], mj = B.mj[m];
+            const double gx = 0.5 * w * B.dphi[mi][a] * B.phi[mj][b] * odx;
+            const double gy = 0.5 * w * B.phi[mi][a] * B.dphi[mj][b] * ody;
+            const double gs = 0.25 * w * B.phi[mi][a] * B.phi[mj][b];
+            for (int v = 0; v < 4; ++v)
+              DM(L, v, i, j, m, nx, ny) += gx * (fx[v] - fxe[v]) + gy * (fy[v] - fye[v]) + gs * s[v];
+          }
+        }
+    }
+
+  /* x faces: face fi between cells fi-1 and fi */
+  for (int j = 0; j < ny; ++j)
+    for (int fi = 0; fi <= nx; ++fi)
+      for (int b = 0; b < nq; ++b) {
+        double dl[4], dr[4], ue[4], f[4];
+        for (int n = 0; n <= B.k; ++n) ly[n] = B.phi[n][b];
+        if (fi > 0) eval_dev(d, fi - 1, j, nx, ny, &B, B.phip, ly, dl);
+        if (fi < nx) eval_dev(d, fi, j, nx, ny, &B, B.phim, ly, dr);
+        if (fi == 0) { memcpy(dl, dr, sizeof dl); dl[1] = -dl[1]; }
+        if (fi == nx) { memcpy(dr, dl, sizeof dr); dr[1] = -dr[1]; }
+        ueq_at(p, fi * dx, (j + 0.5) * dy + 0.5 * dy * B.xq[b], ue);
+        orc_fv2d_face_flux(&fv, 0, ue, dl, dr, f);
+        const double w = 0.5 * B.wq[b] * odx;
+        for (int m = 0; m < nm; ++m) {
+          const double pj = B.phi[B.mj[m]][b];
+          for (int v = 0; v < 4; ++v) {
+            if (fi > 0) DM(L, v, fi - 1, j, m, nx, ny) -= w * f[v] * B.phip[B.mi[m]] * pj;
+            if (fi < nx) DM(L, v, fi, j, m, nx, ny) += w * f[v] * B.phim[B.mi[m]] * pj;
+          }
+        }
+      }
+
+  /* y faces: face fj between cells fj-1 and fj */
+  for (int fj = 0; fj <= ny; ++fj)
+    for (int i = 0; i < nx; ++i)
+      for (int a = 0; a < nq; ++a) {
+        double dl[4], dr[4], ue[4], f[4];
+        for (int n = 0; n <= B.k; ++n) lx[n] = B.phi[n][a];
+        if (fj > 0) eval_dev(d, i, fj - 1, nx, ny, &B, lx, B.phip, dl);
+        if (fj < ny) eval_dev(d, i, fj, nx, ny, &B, lx, B.phim, dr);
+        if (fj == 0) { memcpy(dl, dr, sizeof dl); dl[2] = -dl[2]; }
+        if (fj == ny) { memcpy(dr, dl, sizeof dr); dr[2] = -dr[2]; }
+        ueq_at(p, (i + 0.5) * dx + 0.5 * dx * B.xq[a], fj * dy, ue);
+        orc_fv2d_face_flux(&fv, 1, ue, dl, dr, f);
+        const double w = 0.5 * B.wq[a] * ody;
+        for (int m = 0; m < nm; ++m) {
+          const double pi_ = B.phi[B.mi[m]][a];
+          for (int v = 0; v < 4; ++v) {
+            if (fj > 0) DM(L, v, i, fj - 1, m, nx, ny) -= w * f[v] * pi_ * B.phip[B.mj[m]];
+            if (fj < ny) DM(L, v, i, fj, m, nx, ny) += w * f[v] * pi_ * B.phim[B.mj[m]];
+          }
+        }
+      }
+}
+
+/* TVB limiter on the linear modes of each conservative deviation; the means
+ * are never changed, so cells can be limited in any order. */
+void orc_dg2d_limit(const fvwb_dg_params* p, double* d)
+{
+  if (p->limiter != FVWB_DG_LIMITER_TVB || p->degree < 1) return;
+  const int nx = p->nx, ny = p->ny, nm = orc_dg2d_nmodes(p->degree);
+  const double dx = p->boxlen_x / nx, dy = p->boxlen_y / ny;
+  const double s3 = sqrt(3.0);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i)
+      for (int v = 0; v < 4; ++v) {
+        const double u0 = DM(d, v, i, j, 0, nx, ny);
+        /* neighbour means; walls mirror the cell, normal momentum negated */
+        const double uw = i > 0 ? DM(d, v, i - 1, j, 0, nx, ny) : (v == 1 ? -u0 : u0);
+        const double ue = i < nx - 1 ? DM(d, v, i + 1, j, 0, nx, ny) : (v == 1 ? -u0 : u0);
+        const double us = j > 0 ? DM(d, v, i, j - 1, 0, nx, ny) : (v == 2 ? -u0 : u0);
+        const double un = j < ny - 1 ? DM(d, v, i, j + 1, 0, nx, ny) : (v == 2 ? -u0 : u0);
+        const double a = s3 * DM(d, v, i, j, 1, nx, ny);
+        const double b = s3 * DM(d, v, i, j, 2, nx, ny);
+        const double at = orc_generalized_minmod(a, ue - u0, u0 - uw, p->tvb_m, dx);
+        const double bt = orc_generalized_minmod(b, un - u0, u0 - us, p->tvb_m, dy);
+        if (at != a || bt != b) {
+          DM(d, v, i, j, 1, nx, ny) = at / s3;
+          DM(d, v, i, j, 2, nx, ny) = bt / s3;
+          for (int m = 3; m < nm; ++m) DM(d, v, i, j, m, nx, ny) = 0.0;
+        }
+      }
+}
+
+/* max over cells and volume points of (|u|+c)/dx + (|v|+c)/dy of U_eq + dU */
+double orc_dg2d_max_rate(const fvwb_dg_params* p, const double* d)
+{
+  const int nx = p->nx, ny = p->ny;
+  dg_basis B;
+  make_basis(p->degree, &B);
+  const double dx = p->boxlen_x / nx, dy = p->boxlen_y / ny;
+  double r = 0.0, lx[3], ly[3];
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i)
+      for (int b = 0; b < B.nq; ++b)
+        for (int a = 0; a < B.nq; ++a) {
+          for (int n = 0; n <= B.k; ++n) { lx[n] = B.phi[n][a]; ly[n] = B.phi[n][b]; }
+          double dv[4], ue[4], u[4], w[4];
+          eval_dev(d, i, j, nx, ny, &B, lx, ly, dv);
+          ueq_at(p, (i + 0.5) * dx + 0.5 * dx * B.xq[a], (j + 0.5) * dy + 0.5 * dy * B.xq[b], ue);
+          for (int v = 0; v < 4; ++v) u[v] = ue[v] + dv[v];
+          orc_ref_primitive(p->gamma, u, w);
+          const double c = sqrt(p->gamma * fmax(w[3], 1e-10) / fmax(w[0], 1e-10));
+          r = fmax(r, (fabs(w[1]) + c) / dx + (fabs(w[2]) + c) / dy);
+        }
+  return r;
+}
+
+/* SSP-RK3 with the limiter after every stage; the state is taken as limited
+ * on entry.  dt = cfl/((2k+1) max rate), the last step clipped to t_end. */
+void orc_dg2d_advance(const fvwb_dg_params* p, double* d, double t_end, int max_steps, double* t_out,
+                      int* steps_out, double* dt_last)
+{
+  const size_t n = 4u * (size_t)p->nx * p->ny * orc_dg2d_nmodes(p->degree);
+  double* L = malloc(sizeof(double) * n);
+  double* d1 = malloc(sizeof(double) * n);
+  double* d2 = malloc(sizeof(double) * n);
+  double t = 0.0, dt = 0.0;
+  int iter = 0;
+  while (t < t_end && iter < max_steps) {
+    const double rate = orc_dg2d_max_rate(p, d);
+    dt = p->cfl / ((2.0 * p->degree + 1.0) * rate);
+    if (t + dt > t_end) dt = t_end - t;
+    orc_dg2d_residual(p, d, L);
+    for (size_t q = 0; q < n; ++q) d1[q] = d[q] + dt * L[q];
+    orc_dg2d_limit(p, d1);
+    orc_dg2d_residual(p, d1, L);
+    for (size_t q = 0; q < n; ++q) d2[q] = 0.75 * d[q] + 0.25 * d1[q] + 0.25 * dt * L[q];
+    orc_dg2d_limit(p, d2);
+    orc_dg2d_residual(p, d2, L);
+    for (size_t q = 0; q < n; ++q) d[q] = (1.0 / 3.0) * d[q] + (2.0 / 3.0) * d2[q] + (2.0 / 3.0) * dt * L[q];
+    orc_dg2d_limit(p, d);
+    t = t + dt;
+    iter = iter + 1;
+  }
+  if (t_out) *t_out = t;
+  if (steps_out) *steps_out = iter;
+  if (dt_last) *dt_last = dt;
+  free(L); free(d1); free(d2);
+}
+
+/* values of the deviation at the (k+1)^2 volume points, AoS(4,nx,ny,nq*nq)
+ * with point index a + nq*b */
+void orc_dg2d_nodal(const fvwb_dg_params* p, const double* d, double* nodal)
+{
+  const int nx = p->nx, ny = p->ny;
+  dg_basis B;
+  make_basis(p->degree, &B);
+  const int nq = B.nq, np = nq * nq;
+  double lx[3], ly[3];
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i)
+      for (int b = 0; b < nq; ++b)
+        for (int a = 0; a < nq; ++a) {
+          for (int n = 0; n <= B.k; ++n) { lx[n] = B.phi[n][a]; ly[n] = B.phi[n][b]; }
+          double dv[4];
+          eval_dev(d, i, j, nx, ny, &B, lx, ly, dv);
+          for (int v = 0; v < 4; ++v)
+            nodal[(size_t)v + 4u * ((size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)(a + nq * b)))] = dv[v];
+        }
+  (void)np;
+}
+
+/* L2 projection of nodal values (layout as orc_dg2d_nodal) onto the modes */
+void orc_dg2d_project(const fvwb_dg_params* p, const double* nodal, double* d)
+{
+  const int nx = p->nx, ny = p->ny;
+  dg_basis B;
+  make_basis(p->degree, &B);
+  const int nq = B.nq;
+  memset(d, 0, sizeof(double) * 4 * (size_t)nx * ny * B.nm);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i)
+      for (int b = 0; b < nq; ++b)
+        for (int a = 0; a < nq; ++a) {
+          const double w = 0.25 * B.wq[a] * B.wq[b];
+          for (int m = 0; m < B.nm; ++m) {
+            const double ph = B.phi[B.mi[m]][a] * B.phi[B.mj[m]][b];
+            for (int v = 0; v < 4; ++v)
+              DM(d, v, i, j, m, nx, ny) +=
+                  w * ph * nodal[(size_t)v + 4u * ((size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)(a + nq * b)))];
+          }
+        }
+}

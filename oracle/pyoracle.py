@@ -61,6 +61,28 @@ def make_params_1d(nx, *, bc=BC1D_ZERO_GRADIENT, recon=RECON_FIRST_ORDER, integr
     return Params1D(3, nx, bc, recon, integrator, nequilibrium, gamma, boxlen, cfl, g)
 
 
+DG_LIMITER_NONE, DG_LIMITER_TVB = 0, 1
+
+
+class DGParams(ctypes.Structure):
+    """Mirror of ``fvwb_dg_params`` (include/fvwb.h)."""
+
+    _fields_ = [
+        ("nvar", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("degree", ctypes.c_int32), ("flux", ctypes.c_int32), ("limiter", ctypes.c_int32),
+        ("bc", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("boxlen_x", ctypes.c_double), ("boxlen_y", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("rho0", ctypes.c_double), ("p0", ctypes.c_double),
+        ("g", ctypes.c_double), ("tvb_m", ctypes.c_double),
+    ]
+
+
+def make_dg_params(nx, ny, *, degree=2, flux=FLUX_HLL, limiter=DG_LIMITER_TVB, gamma=1.4, boxlen_x=1.0,
+                   boxlen_y=1.0, cfl=0.8, rho0=1.0, p0=1.0, g=1.0, tvb_m=50.0) -> DGParams:
+    return DGParams(4, nx, ny, degree, flux, limiter, BC_REFLECT, 0, gamma, boxlen_x, boxlen_y, cfl, rho0, p0,
+                    g, tvb_m)
+
+
 def make_params(nx, ny, *, bc=BC_FROZEN_RING, recon=RECON_FIRST_ORDER, flux=FLUX_RUSANOV,
                 integrator=RK2_HEUN, precision=FP64, gamma=GAMMA_REF, boxlen_x=1.0,
                 boxlen_y=1.0, cfl=0.5, rho0=RHO0_REF, p0=1.0, g=1.0) -> Params:
@@ -116,6 +138,22 @@ def lib():
         L.orc_fv1d_equilibrium.argtypes = [p1, ctypes.c_double, dp]
         L.orc_fv1d_advance.argtypes = [p1, dp, dp, ctypes.c_double, ctypes.c_int, dp,
                                        ctypes.POINTER(ctypes.c_int), dp]
+        pd = ctypes.POINTER(DGParams)
+        L.orc_dg_legendre.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_dg_legendre.restype = ctypes.c_double
+        L.orc_dg_legendre_prime.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_dg_legendre_prime.restype = ctypes.c_double
+        L.orc_dg_gl_quadrature.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_ref_dg2d_modes_from_nodes.argtypes = [ctypes.c_int] * 3 + [dp, dp]
+        L.orc_ref_dg2d_nodes_from_modes.argtypes = [ctypes.c_int] * 3 + [dp, dp]
+        L.orc_dg2d_nmodes.argtypes = [ctypes.c_int]
+        L.orc_dg2d_residual.argtypes = [pd, dp, dp]
+        L.orc_dg2d_limit.argtypes = [pd, dp]
+        L.orc_dg2d_max_rate.argtypes = [pd, dp]
+        L.orc_dg2d_max_rate.restype = ctypes.c_double
+        L.orc_dg2d_advance.argtypes = [pd, dp, ctypes.c_double, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_int), dp]
+        L.orc_dg2d_nodal.argtypes = [pd, dp, dp]
+        L.orc_dg2d_project.argtypes = [pd, dp, dp]
         _lib = L
     return _lib
 
@@ -356,3 +394,121 @@ def perturbed_state_1d(p: Params1D, amp_bump=1e-3, x0=0.5, width=0.05, amp_noise
             scale = u[0] if v < 2 else u[2]
             u[v] += amp_noise * scale * hash_noise(nx, 1, salt * 3 + v)[:, 0]
     return u, w
+
+
+# ---- 2D modal DG ---------------------------------------------------------------------
+def dg_legendre(x, n):
+    return lib().orc_dg_legendre(float(x), int(n))
+
+
+def dg_legendre_prime(x, n):
+    return lib().orc_dg_legendre_prime(float(x), int(n))
+
+
+def dg_gl_quadrature(n):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orc_dg_gl_quadrature(n, _p(x), _p(w))
+    return x, w
+
+
+def ref_dg2d_modes_from_nodes(nodes):
+    nodes = fortran(nodes)
+    _, nx, ny, o, _ = nodes.shape
+    out = np.zeros_like(nodes, order="F")
+    lib().orc_ref_dg2d_modes_from_nodes(nx, ny, o, _p(nodes), _p(out))
+    return out
+
+
+def ref_dg2d_nodes_from_modes(modes):
+    modes = fortran(modes)
+    _, nx, ny, o, _ = modes.shape
+    out = np.zeros_like(modes, order="F")
+    lib().orc_ref_dg2d_nodes_from_modes(nx, ny, o, _p(modes), _p(out))
+    return out
+
+
+def dg_nmodes(degree):
+    return (degree + 1) * (degree + 2) // 2
+
+
+def dg2d_residual(p: DGParams, d):
+    d = fortran(d)
+    out = np.zeros_like(d, order="F")
+    lib().orc_dg2d_residual(ctypes.byref(p), _p(d), _p(out))
+    return out
+
+
+def dg2d_limit(p: DGParams, d):
+    d = fortran(d).copy(order="F")
+    lib().orc_dg2d_limit(ctypes.byref(p), _p(d))
+    return d
+
+
+def dg2d_max_rate(p: DGParams, d):
+    return lib().orc_dg2d_max_rate(ctypes.byref(p), _p(fortran(d)))
+
+
+def dg2d_advance(p: DGParams, d, t_end=float("inf"), max_steps=1 << 30):
+    d = fortran(d).copy(order="F")
+    t, dt = ctypes.c_double(), ctypes.c_double()
+    n = ctypes.c_int()
+    lib().orc_dg2d_advance(ctypes.byref(p), _p(d), t_end, max_steps, ctypes.byref(t), ctypes.byref(n),
+                           ctypes.byref(dt))
+    return d, t.value, n.value, dt.value
+
+
+def dg2d_nodal(p: DGParams, d):
+    d = fortran(d)
+    nq = p.degree + 1
+    out = np.zeros((4, p.nx, p.ny, nq * nq), order="F")
+    lib().orc_dg2d_nodal(ctypes.byref(p), _p(d), _p(out))
+    return out
+
+
+def dg2d_project(p: DGParams, nodal):
+    nodal = fortran(nodal)
+    out = np.zeros((4, p.nx, p.ny, dg_nmodes(p.degree)), order="F")
+    lib().orc_dg2d_project(ctypes.byref(p), _p(nodal), _p(out))
+    return out
+
+
+def dg2d_points(p: DGParams):
+    """Physical coordinates (X, Y) of the volume points, shape (nx, ny, nq*nq)."""
+    nq = p.degree + 1
+    xq, _ = dg_gl_quadrature(nq)
+    dx, dy = p.boxlen_x / p.nx, p.boxlen_y / p.ny
+    xc = (np.arange(p.nx) + 0.5) * dx
+    yc = (np.arange(p.ny) + 0.5) * dy
+    X = np.zeros((p.nx, p.ny, nq * nq))
+    Y = np.zeros((p.nx, p.ny, nq * nq))
+    for b in range(nq):
+        for a in range(nq):
+            X[:, :, a + nq * b] = (xc + 0.5 * dx * xq[a])[:, None]
+            Y[:, :, a + nq * b] = (yc + 0.5 * dy * xq[b])[None, :]
+    return X, Y
+
+
+def fourier_modes(n=16, seed=0xF00D, mmax=8):
+    """(a_k, m_k, n_k, theta_k) of the DG perturbation: a ~ U[0.5,1], (m,n) in
+    {1..mmax}^2, theta ~ U[0, 2 pi), from numpy's PCG64 seeded with `seed`."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.5, 1.0, n)
+    mk = rng.integers(1, mmax + 1, n)
+    nk = rng.integers(1, mmax + 1, n)
+    th = rng.uniform(0.0, 2 * np.pi, n)
+    return a, mk, nk, th
+
+
+def dg2d_fourier_state(p: DGParams, amp=1e-4, modes=None):
+    """Deviation modes of a pressure perturbation
+    dp = amp * sum_k a_k cos(2 pi (m_k x + n_k y) + theta_k) / sum_k |a_k|
+    (dE = dp/(gamma-1), other deviations 0), projected on P_k."""
+    a, mk, nk, th = fourier_modes() if modes is None else modes
+    X, Y = dg2d_points(p)
+    dp = np.zeros_like(X)
+    for q in range(len(a)):
+        dp += a[q] * np.cos(2 * np.pi * (mk[q] * X + nk[q] * Y) + th[q])
+    dp *= amp / np.sum(np.abs(a))
+    nodal = np.zeros((4, p.nx, p.ny, X.shape[2]), order="F")
+    nodal[3] = dp / (p.gamma - 1.0)
+    return dg2d_project(p, nodal)

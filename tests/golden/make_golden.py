@@ -142,6 +142,11 @@ def main():
     f1 = rr.run("bench1d_llflux", ul=ul, ur=ur, fl=fl1, fr=fr1)["f"]
     out["bench1d_llflux"] = {"ul": ul, "ur": ur, "fl": fl1, "fr": fr1, "f": f1, "gamma": np.array(p1.gamma)}
 
+    # A17 tensor-basis projections of the reference's 2D DG (8x8 cells, order 2)
+    a = np.asfortranarray(np.random.default_rng(17).normal(size=(4, 8, 8, 2, 2)))
+    out["dg2d_projections"] = {"a": a, "modes_from_nodes": rr.run("dg2d_modes_from_nodes", a=a)["r"],
+                               "nodes_from_modes": rr.run("dg2d_nodes_from_modes", a=a)["r"]}
+
     for name, arrs in out.items():
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrs.items()})

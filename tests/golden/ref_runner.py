@@ -153,6 +153,14 @@ def _child(op, fin, fout):
                     dP[q, n] = L.legendre_prime_(ctypes.byref(a), ctypes.byref(nn))
             out["P"] = P
             out["dP"] = dP
+        elif op in ("dg2d_modes_from_nodes", "dg2d_nodes_from_modes"):
+            a = np.asfortranarray(z["a"])
+            _, nx, ny, o, _ = a.shape
+            res = np.zeros_like(a, order="F")
+            fn = L.get_modes_from_nodes_ if op == "dg2d_modes_from_nodes" else L.get_nodes_from_modes_
+            sx, sy, ox, oy = (ctypes.c_int(v) for v in (nx, ny, o, o))
+            fn(_dp(a), _dp(res), ctypes.byref(sx), ctypes.byref(sy), ctypes.byref(ox), ctypes.byref(oy))
+            out["r"] = res
         elif op == "dg2d_gl_quadrature":
             res = {}
             for n in range(1, 7):

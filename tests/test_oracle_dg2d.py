@@ -1,0 +1,174 @@
+"""The 2D modal DG oracle.  The reference's building blocks (Legendre basis,
+Gauss-Legendre quadrature, tensor projections of 2d/) against the fixtures
+written from the flang-built reference, bit for bit; the configured P_k
+deviation-form scheme against what the mathematics fixes: closed-form nodes
+and weights, the mass identity, exact rest, degree 0 equal to the first-order
+FV oracle, convergence at order k+1, wall mass conservation, x<->y symmetry,
+and a TVB limiter that keeps means and leaves smooth data alone."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import ref_runner as rr
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with np.load(os.path.join(GOLD, name + ".npz")) as z:
+        return {k: z[k] for k in z.files}
+
+
+# ---- the reference's building blocks -------------------------------------------------
+def test_legendre_matches_reference_golden_bitwise():
+    g = load("dg2d_legendre")
+    P = np.array([[po.dg_legendre(x, n) for n in range(7)] for x in g["x"]])
+    dP = np.array([[po.dg_legendre_prime(x, n) for n in range(7)] for x in g["x"]])
+    np.testing.assert_array_equal(P, g["P"])
+    np.testing.assert_array_equal(dP, g["dP"])
+    # 2d/legendre.f90:5 clamps the argument: P_n(1.5) == P_n(1) = sqrt(2n+1)
+    np.testing.assert_allclose(P[-1], np.sqrt(2 * np.arange(7) + 1.0), rtol=1e-15)
+
+
+def test_gl_quadrature_matches_reference_golden_and_closed_forms():
+    g = load("dg2d_gl_quadrature")
+    for n in range(1, 7):
+        x, w = po.dg_gl_quadrature(n)
+        np.testing.assert_array_equal(x, g[f"x{n}"])
+        np.testing.assert_array_equal(w, g[f"w{n}"])
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        np.testing.assert_allclose(np.sort(x), xr, atol=2e-15)
+        np.testing.assert_allclose(w[np.argsort(x)], wr, atol=2e-15)
+    x, w = po.dg_gl_quadrature(3)
+    np.testing.assert_allclose(sorted(x), [-np.sqrt(0.6), 0.0, np.sqrt(0.6)], atol=1e-15)
+    np.testing.assert_allclose(sorted(w), [5 / 9, 5 / 9, 8 / 9], atol=1e-15)
+
+
+def test_tensor_projections_match_reference_golden_bitwise():
+    g = load("dg2d_projections")
+    np.testing.assert_array_equal(po.ref_dg2d_modes_from_nodes(g["a"]), g["modes_from_nodes"])
+    np.testing.assert_array_equal(po.ref_dg2d_nodes_from_modes(g["a"]), g["nodes_from_modes"])
+    # with as many nodes as modes the two are inverse to each other
+    back = po.ref_dg2d_modes_from_nodes(po.ref_dg2d_nodes_from_modes(g["a"]))
+    np.testing.assert_allclose(back, g["a"], atol=1e-14)
+
+
+@pytest.mark.skipif(not rr.available("dg2d"), reason="oracle/_ref not built")
+def test_tensor_projections_match_live_reference():
+    a = np.asfortranarray(np.random.default_rng(5).normal(size=(4, 8, 8, 2, 2)))
+    np.testing.assert_array_equal(po.ref_dg2d_modes_from_nodes(a), rr.run("dg2d_modes_from_nodes", a=a)["r"])
+    np.testing.assert_array_equal(po.ref_dg2d_nodes_from_modes(a), rr.run("dg2d_nodes_from_modes", a=a)["r"])
+
+
+# ---- the configured scheme -------------------------------------------------------------
+def rand_dev(p, scale=1e-3, seed=0):
+    rng = np.random.default_rng(seed)
+    return np.asfortranarray(scale * rng.normal(size=(4, p.nx, p.ny, po.dg_nmodes(p.degree))))
+
+
+@pytest.mark.parametrize("degree", (0, 1, 2))
+def test_projection_of_nodal_values_is_identity(degree):
+    """(1/4) sum_q w_q phi_m phi_n = delta_mn on (k+1)^2 Gauss points."""
+    p = po.make_dg_params(5, 4, degree=degree)
+    d = rand_dev(p, 1.0)
+    np.testing.assert_allclose(po.dg2d_project(p, po.dg2d_nodal(p, d)), d, atol=2e-15)
+
+
+@pytest.mark.parametrize("degree", (0, 1, 2))
+@pytest.mark.parametrize("flux", (po.FLUX_RUSANOV, po.FLUX_HLL))
+def test_rest_is_exact(degree, flux):
+    p = po.make_dg_params(7, 6, degree=degree, flux=flux, g=1.7, rho0=1.2, p0=0.9)
+    d = np.zeros((4, 7, 6, po.dg_nmodes(degree)), order="F")
+    assert np.count_nonzero(po.dg2d_residual(p, d)) == 0
+    d2, t, n, _ = po.dg2d_advance(p, d, max_steps=5)
+    assert n == 5 and np.count_nonzero(d2) == 0
+
+
+@pytest.mark.parametrize("flux", (po.FLUX_RUSANOV, po.FLUX_HLL))
+def test_degree_zero_is_first_order_fv(flux):
+    """P0 with one Gauss point per direction is the first-order FV scheme with
+    walls: the same face fluxes at the face midpoints and the same source."""
+    nx, ny = 9, 7
+    pd = po.make_dg_params(nx, ny, degree=0, flux=flux, g=1.3, rho0=1.1, p0=0.8)
+    pf = po.make_params(nx, ny, recon=po.RECON_FIRST_ORDER, flux=flux, bc=po.BC_REFLECT, gamma=1.4, rho0=1.1,
+                        p0=0.8, g=1.3)
+    d = rand_dev(pd, 1e-2, seed=3)
+    w = po.equilibrium_primitive(pf)
+    u = po.conservative(pf.gamma, w) + d[:, :, :, 0]
+    np.testing.assert_allclose(po.dg2d_residual(pd, d)[:, :, :, 0], po.fv2d_residual(pf, u, w), rtol=0, atol=1e-13)
+
+
+def test_transpose_symmetry():
+    p = po.make_dg_params(6, 6, degree=2, g=1.0)
+    d = rand_dev(p, 1e-2, seed=4)
+    # transposing the grid swaps x<->y, the two momenta and modes (1,0)<->(0,1), (2,0)<->(0,2)
+    perm = [0, 2, 1, 5, 4, 3]
+    dt_ = d.transpose(0, 2, 1, 3)[[0, 2, 1, 3]][:, :, :, perm]
+    L = po.dg2d_residual(p, d)
+    Lt = po.dg2d_residual(p, np.asfortranarray(dt_))
+    np.testing.assert_allclose(Lt, L.transpose(0, 2, 1, 3)[[0, 2, 1, 3]][:, :, :, perm], rtol=0, atol=1e-13)
+
+
+def test_walls_conserve_mass():
+    p = po.make_dg_params(12, 10, degree=2, g=1.0, limiter=po.DG_LIMITER_NONE)
+    d = po.dg2d_fourier_state(p, amp=1e-2)
+    d2, _, n, _ = po.dg2d_advance(p, d, max_steps=8)
+    assert n == 8
+    assert abs(d2[0, :, :, 0].sum() - d[0, :, :, 0].sum()) < 1e-15
+
+
+def _acoustic(n, degree):
+    """Pressure wave dp = 1e-3 cos(pi x) cos(pi y) on the g = 1 atmosphere
+    (compatible with the walls: its even extension is smooth), t = 0.05."""
+    p = po.make_dg_params(n, n, degree=degree, g=1.0, limiter=po.DG_LIMITER_NONE, cfl=0.5)
+    modes = (np.array([1.0, 1.0]), np.array([0.5, 0.5]), np.array([0.5, -0.5]), np.array([0.0, 0.0]))
+    d = po.dg2d_fourier_state(p, amp=1e-3, modes=modes)
+    d2, t, _, _ = po.dg2d_advance(p, d, t_end=0.05)
+    assert abs(t - 0.05) < 1e-15
+    return d2[:, :, :, 0]   # cell means
+
+
+_REF = {}
+
+
+@pytest.mark.parametrize("degree,lo", [(1, 2.2), (2, 2.9)])
+def test_convergence_order(degree, lo):
+    """Cell means against a P2 run 4-8x finer: the energy error falls at
+    order about k+1 (measured 2.7 for P1, 3.4 for P2)."""
+    if "ref" not in _REF:
+        _REF["ref"] = _acoustic(64, 2)
+    ref = _REF["ref"]
+    errs = []
+    for n in (8, 16):
+        f = 64 // n
+        coarse = ref.reshape(4, n, f, n, f).mean(axis=(2, 4))
+        errs.append(np.mean(np.abs(_acoustic(n, degree)[3] - coarse[3])))
+    order = np.log2(errs[0] / errs[1])
+    assert order > lo, (errs, order)
+
+
+def test_tvb_limiter_keeps_means_and_smooth_data():
+    p = po.make_dg_params(16, 16, degree=2, tvb_m=50.0)
+    d = po.dg2d_fourier_state(p, amp=1e-4)
+    np.testing.assert_array_equal(po.dg2d_limit(p, d), d)       # slopes below M dx^2: untouched
+    # a pressure step: limited cells keep their means, lose their quadratic modes
+    X, Y = po.dg2d_points(p)
+    nodal = np.zeros((4, 16, 16, 9), order="F")
+    nodal[3] = np.where(X < 0.5, 1.0, 0.0)
+    nodal[0] = np.where(X + Y < 1.0, 0.5, 0.0)
+    d = po.dg2d_project(p, nodal)
+    lim = po.dg2d_limit(p, d)
+    np.testing.assert_array_equal(lim[:, :, :, 0], d[:, :, :, 0])
+    changed = np.any(lim != d, axis=3)
+    assert changed.any()
+    assert np.all(lim[:, :, :, 3:][changed] == 0.0)
+    p0 = po.make_dg_params(16, 16, degree=2, tvb_m=0.0)
+    lim0 = po.dg2d_limit(p0, d)
+    # TVD (M = 0): linear modes never exceed the neighbour differences
+    s3 = np.sqrt(3.0)
+    m = lim0[3, :, :, 0]
+    dxp = np.diff(m, axis=0)
+    a = s3 * lim0[3, 1:-1, :, 1]
+    assert np.all(np.abs(a) <= np.maximum(np.abs(dxp[1:]), np.abs(dxp[:-1])) + 1e-15)
