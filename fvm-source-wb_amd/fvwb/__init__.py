@@ -79,6 +79,29 @@ def make_params_1d(nx, *, bc=BC1D_WALL, recon=RECON_MUSCL_MINMOD, integrator=RK2
     return Params1D(3, nx, bc, recon, integrator, nequilibrium, gamma, boxlen, cfl, g)
 
 
+DG_LIMITER_NONE, DG_LIMITER_TVB = 0, 1
+
+
+class DGParams(ctypes.Structure):
+    """``fvwb_dg_params`` of include/fvwb.h."""
+
+    _fields_ = [
+        ("nvar", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("degree", ctypes.c_int32), ("flux", ctypes.c_int32), ("limiter", ctypes.c_int32),
+        ("bc", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("boxlen_x", ctypes.c_double), ("boxlen_y", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("rho0", ctypes.c_double), ("p0", ctypes.c_double),
+        ("g", ctypes.c_double), ("tvb_m", ctypes.c_double),
+    ]
+
+
+def make_dg_params(nx, ny, *, degree=2, flux=FLUX_HLL, limiter=DG_LIMITER_TVB, gamma=1.4, boxlen_x=1.0,
+                   boxlen_y=1.0, cfl=0.8, rho0=1.0, p0=1.0, g=1.0, tvb_m=50.0) -> DGParams:
+    """Defaults: the DG gravity workload (P2, HLL, TVB with M = 50, walls, CFL 0.8)."""
+    return DGParams(4, nx, ny, degree, flux, limiter, BC_REFLECT, 0, gamma, boxlen_x, boxlen_y, cfl, rho0, p0,
+                    g, tvb_m)
+
+
 _lib = None
 
 
@@ -135,6 +158,19 @@ def lib():
         "fvwb_batch_autotune": [H, ctypes.c_int32, ctypes.c_char_p, ip, ip],
         "fvwb_get_launch": [H, ip, ip],
     }
+    pdg = ctypes.POINTER(DGParams)
+    sig.update({
+        "fvwb_dg_create": [ctypes.POINTER(H), pdg],
+        "fvwb_dg_destroy": [H],
+        "fvwb_dg_compute_residual": [H, dp, dp],
+        "fvwb_dg_upload": [H, dp],
+        "fvwb_dg_init_fourier": [H, ctypes.c_int32, dp, dp, dp, dp, ctypes.c_double],
+        "fvwb_dg_advance": [H, ctypes.c_int32, ctypes.c_double],
+        "fvwb_dg_advance_async": [H, ctypes.c_int32, ctypes.c_double],
+        "fvwb_dg_download": [H, dp],
+        "fvwb_dg_get_time": [H, dp, dp, ip],
+        "fvwb_dg_get_stream": [H, ctypes.POINTER(ctypes.c_void_p)],
+    })
     p1 = ctypes.POINTER(Params1D)
     sig.update({
         "fvwb1d_params_reference": [p1, ctypes.c_int32, ctypes.c_int32],
@@ -160,6 +196,8 @@ def lib():
     L.fvwb_batch_last_error.restype = ctypes.c_char_p
     L.fvwb1d_last_error.argtypes = [H]
     L.fvwb1d_last_error.restype = ctypes.c_char_p
+    L.fvwb_dg_last_error.argtypes = [H]
+    L.fvwb_dg_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -509,4 +547,83 @@ class Solver1D:
     def stream(self):
         s = ctypes.c_void_p()
         _check1d(lib().fvwb1d_get_stream(self._h, ctypes.byref(s)), self._h)
+        return s.value or 0
+
+
+# ---- 2D modal DG ---------------------------------------------------------------------
+def _checkdg(rc, h=None):
+    if rc != OK:
+        msg = lib().fvwb_dg_last_error(h)
+        raise FvwbError(rc, msg.decode() if msg else "")
+
+
+class SolverDG:
+    """P2 modal DG of the deviation on the GPU (FP32, projections on MFMA).
+    Modes are float64 AoS(4,nx,ny,6): numpy shape (4, nx, ny, 6), order="F"."""
+
+    NM = 6
+
+    def __init__(self, params: DGParams):
+        self.params = params
+        self.shape = (4, params.nx, params.ny, self.NM)
+        self._h = ctypes.c_void_p()
+        _checkdg(lib().fvwb_dg_create(ctypes.byref(self._h), ctypes.byref(params)))
+
+    def close(self):
+        if self._h:
+            lib().fvwb_dg_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _arr(self, a):
+        a = np.asfortranarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise FvwbError(ERR_ARG, f"expected shape {self.shape}, got {a.shape}")
+        return a
+
+    def compute_residual(self, modes):
+        m = self._arr(modes)
+        out = np.zeros(self.shape, order="F")
+        _checkdg(lib().fvwb_dg_compute_residual(self._h, _dp(m), _dp(out)), self._h)
+        return out
+
+    def upload(self, modes):
+        m = self._arr(modes)
+        _checkdg(lib().fvwb_dg_upload(self._h, _dp(m)), self._h)
+
+    def init_fourier(self, a, mk, nk, theta, amp):
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (a, mk, nk, theta)]
+        _checkdg(lib().fvwb_dg_init_fourier(self._h, len(arrs[0]), *(_dp(x) for x in arrs), float(amp)), self._h)
+
+    def advance(self, nsteps, t_end=math.inf):
+        _checkdg(lib().fvwb_dg_advance(self._h, int(nsteps), float(t_end)), self._h)
+
+    def advance_async(self, nsteps, t_end=math.inf):
+        _checkdg(lib().fvwb_dg_advance_async(self._h, int(nsteps), float(t_end)), self._h)
+
+    def download(self):
+        out = np.zeros(self.shape, order="F")
+        _checkdg(lib().fvwb_dg_download(self._h, _dp(out)), self._h)
+        return out
+
+    def time(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int32()
+        _checkdg(lib().fvwb_dg_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), self._h)
+        return t.value, dt.value, n.value
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        _checkdg(lib().fvwb_dg_get_stream(self._h, ctypes.byref(s)), self._h)
         return s.value or 0

@@ -294,6 +294,31 @@ typedef struct fvwb_dg_params {
   double  tvb_m;       /* TVB constant M: slopes below M*dx^2 are kept          */
 } fvwb_dg_params;
 
+typedef struct fvwb_dg* fvwb_dg_handle;
+/* The device path: FP32 modes and arithmetic (FP64 equilibrium tables), the
+ * projections on the matrix cores, degree 2 only. */
+int fvwb_dg_create(fvwb_dg_handle* h, const fvwb_dg_params* p);
+int fvwb_dg_destroy(fvwb_dg_handle h);
+const char* fvwb_dg_last_error(fvwb_dg_handle h);
+/* dudt of the deviation modes (host AoS(4,nx,ny,6) FP64 in and out): the
+ * counterpart of compute_update (2d/benchmark_2d_dg.f90:1137-1479) for the
+ * deviation form, P2 basis and the face flux of p.flux. */
+int fvwb_dg_compute_residual(fvwb_dg_handle h, const double* modes, double* dudt);
+/* Device state: upload deviation modes (limited on arrival when p.limiter is
+ * TVB), or build them from a pressure perturbation
+ * dp = amp * sum_q a[q] cos(2 pi (mk[q] x + nk[q] y) + theta[q]) / sum_q |a[q]|
+ * (dE = dp/(gamma-1)) projected on P2; the clock restarts at 0. */
+int fvwb_dg_upload(fvwb_dg_handle h, const double* modes);
+int fvwb_dg_init_fourier(fvwb_dg_handle h, int32_t nterms, const double* a, const double* mk, const double* nk,
+                         const double* theta, double amp);
+/* nsteps SSP-RK3 steps (limiter after every stage; dt = cfl/(5 max rate) from
+ * the device, the last step clipped to t_end; no host synchronisation inside). */
+int fvwb_dg_advance(fvwb_dg_handle h, int32_t nsteps, double t_end);
+int fvwb_dg_advance_async(fvwb_dg_handle h, int32_t nsteps, double t_end);
+int fvwb_dg_download(fvwb_dg_handle h, double* modes);
+int fvwb_dg_get_time(fvwb_dg_handle h, double* t, double* dt_last, int32_t* steps_done);
+int fvwb_dg_get_stream(fvwb_dg_handle h, void** stream);
+
 #ifdef __cplusplus
 }
 #endif

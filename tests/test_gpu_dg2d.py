@@ -1,0 +1,129 @@
+"""P2 modal DG on the GPU (FP32, projections on the matrix cores) against the
+FP64 oracle: residuals, the TVB limiter, whole SSP-RK3 runs, the Fourier
+initial state, exact rest and the clock.
+
+Tolerances: the device works in FP32 on the deviation (about 6e-8 relative to
+the deviation, never to the O(1) state), so residuals agree to 2e-5 of their
+maximum and states after tens of steps to 1e-5 of the deviation's maximum; the
+resting atmosphere stays exactly zero."""
+import sys
+
+import numpy as np
+import pytest
+
+import fvwb
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_p(op):
+    return fvwb.DGParams(op.nvar, op.nx, op.ny, op.degree, op.flux, op.limiter, op.bc, 0, op.gamma, op.boxlen_x,
+                         op.boxlen_y, op.cfl, op.rho0, op.p0, op.g, op.tvb_m)
+
+
+def smooth_noisy(op, amp=1e-3, noise=1e-4, seed=1):
+    d = po.dg2d_fourier_state(op, amp=amp)
+    rng = np.random.default_rng(seed)
+    d = d + noise * rng.normal(size=d.shape) * np.array([1.0, 0.5, 0.5, 2.0])[:, None, None, None]
+    return np.asfortranarray(d)
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (40, 37), (17, 5), (64, 64), (2, 33)])
+@pytest.mark.parametrize("flux", (po.FLUX_HLL, po.FLUX_RUSANOV))
+def test_residual_matches_oracle(shape, flux):
+    nx, ny = shape
+    say(f"case: DG residual {nx}x{ny} flux={flux}")
+    op = po.make_dg_params(nx, ny, flux=flux, g=1.3, rho0=1.1, p0=0.9)
+    d = smooth_noisy(op, seed=nx + ny)
+    ref = po.dg2d_residual(op, d)
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        got = s.compute_residual(d)
+    err = np.max(np.abs(got - ref))
+    say(f"   max |L| {np.max(np.abs(ref)):.3e}, max err {err:.3e}")
+    assert err <= 2e-5 * np.max(np.abs(ref))
+
+
+def test_rest_is_exact():
+    say("case: DG rest")
+    op = po.make_dg_params(37, 29, g=2.0)
+    z = np.zeros((4, 37, 29, 6), order="F")
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        assert np.count_nonzero(s.compute_residual(z)) == 0
+        s.upload(z)
+        s.advance(25)
+        assert s.time()[2] == 25
+        assert np.count_nonzero(s.download()) == 0
+
+
+def test_limiter_matches_oracle():
+    say("case: DG TVB limiter on a step")
+    op = po.make_dg_params(32, 24, tvb_m=50.0)
+    X, Y = po.dg2d_points(op)
+    nodal = np.zeros((4, 32, 24, 9), order="F")
+    nodal[3] = np.where(X < 0.37, 0.5, 0.0)     # slopes well above M dx^2 = 0.049
+    nodal[0] = np.where(X + Y < 0.8, 0.05, 0.0)
+    d = po.dg2d_project(op, nodal)
+    ref = po.dg2d_limit(op, d)
+    assert np.any(ref != d)
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        s.upload(d)
+        got = s.download()
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-7 * np.max(np.abs(d)))
+
+
+@pytest.mark.parametrize("limiter", (po.DG_LIMITER_NONE, po.DG_LIMITER_TVB))
+def test_run_matches_oracle(limiter):
+    say(f"case: DG 30 SSP-RK3 steps limiter={limiter}")
+    op = po.make_dg_params(48, 40, limiter=limiter, g=1.0)
+    d = po.dg2d_limit(op, po.dg2d_fourier_state(op, amp=1e-4))
+    ref, t, n, dt = po.dg2d_advance(op, d, max_steps=30)
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        s.upload(d)
+        s.advance(30)
+        tg, dtg, ng = s.time()
+        got = s.download()
+    assert ng == n == 30 and abs(tg - t) <= 1e-6 * t
+    scale = np.max(np.abs(ref))
+    err = np.max(np.abs(got - ref))
+    l1 = np.mean(np.abs(got[0, :, :, 0] - ref[0, :, :, 0]))
+    say(f"   max |d| {scale:.3e}, max err {err:.3e}, L1 density (means) {l1:.3e}")
+    assert err <= 1e-5 * scale and l1 <= 1e-6
+
+
+def test_fourier_initial_state_and_t_end():
+    say("case: DG init_fourier, t_end clip")
+    op = po.make_dg_params(32, 32)
+    a, mk, nk, th = po.fourier_modes()
+    ref = po.dg2d_limit(op, po.dg2d_fourier_state(op, amp=1e-4))
+    t_end = 3e-3
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        s.init_fourier(a, mk.astype(float), nk.astype(float), th, 1e-4)
+        np.testing.assert_allclose(s.download(), ref, rtol=0, atol=2e-7 * np.max(np.abs(ref)))
+        s.advance(1000, t_end)
+        t, dt, n = s.time()
+        s.advance(5, t_end)                 # a finished instance does not move
+        assert s.time() == (t, dt, n)
+    _, to, no, _ = po.dg2d_advance(op, ref, t_end=t_end)
+    assert t == t_end and n == no
+
+
+def test_error_paths():
+    say("case: DG error paths")
+    with pytest.raises(fvwb.FvwbError):
+        fvwb.SolverDG(gpu_p(po.make_dg_params(8, 8, degree=1)))
+    op = po.make_dg_params(8, 8)
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        with pytest.raises(fvwb.FvwbError) as e:
+            s.advance(1)
+        assert e.value.code == fvwb.ERR_STATE
+        d = np.zeros((4, 8, 8, 6), order="F")
+        d[3, :, :, 0] = -10.0                 # negative pressure
+        with pytest.raises(fvwb.FvwbError) as e:
+            s.upload(d)
+        assert e.value.code == fvwb.ERR_NONPHYSICAL
