@@ -15,7 +15,9 @@ FP32-bulk deviation form, MUSCL-minmod, HLL, reflecting walls, SSP-RK3; launch
 shape chosen by the library's autotuner.  Beside it: the plain FP64 kernel on
 the same batch (speedup and L1 density error of FP32 vs FP64), the 8192x8192
 single problem in both precisions, a pure-equilibrium drift run, the roofline
-of the stage kernel and the FP64 CPU oracle timed on a bounded sample.
+of the stage kernel and the FP64 CPU oracle timed on a bounded sample; then the
+P2 modal DG workload (1024^2, 100 steps) and the 1D workload (256 cells to
+t = 0.25, alone and 4096 side by side).
 """
 import argparse
 import ctypes
@@ -140,6 +142,50 @@ def measured_traffic(cells, shape):
     return d["hbm_bytes_per_launch"], "profiles/fv2d_batch512_fp32_traffic.json"
 
 
+def time_dg(ev, n, steps, warmup):
+    """The DG workload: n x n cells, P2 (6 modes, 3x3 Gauss points), HLL, TVB
+    (M = 50), walls, SSP-RK3, g = 1 atmosphere plus a 16-term Fourier pressure
+    perturbation of amplitude 1e-4 (seed 0xF00D)."""
+    a, mk, nk, th = fvwb.dg_fourier_terms()
+    p = fvwb.make_dg_params(n, n)
+    s = fvwb.SolverDG(p)
+    s.init_fourier(a, mk.astype(float), nk.astype(float), th, 1e-4)
+    if warmup:
+        s.advance(warmup)
+        s.init_fourier(a, mk.astype(float), nk.astype(float), th, 1e-4)
+    st = s.stream()
+    e0, e1 = ev.event(), ev.event()
+    ev.sync()
+    ev.record(e0, st)
+    s.advance_async(steps)
+    ev.record(e1, st)
+    ms = ev.elapsed_ms(e0, e1)
+    ev.sync()
+    assert s.time()[2] == steps
+    s.close()
+    return ms / 1e3
+
+
+def time_fv1d(ev, ninst, n=256, t_end=0.25):
+    """The 1D gravity workload: n cells, MUSCL-minmod, Rusanov, SSP-RK2, walls,
+    CFL 0.5, rho = p = exp(-x) plus a pressure pulse 1e-3 exp(-(x-0.5)^2/(2 0.05^2)),
+    run to t_end; `ninst` copies run side by side, one workgroup each."""
+    ps = [fvwb.make_params_1d(n) for _ in range(ninst)]
+    s = fvwb.Solver1D(ps)
+    s.init_gaussian(np.full(ninst, 1e-3), 0.5, 0.05)
+    st = s.stream()
+    e0, e1 = ev.event(), ev.event()
+    ev.sync()
+    ev.record(e0, st)
+    s.advance(t_end=t_end)
+    ev.record(e1, st)
+    ms = ev.elapsed_ms(e0, e1)
+    t, _, nsteps = s.time(0)
+    assert t == t_end
+    s.close()
+    return ms / 1e3, nsteps
+
+
 def cpu_baseline():
     """FP64 CPU oracle (plain C port, one core) on a bounded sample: one
     2048x2048 problem of the same scheme, 2 SSP-RK3 steps."""
@@ -168,6 +214,8 @@ def main():
     ap.add_argument("--shape", default=None,
                     help="ROWS,CPL: fixed launch shape for the batch instead of the autotuner")
     ap.add_argument("--skip-single", action="store_true", help="skip the 8192^2 single-problem runs")
+    ap.add_argument("--dg-size", type=int, default=1024, help="cells per side of the DG run (0: skip)")
+    ap.add_argument("--fv1d-batch", type=int, default=4096, help="1D instances in the batched 1D run (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -236,6 +284,31 @@ def main():
     traffic, traffic_note = measured_traffic(cells, shape32)
     cpu = None if args.no_cpu else cpu_baseline()
 
+    # ---- DG workload (P2 modal DG, projections on MFMA) ----------------------------------
+    dg = None
+    if args.dg_size > 0:
+        ndg, kdg = args.dg_size, 100
+        sec_dg = time_dg(ev, ndg, kdg, 3)
+        cu_dg = ndg * ndg * kdg / sec_dg
+        # issued MFMA work: 76 v_mfma_f32_16x16x4f32 (2048 flops each) per 16 cells per stage
+        mfma_tflops = cu_dg * 3 * 76 * 2048 / 16 / 1e12
+        dg = {"workload": f"{ndg}x{ndg} P2 modal DG, 3x3 Gauss, HLL, TVB M=50, walls, SSP-RK3, FP32 + FP64 tables",
+              "value": cu_dg, "unit": "cell-updates/s", "steps": kdg, "ms_per_step": sec_dg * 1e3 / kdg,
+              "bytes_per_cell_update": 768, "hbm_frac": cu_dg * 768 / 8e12,
+              "mfma_tflops_issued": mfma_tflops, "mfma_frac_of_fp32_peak": mfma_tflops / 157.3}
+        log(f"dg {ndg}^2: {cu_dg:.4e} cell-updates/s, {sec_dg * 1e3 / kdg:.3f} ms/step")
+
+    # ---- 1D workload (256 cells to t = 0.25, whole run in one launch) ---------------------
+    fv1d = None
+    if args.fv1d_batch > 0:
+        sec1, n1s = time_fv1d(ev, 1)
+        secb, nbs = time_fv1d(ev, args.fv1d_batch)
+        fv1d = {"workload": "256 cells, MUSCL-minmod, Rusanov, SSP-RK2, walls, CFL 0.5, t_end 0.25, FP64",
+                "single_run_ms": sec1 * 1e3, "steps": n1s,
+                "batch_instances": args.fv1d_batch, "batch_run_ms": secb * 1e3,
+                "batch_value": args.fv1d_batch * 256 * nbs / secb, "unit": "cell-updates/s"}
+        log(f"fv1d: one run {sec1 * 1e3:.3f} ms ({n1s} steps); {args.fv1d_batch} runs {secb * 1e3:.3f} ms")
+
     out = {
         "metric": METRIC,
         "value": rate32,
@@ -268,6 +341,8 @@ def main():
         "linf_density_err_fp32_vs_fp64": linf_max,
         "single_8192": single,
         "hydrostatic_drift": drift,
+        "dg_p2": dg,
+        "fv1d": fv1d,
     }
     print(json.dumps(out), flush=True)
 

@@ -551,6 +551,18 @@ class Solver1D:
 
 
 # ---- 2D modal DG ---------------------------------------------------------------------
+def dg_fourier_terms(n=16, seed=0xF00D, mmax=8):
+    """(a_k, m_k, n_k, theta_k) of the DG workload's pressure perturbation:
+    a ~ U[0.5,1], (m,n) in {1..mmax}^2, theta ~ U[0, 2 pi), drawn from numpy's
+    PCG64 seeded with `seed` (the problem definition, not a solver input)."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.5, 1.0, n)
+    mk = rng.integers(1, mmax + 1, n)
+    nk = rng.integers(1, mmax + 1, n)
+    th = rng.uniform(0.0, 2 * np.pi, n)
+    return a, mk, nk, th
+
+
 def _checkdg(rc, h=None):
     if rc != OK:
         msg = lib().fvwb_dg_last_error(h)

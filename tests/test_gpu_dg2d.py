@@ -99,7 +99,9 @@ def test_run_matches_oracle(limiter):
 def test_fourier_initial_state_and_t_end():
     say("case: DG init_fourier, t_end clip")
     op = po.make_dg_params(32, 32)
-    a, mk, nk, th = po.fourier_modes()
+    a, mk, nk, th = fvwb.dg_fourier_terms()
+    for x, y in zip((a, mk, nk, th), po.fourier_modes()):
+        np.testing.assert_array_equal(x, y)
     ref = po.dg2d_limit(op, po.dg2d_fourier_state(op, amp=1e-4))
     t_end = 3e-3
     with fvwb.SolverDG(gpu_p(op)) as s:
