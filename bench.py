@@ -129,17 +129,18 @@ def time_single(ev, n, precision, steps, warmup):
 
 def measured_traffic(cells, shape):
     """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs (scaled by the copy-kernel calibration), valid for the
-    workload and launch shape they were taken on; None otherwise."""
+    WRITE_SIZE runs (scaled by the copy-kernel calibration), for the workload
+    and launch shape they were taken on; None for a shape not profiled."""
     path = os.path.join(ROOT, "profiles", "fv2d_batch512_fp32_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
     except OSError:
         return None, "no profile"
-    if d.get("cells") != cells or tuple(d.get("launch_shape", ())) != tuple(shape):
-        return None, f"profile taken on shape {d.get('launch_shape')}, not {list(shape)}"
-    return d["hbm_bytes_per_launch"], "profiles/fv2d_batch512_fp32_traffic.json"
+    key = f"{shape[0]},{shape[1]}"
+    if d.get("cells") != cells or key not in d.get("shapes", {}):
+        return None, f"no profile for shape {list(shape)} (profiled: {sorted(d.get('shapes', {}))})"
+    return d["shapes"][key]["hbm_bytes_per_launch"], f"profiles/fv2d_batch512_fp32_traffic.json shape {key}"
 
 
 def time_dg(ev, n, steps, warmup):
@@ -214,6 +215,7 @@ def main():
     ap.add_argument("--shape", default=None,
                     help="ROWS,CPL: fixed launch shape for the batch instead of the autotuner")
     ap.add_argument("--skip-single", action="store_true", help="skip the 8192^2 single-problem runs")
+    ap.add_argument("--fp32-only", action="store_true", help="time the FP32 batch only (no JSON line)")
     ap.add_argument("--dg-size", type=int, default=1024, help="cells per side of the DG run (0: skip)")
     ap.add_argument("--fv1d-batch", type=int, default=4096, help="1D instances in the batched 1D run (0: skip)")
     args = ap.parse_args()
@@ -232,6 +234,10 @@ def main():
     rate32 = cells * K / sec32
     log(f"batch fp32: {len(sides)} instances, {cells} cells, shape {shape32}, {K} steps in "
         f"{sec32 * 1e3:.2f} ms -> {rate32:.4e} cell-updates/s")
+    if args.fp32_only:     # profiling runs of the FP32 batch kernel alone
+        log(f"fp32-only: {rate32:.4e} cell-updates/s, shape {shape32}")
+        b32.close()
+        return
     sec64, b64, shape64 = time_batch(ev, sides, gs, amps, fvwb.FP64, K, W, fixed)
     rate64 = cells * K / sec64
     log(f"batch fp64: shape {shape64}, {K} steps in {sec64 * 1e3:.2f} ms -> {rate64:.4e} cell-updates/s")
