@@ -40,9 +40,9 @@ constexpr int TILE = 16;   // cells per tile side
 
 // Per-lane MFMA operand constants, built on the host from the basis.
 struct Consts {
-  float fwd_node[16][8];   // V[point][mode], points a + 3b (9 used)
+  float fwd_node[16][8];   // V[row][mode], row 4g + r -> Gauss node g + 4r (node a + 3b; 9 used)
   float fwd_face[16][8];   // trace points: 0-2 left (xi=-1, eta_b), 3-5 right, 6-8 bottom (xi_a, eta=-1), 9-11 top
-  float back_vol[3][16][16];  // [quantity][mode][point]: 0.5 w dphi_i phi_j (F_x), 0.5 w phi_i dphi_j (F_y), 0.25 w phi phi (S)
+  float back_vol[3][16][16];  // [quantity][mode][row as fwd_node]: 0.5 w dphi_i phi_j (F_x), 0.5 w phi_i dphi_j (F_y), 0.25 w phi phi (S)
   float back_face[16][12];    // [mode][face point]: +-0.5 w phi (flux pre-scaled by 1/dx or 1/dy)
 };
 
@@ -100,6 +100,12 @@ __device__ inline FaceEq<float> eq_at(const Args& a, double ex, double ey) {
   return q;
 }
 
+// Lane-local node assignment of the forward volume product: row 4*g + r of the
+// node block (g = lane/16) is Gauss node g + 4r, so the nine nodes spread as
+// 3,2,2,2 over the four lane groups and the backward volume product needs only
+// r = 0..2 (9 steps of 4 k-slots) instead of 12.
+__host__ __device__ constexpr int node_of_row(int row) { return (row >> 2) + 4 * (row & 3); }
+
 template <int FLUX>
 __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   const Ctrl* ck = a.ctrl;
@@ -115,71 +121,116 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   }
   const float dtf = (float)dt;
 
-  __shared__ float XT[2][4][TILE + 1][TILE][NQ];   // x faces: [side][var][face][row][point]
-  __shared__ float YT[2][4][TILE + 1][TILE][NQ];   // y faces: [side][var][face][col][point]
+  // face traces, then face fluxes: the innermost index runs along the lanes
+  __shared__ float XT[2][4][TILE][NQ][TILE + 1];   // x faces: [side][var][row][point][face]
+  __shared__ float YT[2][4][TILE + 1][NQ][TILE];   // y faces: [side][var][face][point][col]
+  // equilibrium tables of the tile (halo included): e^{-kx}, e^{-ky} at nodes and faces
+  __shared__ double Exn[3 * TILE], Eyn[3 * TILE], Exf[TILE + 1], Eyf[TILE + 1];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c16 = lane & 15, g4 = lane >> 4;
   const int tx0 = blockIdx.x * TILE, ty0 = blockIdx.y * TILE;
   const Consts& C = *a.C;
 
+  {
+    const int q = threadIdx.x;
+    if (q < 3 * TILE) {
+      const int ci = min(tx0 + q / 3, a.nx - 1), cj = min(ty0 + q / 3, a.ny - 1);
+      Exn[q] = a.exn[3 * ci + q % 3];
+      Eyn[q] = a.eyn[3 * cj + q % 3];
+    } else if (q < 3 * TILE + TILE + 1) {
+      const int f = q - 3 * TILE;
+      Exf[f] = a.exf[min(tx0 + f, a.nx)];
+      Eyf[f] = a.eyf[min(ty0 + f, a.ny)];
+    }
+  }
+
   Phys<float> P;
   P.gamma = a.gamma; P.gm1 = a.gamma - 1.0f; P.half = 0.5f; P.g = a.g;
 
-  // forward operands: lane supplies V[point = lane%16][mode = lane/16 + 4s]
+  // all mode loads of the tile up front: lane supplies mode g4 (+4) of cell c16
+  float B0[4][4], B1[4][4], N0[4][4], N1[4][4];   // [row group][var]: stage input and U^n
+  const int i = tx0 + c16;
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) {
+    const int j = ty0 + 4 * wave + rr;
+    const bool ok = j < a.ny && i < a.nx;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      B0[rr][v] = ok ? ld_mode(a.in, a.plane, a.pitch, v, g4, i, j) : 0.f;
+      B1[rr][v] = (ok && g4 + 4 < NM) ? ld_mode(a.in, a.plane, a.pitch, v, g4 + 4, i, j) : 0.f;
+      const bool okn = ok && a.alpha != 0.f;
+      N0[rr][v] = okn ? ld_mode(a.un, a.plane, a.pitch, v, g4, i, j) : 0.f;
+      N1[rr][v] = (okn && g4 + 4 < NM) ? ld_mode(a.un, a.plane, a.pitch, v, g4 + 4, i, j) : 0.f;
+    }
+  }
+  // halo cell of this wave: below, above, left or right of the tile
+  int hi, hj;
+  bool hok;
+  if (wave == 0) { hi = tx0 + c16; hj = ty0 - 1; hok = ty0 >= 1; }
+  else if (wave == 1) { hi = tx0 + c16; hj = ty0 + TILE; hok = ty0 + TILE < a.ny; }
+  else if (wave == 2) { hi = tx0 - 1; hj = ty0 + c16; hok = tx0 >= 1; }
+  else { hi = tx0 + TILE; hj = ty0 + c16; hok = tx0 + TILE < a.nx; }
+  hok = hok && hi < a.nx && hj < a.ny;
+  float H0[4], H1[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    H0[v] = hok ? ld_mode(a.in, a.plane, a.pitch, v, g4, hi, hj) : 0.f;
+    H1[v] = (hok && g4 + 4 < NM) ? ld_mode(a.in, a.plane, a.pitch, v, g4 + 4, hi, hj) : 0.f;
+  }
+
+  // forward operands: lane supplies V[row = lane%16][mode = lane/16 + 4s]
   const float An0 = C.fwd_node[c16][g4], An1 = C.fwd_node[c16][g4 + 4];
   const float Af0 = C.fwd_face[c16][g4], Af1 = C.fwd_face[c16][g4 + 4];
-  // backward operands: lane supplies B[mode = lane%16][k-slot = 4*(lane/16) + s]
-  float Abv[3][4], Abf[3];
+  // backward operands: lane supplies B[mode = lane%16][row = 4*(lane/16) + r]
+  float Abv[3][3], Abf[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) Abv[q][s] = C.back_vol[q][c16][4 * g4 + s];
+    for (int r = 0; r < 3; ++r) Abv[q][r] = C.back_vol[q][c16][4 * g4 + r];
 #pragma unroll
   for (int s = 0; s < 3; ++s) Abf[s] = C.back_face[c16][4 * s + g4];
+
+  __syncthreads();   // equilibrium tables
 
   v4f acc[4][4];   // [row group][var]: D[mode 4*(l/16)+r][cell l%16]
 
   // ---- phase 1: interior rows (volume terms + traces) --------------------------------
 #pragma unroll
   for (int rr = 0; rr < 4; ++rr) {
-    const int jl = 4 * wave + rr, j = ty0 + jl, i = tx0 + c16;
-    const bool row_ok = j < a.ny;
+    const int jl = 4 * wave + rr;
     v4f dn[4], df[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      float b0 = 0.f, b1 = 0.f;
-      if (row_ok && i < a.nx) {
-        b0 = ld_mode(a.in, a.plane, a.pitch, v, g4, i, j);
-        if (g4 + 4 < NM) b1 = ld_mode(a.in, a.plane, a.pitch, v, g4 + 4, i, j);
-      }
       v4f z = {0.f, 0.f, 0.f, 0.f};
-      dn[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(An0, b0, z, 0, 0, 0);
-      dn[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(An1, b1, dn[v], 0, 0, 0);
-      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af0, b0, z, 0, 0, 0);
-      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af1, b1, df[v], 0, 0, 0);
+      dn[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(An0, B0[rr][v], z, 0, 0, 0);
+      dn[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(An1, B1[rr][v], dn[v], 0, 0, 0);
+      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af0, B0[rr][v], z, 0, 0, 0);
+      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af1, B1[rr][v], df[v], 0, 0, 0);
     }
-    // traces to LDS: lane holds points 4*g4 + r of cell c16
+    // traces to LDS: lane holds trace points 4*g4 + r of cell c16
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int pt = 4 * g4 + r;
+#pragma unroll
       for (int v = 0; v < 4; ++v) {
         const float val = df[v][r];
-        if (pt < 3) XT[1][v][c16][jl][pt] = val;                     // left trace -> x face c16, right side
-        else if (pt < 6) XT[0][v][c16 + 1][jl][pt - 3] = val;        // right trace -> x face c16+1, left side
-        else if (pt < 9) YT[1][v][jl][c16][pt - 6] = val;            // bottom trace -> y face jl, upper side
-        else if (pt < 12) YT[0][v][jl + 1][c16][pt - 9] = val;       // top trace -> y face jl+1, lower side
+        if (pt < 3) XT[1][v][jl][pt][c16] = val;                     // left trace -> x face c16, right side
+        else if (pt < 6) XT[0][v][jl][pt - 3][c16 + 1] = val;        // right trace -> x face c16+1, left side
+        else if (pt < 9) YT[1][v][jl][pt - 6][c16] = val;            // bottom trace -> y face jl, upper side
+        else if (pt < 12) YT[0][v][jl + 1][pt - 9][c16] = val;       // top trace -> y face jl+1, lower side
       }
     }
-    // volume terms at the lane's nodes
-    const int ic = i < a.nx ? i : a.nx - 1, jc = j < a.ny ? j : a.ny - 1;
-    float X[3][4][4];   // [quantity][r][var]
+    // volume terms at the lane's nodes g4, g4+4, g4+8
+    float X[3][3][4];   // [quantity][r][var]
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int pt = 4 * g4 + r;
-      if (pt < 9) {
-        const int qa = pt % 3, qb = pt / 3;
-        FaceEq<float> q = eq_at(a, a.exn[3 * ic + qa], a.eyn[3 * jc + qb]);
+    for (int r = 0; r < 3; ++r) {
+      const int node = g4 + 4 * r;
+      if (node < 9) {
+        const int qa = node % 3, qb = node / 3;
+        const double e = Exn[3 * c16 + qa] * Eyn[3 * jl + qb], pe = a.p0 * e, ee = pe / a.gm1d;
+        FaceEq<float> q;
+        q.rho = (float)(a.rho0 * e); q.p = (float)pe; q.e = (float)ee; q.ep = (float)(ee + pe);
         Vec4<float> d = {dn[0][r], dn[1][r], dn[2][r], dn[3][r]};
         Vec4<float> fx, fy;
         point_terms(P, q, d, fx, fy);
@@ -187,7 +238,9 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
         X[1][r][0] = fy.r * a.ody; X[1][r][1] = fy.mx * a.ody; X[1][r][2] = fy.my * a.ody; X[1][r][3] = fy.e * a.ody;
         X[2][r][0] = 0.f; X[2][r][1] = -a.g * d.r; X[2][r][2] = -a.g * d.r; X[2][r][3] = -a.g * (d.mx + d.my);
       } else {
+#pragma unroll
         for (int q = 0; q < 3; ++q)
+#pragma unroll
           for (int v = 0; v < 4; ++v) X[q][r][v] = 0.f;
       }
     }
@@ -198,41 +251,30 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
 #pragma unroll
       for (int q = 0; q < 3; ++q)
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-          acc[rr][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Abv[q][s], X[q][s][v], acc[rr][v], 0, 0, 0);
+        for (int r = 0; r < 3; ++r)
+          acc[rr][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Abv[q][r], X[q][r][v], acc[rr][v], 0, 0, 0);
     }
   }
 
   // ---- phase 1b: halo traces ------------------------------------------------------
   {
-    int hi, hj;
-    bool ok;
-    if (wave == 0) { hi = tx0 + c16; hj = ty0 - 1; ok = ty0 >= 1; }                // below the tile
-    else if (wave == 1) { hi = tx0 + c16; hj = ty0 + TILE; ok = ty0 + TILE < a.ny; } // above
-    else if (wave == 2) { hi = tx0 - 1; hj = ty0 + c16; ok = tx0 >= 1; }            // left
-    else { hi = tx0 + TILE; hj = ty0 + c16; ok = tx0 + TILE < a.nx; }                // right
-    ok = ok && hi < a.nx && hj < a.ny;
     v4f df[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      float b0 = 0.f, b1 = 0.f;
-      if (ok) {
-        b0 = ld_mode(a.in, a.plane, a.pitch, v, g4, hi, hj);
-        if (g4 + 4 < NM) b1 = ld_mode(a.in, a.plane, a.pitch, v, g4 + 4, hi, hj);
-      }
       v4f z = {0.f, 0.f, 0.f, 0.f};
-      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af0, b0, z, 0, 0, 0);
-      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af1, b1, df[v], 0, 0, 0);
+      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af0, H0[v], z, 0, 0, 0);
+      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af1, H1[v], df[v], 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int pt = 4 * g4 + r;
+#pragma unroll
       for (int v = 0; v < 4; ++v) {
         const float val = df[v][r];
-        if (wave == 0 && pt >= 9 && pt < 12) YT[0][v][0][c16][pt - 9] = val;       // top trace of the row below
-        if (wave == 1 && pt >= 6 && pt < 9) YT[1][v][TILE][c16][pt - 6] = val;     // bottom trace of the row above
-        if (wave == 2 && pt >= 3 && pt < 6) XT[0][v][0][c16][pt - 3] = val;        // right trace of the left column
-        if (wave == 3 && pt < 3) XT[1][v][TILE][c16][pt] = val;                     // left trace of the right column
+        if (wave == 0 && pt >= 9 && pt < 12) YT[0][v][0][pt - 9][c16] = val;       // top trace of the row below
+        if (wave == 1 && pt >= 6 && pt < 9) YT[1][v][TILE][pt - 6][c16] = val;     // bottom trace of the row above
+        if (wave == 2 && pt >= 3 && pt < 6) XT[0][v][c16][pt - 3][0] = val;        // right trace of the left column
+        if (wave == 3 && pt < 3) XT[1][v][c16][pt][TILE] = val;                     // left trace of the right column
       }
     }
   }
@@ -240,35 +282,40 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
 
   // ---- phase 2: face fluxes (minus equilibrium flux), scaled by 1/dx or 1/dy ----------
   for (int it = threadIdx.x; it < (TILE + 1) * TILE * NQ; it += 256) {
-    const int pt = it % NQ, row = (it / NQ) % TILE, f = it / (NQ * TILE);
-    // x face f between columns f-1 and f (local), row `row`
+    // x face f (innermost), row, point
     {
+      const int f = it % (TILE + 1), pt = (it / (TILE + 1)) % NQ, row = it / ((TILE + 1) * NQ);
       const int fi = tx0 + f, j = ty0 + row;
-      if (fi <= a.nx && j < a.ny && f <= TILE) {
+      if (fi <= a.nx && j < a.ny) {
         const bool hasl = fi >= 1, hasr = fi <= a.nx - 1;
-        Vec4<float> dl = {XT[0][0][f][row][pt], XT[0][1][f][row][pt], XT[0][2][f][row][pt], XT[0][3][f][row][pt]};
-        Vec4<float> dr = {XT[1][0][f][row][pt], XT[1][1][f][row][pt], XT[1][2][f][row][pt], XT[1][3][f][row][pt]};
+        Vec4<float> dl = {XT[0][0][row][pt][f], XT[0][1][row][pt][f], XT[0][2][row][pt][f], XT[0][3][row][pt][f]};
+        Vec4<float> dr = {XT[1][0][row][pt][f], XT[1][1][row][pt][f], XT[1][2][row][pt][f], XT[1][3][row][pt][f]};
         if (!hasl) dl = {dr.r, -dr.mx, dr.my, dr.e};
         if (!hasr) dr = {dl.r, -dl.mx, dl.my, dl.e};
-        FaceEq<float> q = eq_at(a, a.exf[fi], a.eyn[3 * j + pt]);
+        const double e = Exf[f] * Eyn[3 * row + pt], pe = a.p0 * e, ee = pe / a.gm1d;
+        FaceEq<float> q;
+        q.rho = (float)(a.rho0 * e); q.p = (float)pe; q.e = (float)ee; q.ep = (float)(ee + pe);
         Vec4<float> F = face_flux<float, FLUX, 0, FORM_DEV>(P, q, dl, dr);
-        XT[0][0][f][row][pt] = F.r * a.odx; XT[0][1][f][row][pt] = F.mx * a.odx;
-        XT[0][2][f][row][pt] = F.my * a.odx; XT[0][3][f][row][pt] = F.e * a.odx;
+        XT[0][0][row][pt][f] = F.r * a.odx; XT[0][1][row][pt][f] = F.mx * a.odx;
+        XT[0][2][row][pt][f] = F.my * a.odx; XT[0][3][row][pt][f] = F.e * a.odx;
       }
     }
-    // y face f between rows f-1 and f (local), column `row`
+    // y face f, point, column (innermost)
     {
-      const int fj = ty0 + f, i = tx0 + row;
-      if (fj <= a.ny && i < a.nx && f <= TILE) {
+      const int col = it % TILE, pt = (it / TILE) % NQ, f = it / (TILE * NQ);
+      const int fj = ty0 + f, ii = tx0 + col;
+      if (fj <= a.ny && ii < a.nx) {
         const bool hasb = fj >= 1, hast = fj <= a.ny - 1;
-        Vec4<float> dl = {YT[0][0][f][row][pt], YT[0][1][f][row][pt], YT[0][2][f][row][pt], YT[0][3][f][row][pt]};
-        Vec4<float> dr = {YT[1][0][f][row][pt], YT[1][1][f][row][pt], YT[1][2][f][row][pt], YT[1][3][f][row][pt]};
+        Vec4<float> dl = {YT[0][0][f][pt][col], YT[0][1][f][pt][col], YT[0][2][f][pt][col], YT[0][3][f][pt][col]};
+        Vec4<float> dr = {YT[1][0][f][pt][col], YT[1][1][f][pt][col], YT[1][2][f][pt][col], YT[1][3][f][pt][col]};
         if (!hasb) dl = {dr.r, dr.mx, -dr.my, dr.e};
         if (!hast) dr = {dl.r, dl.mx, -dl.my, dl.e};
-        FaceEq<float> q = eq_at(a, a.exn[3 * i + pt], a.eyf[fj]);
+        const double e = Exn[3 * col + pt] * Eyf[f], pe = a.p0 * e, ee = pe / a.gm1d;
+        FaceEq<float> q;
+        q.rho = (float)(a.rho0 * e); q.p = (float)pe; q.e = (float)ee; q.ep = (float)(ee + pe);
         Vec4<float> F = face_flux<float, FLUX, 1, FORM_DEV>(P, q, dl, dr);
-        YT[0][0][f][row][pt] = F.r * a.ody; YT[0][1][f][row][pt] = F.mx * a.ody;
-        YT[0][2][f][row][pt] = F.my * a.ody; YT[0][3][f][row][pt] = F.e * a.ody;
+        YT[0][0][f][pt][col] = F.r * a.ody; YT[0][1][f][pt][col] = F.mx * a.ody;
+        YT[0][2][f][pt][col] = F.my * a.ody; YT[0][3][f][pt][col] = F.e * a.ody;
       }
     }
   }
@@ -277,40 +324,43 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   // ---- phase 3: face integrals, stage combination, store -----------------------------
 #pragma unroll
   for (int rr = 0; rr < 4; ++rr) {
-    const int jl = 4 * wave + rr, j = ty0 + jl, i = tx0 + c16;
+    const int jl = 4 * wave + rr, j = ty0 + jl;
+    const bool ok = j < a.ny && i < a.nx;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
 #pragma unroll
       for (int s = 0; s < 3; ++s) {
         const int k = 4 * s + g4;   // face point fed by this lane in step s
         float b;
-        if (k < 3) b = XT[0][v][c16][jl][k];
-        else if (k < 6) b = XT[0][v][c16 + 1][jl][k - 3];
-        else if (k < 9) b = YT[0][v][jl][c16][k - 6];
-        else b = YT[0][v][jl + 1][c16][k - 9];
-        if (!(j < a.ny && i < a.nx)) b = 0.f;
+        if (k < 3) b = XT[0][v][jl][k][c16];
+        else if (k < 6) b = XT[0][v][jl][k - 3][c16 + 1];
+        else if (k < 9) b = YT[0][v][jl][k - 6][c16];
+        else b = YT[0][v][jl + 1][k - 9][c16];
+        if (!ok) b = 0.f;
         acc[rr][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Abf[s], b, acc[rr][v], 0, 0, 0);
       }
     }
-    if (j < a.ny && i < a.nx && g4 < 2) {
+    // U and U^n in the output layout: mode 4*g4 + r of cell c16 sits in lane
+    // 16*r + c16, as its first (g4 == 0) or second (g4 == 1) loaded mode
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 4; ++r) {
+      const int src = 16 * r + c16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float u0 = __shfl(B0[rr][v], src), u1 = __shfl(B1[rr][v], src);
+        const float n0 = __shfl(N0[rr][v], src), n1 = __shfl(N1[rr][v], src);
         const int m = 4 * g4 + r;
-        if (m < NM) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const long long o = (long long)(v * NM + m) * a.plane + (long long)j * a.pitch + i;
-            const float L = acc[rr][v][r];
-            float res;
-            if (a.residual_only) {
-              res = L;
-            } else {
-              const float u = a.in[o];
-              res = a.beta * (u + dtf * L);
-              if (a.alpha != 0.f) res = a.alpha * a.un[o] + res;
-            }
-            a.out[o] = res;
+        if (ok && g4 < 2 && m < NM) {
+          const float u = g4 == 0 ? u0 : u1, un = g4 == 0 ? n0 : n1;
+          const float L = acc[rr][v][r];
+          float res;
+          if (a.residual_only) {
+            res = L;
+          } else {
+            res = a.beta * (u + dtf * L);
+            if (a.alpha != 0.f) res = a.alpha * un + res;
           }
+          a.out[(long long)(v * NM + m) * a.plane + (long long)j * a.pitch + i] = res;
         }
       }
     }
@@ -567,16 +617,17 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   Consts C;
   std::memset(&C, 0, sizeof C);
   for (int m = 0; m < NM; ++m) {
-    for (int b = 0; b < 3; ++b)
-      for (int a = 0; a < 3; ++a) {
-        const int pt = a + 3 * b;
-        const double w = h->wq[a] * h->wq[b];
-        const double pa = leg(h->xq[a], MI[m]), pb = leg(h->xq[b], MJ[m]);
-        C.fwd_node[pt][m] = (float)(pa * pb);
-        C.back_vol[0][m][pt] = (float)(0.5 * w * legp(h->xq[a], MI[m]) * pb);
-        C.back_vol[1][m][pt] = (float)(0.5 * w * pa * legp(h->xq[b], MJ[m]));
-        C.back_vol[2][m][pt] = (float)(0.25 * w * pa * pb);
-      }
+    for (int row = 0; row < 16; ++row) {
+      const int node = node_of_row(row);
+      if (node >= 9) continue;
+      const int a = node % 3, b = node / 3;
+      const double w = h->wq[a] * h->wq[b];
+      const double pa = leg(h->xq[a], MI[m]), pb = leg(h->xq[b], MJ[m]);
+      C.fwd_node[row][m] = (float)(pa * pb);
+      C.back_vol[0][m][row] = (float)(0.5 * w * legp(h->xq[a], MI[m]) * pb);
+      C.back_vol[1][m][row] = (float)(0.5 * w * pa * legp(h->xq[b], MJ[m]));
+      C.back_vol[2][m][row] = (float)(0.25 * w * pa * pb);
+    }
     for (int q = 0; q < 3; ++q) {
       C.fwd_face[q][m] = (float)(leg(-1.0, MI[m]) * leg(h->xq[q], MJ[m]));       // left
       C.fwd_face[3 + q][m] = (float)(leg(1.0, MI[m]) * leg(h->xq[q], MJ[m]));    // right
