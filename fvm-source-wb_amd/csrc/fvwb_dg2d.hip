@@ -429,8 +429,8 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
         const float xt = gminmod(x, ue - u0, u0 - uw, a.mdx2);
         const float yt = gminmod(y, un - u0, u0 - us, a.mdy2);
         if (xt != x || yt != y) {
-          md[v][1] = xt / s3;
-          md[v][2] = yt / s3;
+          md[v][1] = xt * (1.0f / 1.7320508075688772f);
+          md[v][2] = yt * (1.0f / 1.7320508075688772f);
           md[v][3] = md[v][4] = md[v][5] = 0.f;
           for (int m = 1; m < NM; ++m) a.st[(long long)(v * NM + m) * a.plane + o] = md[v][m];
         }
@@ -438,12 +438,16 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
     }
     if (a.cfl) {
       const float gm1 = a.gamma - 1.0f;
+      const double e_over = a.p0 / (double)gm1;
+      double ex[NQ], ey[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) { ex[q] = a.exn[3 * i + q]; ey[q] = a.eyn[3 * j + q]; }
 #pragma unroll
       for (int qb = 0; qb < NQ; ++qb)
 #pragma unroll
         for (int qa = 0; qa < NQ; ++qa) {
-          const double e = a.exn[3 * i + qa] * a.eyn[3 * j + qb];
-          const float re = (float)(a.rho0 * e), ee = (float)(a.p0 * e / (double)gm1);
+          const double e = ex[qa] * ey[qb];
+          const float re = (float)(a.rho0 * e), ee = (float)(e_over * e);
           float d[4];
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
@@ -452,10 +456,10 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
                    a.phi[1][qa] * a.phi[1][qb] * mm[4] + a.phi[2][qb] * mm[5];
           }
           const float rho = re + d[0];
-          const float inv = 1.0f / fmaxf(rho, 1e-10f);
+          const float inv = trcp(fmaxf(rho, 1e-10f));
           const float u = d[1] * inv, w = d[2] * inv;
           const float p = gm1 * ((ee + d[3]) - 0.5f * rho * (u * u + w * w));
-          const float cs = sqrtf(a.gamma * fmaxf(p, 1e-10f) * inv);
+          const float cs = tsqrt(a.gamma * fmaxf(p, 1e-10f) * inv);
           const float rt = (fabsf(u) + cs) * a.odx + (fabsf(w) + cs) * a.ody;
           bad |= !(rho > 0.f) || !(p > 0.f) || !(rt < 1e30f);
           rate = fmaxf(rate, rt);

@@ -51,20 +51,22 @@ template <typename T, int C>
 struct Lanes { T v[C]; };
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-template <typename T, int C>
+// AUX is the buffer instruction's cache policy (0 = default).  Nontemporal
+// policies were measured and did not help the stage kernels.
+template <typename T, int C, int AUX = 0>
 __device__ __forceinline__ Lanes<T, C> ldv(const T* row, unsigned boff, int row_bytes) {
   static_assert(C * sizeof(T) == 8 || C * sizeof(T) == 16, "block of 8 or 16 bytes");
   if constexpr (C * sizeof(T) == 16)
-    return __builtin_bit_cast(Lanes<T, C>, __builtin_amdgcn_raw_buffer_load_b128(row_rsrc(row, row_bytes), boff, 0, 0));
+    return __builtin_bit_cast(Lanes<T, C>, __builtin_amdgcn_raw_buffer_load_b128(row_rsrc(row, row_bytes), boff, 0, AUX));
   else
-    return __builtin_bit_cast(Lanes<T, C>, __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(row, row_bytes), boff, 0, 0));
+    return __builtin_bit_cast(Lanes<T, C>, __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(row, row_bytes), boff, 0, AUX));
 }
-template <typename T, int C>
+template <typename T, int C, int AUX = 0>
 __device__ __forceinline__ void stv(T* row, unsigned boff, int row_bytes, Lanes<T, C> x) {
   if constexpr (C * sizeof(T) == 16)
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), row_rsrc(row, row_bytes), boff, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), row_rsrc(row, row_bytes), boff, 0, AUX);
   else
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), row_rsrc(row, row_bytes), boff, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), row_rsrc(row, row_bytes), boff, 0, AUX);
 }
 
 // Deviation state of a lane's C columns on one row.

@@ -7,6 +7,7 @@ program fvwb_abi_check
   use fvwb_iface
   implicit none
   type(fvwb_params) :: p
+  type(fvwb1d_params) :: p1
   type(c_ptr) :: h
   integer(c_int) :: rc
   write(*, '(a,i0)') 'sizeof ', c_sizeof(p)
@@ -18,4 +19,8 @@ program fvwb_abi_check
   rc = fvwb_create(h, p)
   write(*, '(a,i0)') 'create_bad ', rc
   write(*, '(a,a)') 'message ', fvwb_message(fvwb_last_error(c_null_ptr))
+  write(*, '(a,i0)') 'sizeof1d ', c_sizeof(p1)
+  rc = fvwb1d_params_reference(p1, 128_c_int32_t, 2_c_int32_t)
+  write(*, '(a,6(1x,i0))') 'ints1d', p1%nvar, p1%nx, p1%bc, p1%recon, p1%integrator, p1%nequilibrium
+  write(*, '(a,4(1x,z16.16))') 'doubles1d', p1%gamma, p1%boxlen, p1%cfl, p1%g
 end program fvwb_abi_check

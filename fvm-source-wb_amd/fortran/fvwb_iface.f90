@@ -35,6 +35,20 @@ module fvwb_iface
   public :: fvwb_batch_download_deviation, fvwb_batch_get_time
   public :: fvwb_check, fvwb_batch_check, fvwb_message
 
+  ! ---- 1D (benchmark_1d.f90) ----
+  integer(c_int32_t), parameter, public :: FVWB1D_BC_PERIODIC = 1, FVWB1D_BC_ZERO_GRADIENT = 2, &
+       FVWB1D_BC_REFLEXIVE = 3, FVWB1D_BC_WALL = 4
+
+  ! struct fvwb1d_params: six int32 then four doubles (24 + 32 bytes)
+  type, bind(C), public :: fvwb1d_params
+     integer(c_int32_t) :: nvar, nx, bc, recon, integrator, nequilibrium
+     real(c_double) :: gamma, boxlen, cfl, g
+  end type fvwb1d_params
+
+  public :: fvwb1d_params_reference, fvwb1d_create, fvwb1d_destroy, fvwb1d_last_error
+  public :: fvwb1d_compute_update, fvwb1d_compute_max_speed, fvwb1d_upload, fvwb1d_init_gaussian
+  public :: fvwb1d_advance, fvwb1d_download, fvwb1d_get_time, fvwb1d_check
+
   interface
      integer(c_int) function fvwb_params_reference_2d(p, nx, ny, nequilibrium) &
           bind(C, name='fvwb_params_reference_2d')
@@ -184,6 +198,81 @@ module fvwb_iface
        integer(c_int32_t), intent(out) :: steps_done
      end function fvwb_batch_get_time
 
+     integer(c_int) function fvwb1d_params_reference(p, nx, nequilibrium) bind(C, name='fvwb1d_params_reference')
+       import :: c_int, c_int32_t, fvwb1d_params
+       type(fvwb1d_params), intent(out) :: p
+       integer(c_int32_t), value :: nx, nequilibrium
+     end function fvwb1d_params_reference
+
+     integer(c_int) function fvwb1d_create(h, ninst, per_inst) bind(C, name='fvwb1d_create')
+       import :: c_int, c_int32_t, c_ptr, fvwb1d_params
+       type(c_ptr), intent(out) :: h
+       integer(c_int32_t), value :: ninst
+       type(fvwb1d_params), intent(in) :: per_inst(*)
+     end function fvwb1d_create
+
+     integer(c_int) function fvwb1d_destroy(h) bind(C, name='fvwb1d_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function fvwb1d_destroy
+
+     type(c_ptr) function fvwb1d_last_error(h) bind(C, name='fvwb1d_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end function fvwb1d_last_error
+
+     ! drop-in for benchmark_1d.f90:263-377 (compute_update(u,w_eq,dudt), solver 'EQL')
+     integer(c_int) function fvwb1d_compute_update(h, u, w_eq, dudt) bind(C, name='fvwb1d_compute_update')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*), w_eq(*)
+       real(c_double), intent(out) :: dudt(*)
+     end function fvwb1d_compute_update
+
+     ! drop-in for benchmark_1d.f90:157-169 (compute_max_speed(u,cmax))
+     integer(c_int) function fvwb1d_compute_max_speed(h, u, cmax) bind(C, name='fvwb1d_compute_max_speed')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*)
+       real(c_double), intent(out) :: cmax
+     end function fvwb1d_compute_max_speed
+
+     integer(c_int) function fvwb1d_upload(h, inst, u, w_eq) bind(C, name='fvwb1d_upload')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: inst
+       real(c_double), intent(in) :: u(*), w_eq(*)
+     end function fvwb1d_upload
+
+     integer(c_int) function fvwb1d_init_gaussian(h, amp, x0, width) bind(C, name='fvwb1d_init_gaussian')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: amp(*)
+       real(c_double), value :: x0, width
+     end function fvwb1d_init_gaussian
+
+     integer(c_int) function fvwb1d_advance(h, nsteps, t_end) bind(C, name='fvwb1d_advance')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nsteps
+       real(c_double), value :: t_end
+     end function fvwb1d_advance
+
+     integer(c_int) function fvwb1d_download(h, inst, u) bind(C, name='fvwb1d_download')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: inst
+       real(c_double), intent(out) :: u(*)
+     end function fvwb1d_download
+
+     integer(c_int) function fvwb1d_get_time(h, inst, t, dt_last, steps_done) bind(C, name='fvwb1d_get_time')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: inst
+       real(c_double), intent(out) :: t, dt_last
+       integer(c_int32_t), intent(out) :: steps_done
+     end function fvwb1d_get_time
+
      integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s
@@ -233,5 +322,17 @@ contains
        error stop 1
     end if
   end subroutine fvwb_batch_check
+
+  ! Same for a 1D handle.
+  subroutine fvwb1d_check(status, h, what)
+    integer(c_int), intent(in) :: status
+    type(c_ptr), intent(in) :: h
+    character(len=*), intent(in) :: what
+    if (status /= FVWB_OK) then
+       write(0, '(a,a,a,i0,a,a)') 'fvwb: ', what, ' failed with status ', status, ': ', &
+            fvwb_message(fvwb1d_last_error(h))
+       error stop 1
+    end if
+  end subroutine fvwb1d_check
 
 end module fvwb_iface

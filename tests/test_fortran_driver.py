@@ -63,6 +63,12 @@ def test_fortran_params_struct_matches_c_view():
     assert doubles[0] == float(np.float32(1.4)) and doubles[4] == float(np.float32(1.21))
     assert int(lines["create_bad"]) == fvwb.ERR_ARG
     assert "nx" in lines["message"]
+    assert int(lines["sizeof1d"]) == ctypes.sizeof(fvwb.Params1D) == 56
+    c1 = fvwb.reference_params_1d(128, 2)
+    assert [int(v) for v in lines["ints1d"].split()] == [c1.nvar, c1.nx, c1.bc, c1.recon, c1.integrator,
+                                                          c1.nequilibrium]
+    d1 = [np.frombuffer(bytes.fromhex(h), dtype=">f8")[0] for h in lines["doubles1d"].split()]
+    assert d1 == [c1.gamma, c1.boxlen, c1.cfl, c1.g]
 
 
 def test_driver_binds_dropins_to_library():
@@ -74,6 +80,15 @@ def test_driver_binds_dropins_to_library():
     assert {"compute_update_exact_", "compute_max_speed_"} <= defined
     for sym in ("fvwb_compute_update_exact", "fvwb_compute_max_speed", "fvwb_create", "fvwb_upload",
                 "fvwb_advance", "fvwb_download", "fvwb_last_error"):
+        assert sym in undefined and sym in fvwb.declared_symbols()
+
+
+def test_1d_driver_binds_dropins_to_library():
+    nm = subprocess.run(["nm", exe("benchmark_1d_fvwb")], capture_output=True, text=True, check=True).stdout
+    defined = set(re.findall(r"^[0-9a-f]+ T (\S+)$", nm, re.M))
+    undefined = set(re.findall(r"^\s+U (\S+)$", nm, re.M))
+    assert {"compute_update_", "compute_max_speed_"} <= defined
+    for sym in ("fvwb1d_compute_update", "fvwb1d_compute_max_speed", "fvwb1d_create", "fvwb1d_advance"):
         assert sym in undefined and sym in fvwb.declared_symbols()
 
 
@@ -153,3 +168,58 @@ def test_reference_setup_stays_exactly_at_rest(tmp_path, mode):
     assert np.max(np.abs(vals)) <= 4e-16
     x0, y0 = (float(v) for v in lines[1].split()[:2])
     assert abs(x0 - 0.5 / 300) < 1e-8 and abs(y0 - 1.5 / 300) < 1e-8
+
+
+def read_state_1d(path):
+    with open(path, "rb") as f:
+        nx, nvar, steps = np.frombuffer(f.read(12), dtype=np.int32)
+        t = float(np.frombuffer(f.read(8), dtype=np.float64)[0])
+        n = int(nvar) * int(nx)
+        arrs = np.frombuffer(f.read(3 * n * 8), dtype=np.float64)
+    shape = (int(nvar), int(nx))
+    u0, w_eq, u = (arrs[k * n:(k + 1) * n].reshape(shape, order="F") for k in range(3))
+    return int(steps), t, u0, w_eq, u
+
+
+@pytest.mark.gpu
+def test_1d_driver_stage_loop_matches_oracle_bitwise(tmp_path):
+    """nx=64, ninit=2: the driver's Heun loop over the GPU drop-ins equals the
+    same loop over the oracle's restatement of benchmark_1d.f90, bit for bit
+    (the drop-ins and the oracle round identically, and the loop arithmetic is
+    elementwise without contraction on both sides)."""
+    tend = 0.05
+    prefix = str(tmp_path / "s1")
+    r = run([exe("benchmark_1d_fvwb_64"), "stage", repr(tend), prefix])
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps, t, u0, w_eq, u = read_state_1d(prefix + ".bin")
+    p = po.make_params_1d(64)
+    dx = 1.0 / 64
+    uo, to, n = np.asfortranarray(u0.copy()), 0.0, 0
+    while to < tend:
+        cmax = po.ref1d_max_speed(p, uo)
+        dt = float(np.float32(0.8)) * dx / cmax / (2.0 * 1.0 + 1.0)
+        d = po.ref1d_update(p, uo, w_eq)
+        w1 = uo + dt * d
+        d = po.ref1d_update(p, w1, w_eq)
+        uo = 0.5 * uo + 0.5 * w1 + 0.5 * dt * d
+        to, n = to + dt, n + 1
+    print(f"steps {steps} vs {n}, t {t} vs {to}, max|du| {np.max(np.abs(u - uo)):.3e}", flush=True)
+    assert steps == n and t == to
+    assert np.array_equal(u, uo)
+
+
+@pytest.mark.gpu
+def test_1d_driver_resident_run_matches_oracle_bitwise(tmp_path):
+    """Resident mode runs the configured scheme (MUSCL, Rusanov, SSP-RK2, walls,
+    CFL 0.5) to tend in one launch: equal to the oracle's run bit for bit."""
+    tend = 0.1
+    prefix = str(tmp_path / "r1")
+    r = run([exe("benchmark_1d_fvwb"), "resident", repr(tend), prefix])
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps, t, u0, w_eq, u = read_state_1d(prefix + ".bin")
+    p = po.make_params_1d(128, recon=po.RECON_MUSCL_MINMOD, bc=po.BC1D_WALL, integrator=po.RK2_HEUN, cfl=0.5)
+    uo, to, no, _ = po.fv1d_advance(p, u0, w_eq, t_end=tend)
+    assert steps == no and t == to == tend
+    assert np.array_equal(u, uo)
+    lines = open(prefix + "_final.dat").read().splitlines()
+    assert len(lines) == 128 and all(re.fullmatch(r"(?:[ -]\d\.\d{5}E[+-]\d\d ?){2}", ln) for ln in lines)
