@@ -220,6 +220,7 @@ def main():
     ap.add_argument("--fp32-only", action="store_true", help="time the FP32 batch only (no JSON line)")
     ap.add_argument("--dg-size", type=int, default=1024, help="cells per side of the DG run (0: skip)")
     ap.add_argument("--fv1d-batch", type=int, default=4096, help="1D instances in the batched 1D run (0: skip)")
+    ap.add_argument("--dg-only", action="store_true", help="time the DG workload only (no JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -227,6 +228,11 @@ def main():
         sys.exit(2)
     ev = HipEvents()
     K, W = args.steps, args.warmup
+    if args.dg_only:       # profiling runs of the DG kernels alone
+        sec_dg = time_dg(ev, args.dg_size, K, W)
+        log(f"dg-only {args.dg_size}^2: {args.dg_size ** 2 * K / sec_dg:.4e} cell-updates/s, "
+            f"{sec_dg * 1e3 / K:.3f} ms/step")
+        return
 
     # ---- headline: the 512-instance batch -------------------------------------------
     sides, gs, amps = batch_workload()

@@ -6,18 +6,27 @@
 // dg_with_source.f90:1749-2031: all fluxes and the source are deviations from
 // the equilibrium's, so a resting atmosphere is an exact fixed point.
 //
-// The projections are small dense products per cell and run on the matrix
-// cores as v_mfma_f32_16x16x4f32 with 16 cells in the N dimension:
-//   forward:  [16 points x 8 modes] x [8 modes x 16 cells]  (volume nodes, and
-//             separately the 12 face points)
-//   backward: [16 modes x 48 inputs] x [48 inputs x 16 cells] for the 27 volume
-//             inputs (F_x/dx, F_y/dy, S at 9 nodes, zero-padded to 16 each) and
-//             [16 x 12] x [12 x 16] for the face fluxes.
-// Lane l of the output holds points (or modes) 4*(l/16)+r of cell l%16, so the
-// point-wise flux work is lane-local and each lane feeds its own outputs into
-// the backward product as its k-slot (the k order of a product is free).
-// A workgroup owns a 16x16 tile; its 1-cell halo only contributes face traces.
-// Face traces are exchanged through LDS, where the face fluxes are computed.
+// Work decomposition: one wave sweeps a strip of 16 columns (14 owned plus one
+// halo column on each side) upward through a segment of rows, one row of 16
+// cells per step.  The projections of a row are small dense products and run on
+// the matrix cores as v_mfma_f32_16x16x4f32 with the 16 cells in the N
+// dimension:
+//   forward:  [16 points x 8 modes] x [8 modes x 16 cells], once for the 9
+//             volume nodes and once for the 12 face points;
+//   backward: [16 outputs x 4 inputs] x [4 inputs x 16 cells] per step, 6 steps
+//             for the flux terms at the nodes and 4 for the face fluxes.
+// Lane l = 16 g + c handles cell c; in the forward products it receives node
+// (r, g) (r = 0..2) and the left, right, bottom and top traces at Gauss point g
+// (g = 0..2); in the backward products it feeds exactly those values as k-slot
+// g, so the point-wise flux work stays in the lane.  The output rows of the
+// backward products are ordered so that lane (g, c) receives modes g and g+4 of
+// cell c: the layout in which it loaded them, so the stage combination needs no
+// data movement.  x neighbours are one DPP row shift away; the trace of the row
+// below is carried in registers from the previous step.  No LDS, no barriers.
+//
+// The source deviation -g (0, drho, drho, dmx + dmy) is linear in the state and
+// the 3-point Gauss rule integrates phi_i phi_j exactly, so its projection is
+// -g times the modes themselves and is added directly.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -36,18 +45,26 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int NM = 6;      // P2 modes
 constexpr int NQ = 3;      // Gauss points per direction
-constexpr int TILE = 16;   // cells per tile side
+constexpr int SW = 14;     // owned columns per strip (16 lanes, one halo column each side)
+constexpr int GROWS = 2;   // ghost rows (one below row 0, one above row ny-1)
+constexpr unsigned MASKED = 0x80000000u;   // buffer offset past every extent: loads return 0, stores drop
 
-// Per-lane MFMA operand constants, built on the host from the basis.
+// Per-lane MFMA operand constants, built on the host from the basis.  Output row
+// 4g + s of a backward product is mode g (s = 0) or mode g + 4 (s = 1, g < 2).
 struct Consts {
-  float fwd_node[16][8];   // V[row][mode], row 4g + r -> Gauss node g + 4r (node a + 3b; 9 used)
-  float fwd_face[16][8];   // trace points: 0-2 left (xi=-1, eta_b), 3-5 right, 6-8 bottom (xi_a, eta=-1), 9-11 top
-  float back_vol[3][16][16];  // [quantity][mode][row as fwd_node]: 0.5 w dphi_i phi_j (F_x), 0.5 w phi_i dphi_j (F_y), 0.25 w phi phi (S)
-  float back_face[16][12];    // [mode][face point]: +-0.5 w phi (flux pre-scaled by 1/dx or 1/dy)
+  float fwd_node[16][8];        // [row 4g + r: node (x_r, y_g)][mode]; rows with g or r = 3 are zero
+  float fwd_face[16][8];        // [row 4q + s: point q, s = 0 left, 1 right, 2 bottom, 3 top][mode]
+  float back_vol[2][3][16][4];  // [F_x | F_y][step r][output row][k = g: node (x_r, y_g)]
+  float back_face[4][16][4];    // [left | right | bottom | top][output row][k = q]
 };
 
+// The CFL rate max((|u|+c)/dx + (|v|+c)/dy) of parity p is the maximum of
+// rate[p][0..RSLOTS-1]: workgroup b reduces into slot b % RSLOTS, so the atomics
+// of the grid spread over RSLOTS addresses instead of queueing on one.
+constexpr int RSLOTS = 8;
+
 struct Ctrl {
-  uint32_t rate[2];   // FP32 bit patterns of max((|u|+c)/dx + (|v|+c)/dy); positive floats order like uints
+  uint32_t rate[2][RSLOTS];   // FP32 bit patterns; positive floats order like uints
   double t[2];
   double dt_last;
   int32_t steps;
@@ -55,29 +72,33 @@ struct Ctrl {
 };
 
 struct Args {
-  const float* in;     // stage input planes (24 = 4 vars x 6 modes)
+  const float* in;     // stage input planes (24 = 4 vars x 6 modes), base of the ghost frame
   const float* un;     // U^n planes (stage combination), may be null when alpha == 0
   float* out;          // output planes (may alias un)
   const Consts* C;
-  const double* exn;   // e^{-k x} at x nodes   [3*nx]
-  const double* eyn;   // e^{-k y} at y nodes   [3*ny]
-  const double* exf;   // e^{-k x} at x faces   [nx+1]
-  const double* eyf;   // e^{-k y} at y faces   [ny+1]
+  const float* exn;    // e^{-k x} at x nodes   [3*nx]
+  const float* eyn;    // e^{-k y} at y nodes   [3*ny]
+  const float* exf;    // e^{-k x} at x faces   [nx+1]
+  const float* eyf;    // e^{-k y} at y faces   [ny+1]
   Ctrl* ctrl;
   long long plane;
   int pitch, nx, ny, parity;
+  int ntx, nseg, seg_rows;   // strips across x, segments along y, rows per segment
   int first, residual_only;
-  double t_end, cfl_over;   // cfl / (2k+1)
+  double t_end, cfl_over;    // cfl / (2k+1)
   float alpha, beta;
   float gamma, g, odx, ody;
-  double rho0, p0, gm1d;
+  float rho0, p0, ep0;       // rho0, p0 and p0 gamma/(gamma-1): equilibrium rho, p and E+p per e^{-k(x+y)}
 };
 
-__device__ inline float ld_mode(const float* base, long long plane, int pitch, int v, int m, int i, int j) {
-  return base[(long long)(v * NM + m) * plane + (long long)j * pitch + i];
+__device__ inline float ctrl_rate(const Ctrl* c, int par) {
+  uint32_t m = 0u;
+#pragma unroll
+  for (int k = 0; k < RSLOTS; ++k) m = max(m, c->rate[par][k]);
+  return __uint_as_float(m);
 }
 
-// Deviation fluxes along x and y and the wave-speed rate at one point.
+// Deviation fluxes along x and y at one point.
 __device__ inline void point_terms(const Phys<float>& P, const FaceEq<float>& q, Vec4<float> d, Vec4<float>& fx,
                                    Vec4<float>& fy) {
   float rho = q.rho + d.r;
@@ -89,282 +110,255 @@ __device__ inline void point_terms(const Phys<float>& P, const FaceEq<float>& q,
   fy = {d.my, d.mx * v, d.my * v + dp, ep * v};
 }
 
-// Equilibrium at a point from the separable FP64 tables, rounded once to FP32.
-__device__ inline FaceEq<float> eq_at(const Args& a, double ex, double ey) {
-  const double e = ex * ey, pe = a.p0 * e, ee = pe / a.gm1d;
+// Equilibrium at a point from the separable FP32 tables.
+__device__ inline FaceEq<float> eq_at(const Args& a, float e) {
   FaceEq<float> q;
-  q.rho = (float)(a.rho0 * e);
-  q.p = (float)pe;
-  q.e = (float)ee;
-  q.ep = (float)(ee + pe);
+  q.rho = a.rho0 * e;
+  q.p = a.p0 * e;
+  q.e = 0.f;           // not used by the deviation fluxes
+  q.ep = a.ep0 * e;
   return q;
 }
 
-// Lane-local node assignment of the forward volume product: row 4*g + r of the
-// node block (g = lane/16) is Gauss node g + 4r, so the nine nodes spread as
-// 3,2,2,2 over the four lane groups and the backward volume product needs only
-// r = 0..2 (9 steps of 4 k-slots) instead of 12.
-__host__ __device__ constexpr int node_of_row(int row) { return (row >> 2) + 4 * (row & 3); }
+// DPP moves inside each row of 16 lanes (one row = the 16 cells of a strip):
+// lane c receives lane c-1 (c+1); the lane at the row's edge receives 0.
+__device__ __forceinline__ float from_col_left(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x111 /*row_shr:1*/,
+                                                               0xF, 0xF, true));
+}
+__device__ __forceinline__ float from_col_right(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x101 /*row_shl:1*/,
+                                                               0xF, 0xF, true));
+}
+
+// Buffer resource over one row of one variable's six mode planes; offsets past
+// nrec bytes read 0 and drop stores.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mode_rsrc(const float* p, int nrec) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, nrec, 0x00020000);
+}
+__device__ __forceinline__ float ldm(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void stm(__amdgpu_buffer_rsrc_t r, unsigned off, float x) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, off, 0, 0);
+}
+
+__device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; renumber them so that each XCD
+// receives a contiguous range of strips (neighbouring halo columns share its L2).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  int xcd = b & 7, idx = b >> 3, base = nb >> 3, rem = nb & 7;
+  return xcd * base + (xcd < rem ? xcd : rem) + idx;
+}
 
 template <int FLUX>
 __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   const Ctrl* ck = a.ctrl;
   const int par = a.parity;
-  const double t = ck->t[par];
-  if (!a.residual_only && !(t < a.t_end)) return;       // this instance is done
   double dt = 0.0;
   if (!a.residual_only) {
-    const float rate = __uint_as_float(ck->rate[par]);
-    dt = a.cfl_over / (double)rate;
+    const double t = ck->t[par];
+    if (!(t < a.t_end)) return;       // this instance is done
+    dt = a.cfl_over / (double)ctrl_rate(ck, par);
     if (t + dt > a.t_end) dt = a.t_end - t;
-    if (a.first && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.ctrl->rate[par ^ 1] = 0u;
+    if (a.first && blockIdx.x == 0 && threadIdx.x < RSLOTS) a.ctrl->rate[par ^ 1][threadIdx.x] = 0u;
   }
-  const float dtf = (float)dt;
-
-  // face traces, then face fluxes: the innermost index runs along the lanes
-  __shared__ float XT[2][4][TILE][NQ][TILE + 1];   // x faces: [side][var][row][point][face]
-  __shared__ float YT[2][4][TILE + 1][NQ][TILE];   // y faces: [side][var][face][point][col]
-  // equilibrium tables of the tile (halo included): e^{-kx}, e^{-ky} at nodes and faces
-  __shared__ double Exn[3 * TILE], Eyn[3 * TILE], Exf[TILE + 1], Eyf[TILE + 1];
-
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int c16 = lane & 15, g4 = lane >> 4;
-  const int tx0 = blockIdx.x * TILE, ty0 = blockIdx.y * TILE;
-  const Consts& C = *a.C;
-
-  {
-    const int q = threadIdx.x;
-    if (q < 3 * TILE) {
-      const int ci = min(tx0 + q / 3, a.nx - 1), cj = min(ty0 + q / 3, a.ny - 1);
-      Exn[q] = a.exn[3 * ci + q % 3];
-      Eyn[q] = a.eyn[3 * cj + q % 3];
-    } else if (q < 3 * TILE + TILE + 1) {
-      const int f = q - 3 * TILE;
-      Exf[f] = a.exf[min(tx0 + f, a.nx)];
-      Eyf[f] = a.eyf[min(ty0 + f, a.ny)];
-    }
-  }
+  const int w = xcd_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (w >= a.ntx * a.nseg) return;
+  const int strip = w % a.ntx, seg = w / a.ntx;
+  const int y0 = seg * a.seg_rows, y1 = min(y0 + a.seg_rows, a.ny);
+  const int lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+  const int gq = min(g, 2);                   // Gauss point of this lane group (group 3 idles)
+  const int i = strip * SW - 1 + c;           // column of this lane
+  const bool own = c >= 1 && c <= SW && i < a.nx;
+  const int nx = a.nx, ny = a.ny;
+  const long long plane = a.plane;
+  const int pitch = a.pitch;
+  // accumulator acc = U + dt L (the backward operands carry the factor dt),
+  // out = alpha U^n + beta acc; the residual alone is acc with dt = 1 and no U
+  const float sdt = a.residual_only ? 1.f : (float)dt;
+  const float ku = a.residual_only ? 0.f : 1.f;
+  const float gdt = a.g * sdt;
+  const float alpha = a.residual_only ? 0.f : a.alpha;
+  const float beta = a.residual_only ? 1.f : a.beta;
+  const bool with_un = !a.residual_only && a.alpha != 0.f;
 
   Phys<float> P;
   P.gamma = a.gamma; P.gm1 = a.gamma - 1.0f; P.half = 0.5f; P.g = a.g;
 
-  // all mode loads of the tile up front: lane supplies mode g4 (+4) of cell c16
-  float B0[4][4], B1[4][4], N0[4][4], N1[4][4];   // [row group][var]: stage input and U^n
-  const int i = tx0 + c16;
-#pragma unroll
-  for (int rr = 0; rr < 4; ++rr) {
-    const int j = ty0 + 4 * wave + rr;
-    const bool ok = j < a.ny && i < a.nx;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      B0[rr][v] = ok ? ld_mode(a.in, a.plane, a.pitch, v, g4, i, j) : 0.f;
-      B1[rr][v] = (ok && g4 + 4 < NM) ? ld_mode(a.in, a.plane, a.pitch, v, g4 + 4, i, j) : 0.f;
-      const bool okn = ok && a.alpha != 0.f;
-      N0[rr][v] = okn ? ld_mode(a.un, a.plane, a.pitch, v, g4, i, j) : 0.f;
-      N1[rr][v] = (okn && g4 + 4 < NM) ? ld_mode(a.un, a.plane, a.pitch, v, g4 + 4, i, j) : 0.f;
-    }
-  }
-  // halo cell of this wave: below, above, left or right of the tile
-  int hi, hj;
-  bool hok;
-  if (wave == 0) { hi = tx0 + c16; hj = ty0 - 1; hok = ty0 >= 1; }
-  else if (wave == 1) { hi = tx0 + c16; hj = ty0 + TILE; hok = ty0 + TILE < a.ny; }
-  else if (wave == 2) { hi = tx0 - 1; hj = ty0 + c16; hok = tx0 >= 1; }
-  else { hi = tx0 + TILE; hj = ty0 + c16; hok = tx0 + TILE < a.nx; }
-  hok = hok && hi < a.nx && hj < a.ny;
-  float H0[4], H1[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    H0[v] = hok ? ld_mode(a.in, a.plane, a.pitch, v, g4, hi, hj) : 0.f;
-    H1[v] = (hok && g4 + 4 < NM) ? ld_mode(a.in, a.plane, a.pitch, v, g4 + 4, hi, hj) : 0.f;
-  }
-
-  // forward operands: lane supplies V[row = lane%16][mode = lane/16 + 4s]
-  const float An0 = C.fwd_node[c16][g4], An1 = C.fwd_node[c16][g4 + 4];
-  const float Af0 = C.fwd_face[c16][g4], Af1 = C.fwd_face[c16][g4 + 4];
-  // backward operands: lane supplies B[mode = lane%16][row = 4*(lane/16) + r]
-  float Abv[3][3], Abf[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) Abv[q][r] = C.back_vol[q][c16][4 * g4 + r];
-#pragma unroll
-  for (int s = 0; s < 3; ++s) Abf[s] = C.back_face[c16][4 * s + g4];
-
-  __syncthreads();   // equilibrium tables
-
-  v4f acc[4][4];   // [row group][var]: D[mode 4*(l/16)+r][cell l%16]
-
-  // ---- phase 1: interior rows (volume terms + traces) --------------------------------
-#pragma unroll
-  for (int rr = 0; rr < 4; ++rr) {
-    const int jl = 4 * wave + rr;
-    v4f dn[4], df[4];
+  // lane offsets (bytes) of modes g and g+4 of column i inside a row of a variable
+  const unsigned colb = (unsigned)(i + 1) * 4u;
+  const unsigned off0 = (unsigned)g * (unsigned)plane * 4u + colb;
+  const unsigned off1 = g + 4 < NM ? (unsigned)(g + 4) * (unsigned)plane * 4u + colb : MASKED;
+  const unsigned st0 = own ? off0 : MASKED, st1 = own ? off1 : MASKED;
+  const int nrec = (int)((5 * plane + pitch) * 4);
+  auto rs = [&](const float* base, int v, int j) {
+    return mode_rsrc(base + (long long)(v * NM) * plane + (long long)(j + 1) * pitch, nrec);
+  };
+  auto load_row = [&](const float* base, int j, float (&B)[4][2]) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      v4f z = {0.f, 0.f, 0.f, 0.f};
-      dn[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(An0, B0[rr][v], z, 0, 0, 0);
-      dn[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(An1, B1[rr][v], dn[v], 0, 0, 0);
-      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af0, B0[rr][v], z, 0, 0, 0);
-      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af1, B1[rr][v], df[v], 0, 0, 0);
+      const __amdgpu_buffer_rsrc_t r = rs(base, v, j);
+      B[v][0] = ldm(r, off0);
+      B[v][1] = ldm(r, off1);
     }
-    // traces to LDS: lane holds trace points 4*g4 + r of cell c16
+  };
+
+  // U^n of the owned cells only (halo lanes read 0)
+  auto load_own = [&](const float* base, int j, float (&B)[4][2]) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int pt = 4 * g4 + r;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float val = df[v][r];
-        if (pt < 3) XT[1][v][jl][pt][c16] = val;                     // left trace -> x face c16, right side
-        else if (pt < 6) XT[0][v][jl][pt - 3][c16 + 1] = val;        // right trace -> x face c16+1, left side
-        else if (pt < 9) YT[1][v][jl][pt - 6][c16] = val;            // bottom trace -> y face jl, upper side
-        else if (pt < 12) YT[0][v][jl + 1][pt - 9][c16] = val;       // top trace -> y face jl+1, lower side
-      }
+    for (int v = 0; v < 4; ++v) {
+      const __amdgpu_buffer_rsrc_t r = rs(base, v, j);
+      B[v][0] = ldm(r, st0);
+      B[v][1] = ldm(r, st1);
     }
-    // volume terms at the lane's nodes g4, g4+4, g4+8
-    float X[3][3][4];   // [quantity][r][var]
+  };
+
+  // MFMA operands
+  const Consts& C = *a.C;
+  const float An0 = C.fwd_node[c][g], An1 = C.fwd_node[c][g + 4];
+  const float Af0 = C.fwd_face[c][g], Af1 = C.fwd_face[c][g + 4];
+  float Abx[3], Aby[3], Abf[4];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) { Abx[r] = sdt * C.back_vol[0][r][c][g]; Aby[r] = sdt * C.back_vol[1][r][c][g]; }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) Abf[s] = sdt * C.back_face[s][c][g];
+
+  // equilibrium factors of this column: x nodes, the left x face, x node gq
+  const int ic = min(max(i, 0), nx - 1), ifc = min(max(i, 0), nx);
+  float exr[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) exr[r] = a.exn[3 * ic + r];
+  const float exf_i = a.exf[ifc];
+  const float exg = a.exn[3 * ic + gq];
+
+  auto fwd_face = [&](const float (&B)[4][2], v4f (&df)[4]) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const v4f z = {0.f, 0.f, 0.f, 0.f};
+      df[v] = mfma(Af0, B[v][0], z);
+      df[v] = mfma(Af1, B[v][1], df[v]);
+    }
+  };
+  auto mirror_x = [](Vec4<float> d) { return Vec4<float>{d.r, -d.mx, d.my, d.e}; };
+  auto mirror_y = [](Vec4<float> d) { return Vec4<float>{d.r, d.mx, -d.my, d.e}; };
+  // y-face flux at Gauss point gq below row j (lower state dl, upper dr), times 1/dy
+  auto flux_y = [&](int j, Vec4<float> dl, Vec4<float> dr) {
+    const FaceEq<float> q = eq_at(a, exg * a.eyf[j]);
+    const Vec4<float> F = face_flux<float, FLUX, 1, FORM_DEV>(P, q, dl, dr);
+    return Vec4<float>{F.r * a.ody, F.mx * a.ody, F.my * a.ody, F.e * a.ody};
+  };
+  // finish row j: add the flux through its top face, combine and store
+  auto finish = [&](int j, v4f (&acc)[4], const Vec4<float>& Ft, const float (&N)[4][2]) {
+    const float ft[4] = {Ft.r, Ft.mx, Ft.my, Ft.e};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] = mfma(Abf[3], ft[v], acc[v]);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const __amdgpu_buffer_rsrc_t r = rs(a.out, v, j);
+      stm(r, st0, alpha * N[v][0] + beta * acc[v][0]);
+      stm(r, st1, alpha * N[v][1] + beta * acc[v][1]);
+    }
+  };
+
+  // ---- prologue: top traces of the row below the segment, first row's modes ------
+  float Bc[4][2], Bn[4][2], N[4][2];
+  Vec4<float> Tt = {0.f, 0.f, 0.f, 0.f};   // top trace (point gq) of the row below the current one
+  load_row(a.in, y0, Bc);
+  if (y0 > 0) {
+    float Bh[4][2];
+    load_row(a.in, y0 - 1, Bh);
+    v4f df[4];
+    fwd_face(Bh, df);
+    Tt = {df[0][3], df[1][3], df[2][3], df[3][3]};
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) N[v][0] = N[v][1] = 0.f;
+  v4f accp[4];
+
+  for (int j = y0; j < y1; ++j) {
+    // loads in flight during this row: the next row, and U^n of the row finished here
+    load_row(a.in, j + 1, Bn);
+    if (with_un && j > y0) load_own(a.un, j - 1, N);
+
+    v4f dn[4], df[4], acc[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const v4f z = {0.f, 0.f, 0.f, 0.f};
+      dn[v] = mfma(An0, Bc[v][0], z);
+      dn[v] = mfma(An1, Bc[v][1], dn[v]);
+    }
+    fwd_face(Bc, df);
+    // accumulator start: U plus dt times the modal source -g (0, rho, rho, mx + my)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      acc[0][s] = ku * Bc[0][s];
+      acc[1][s] = ku * Bc[1][s] - gdt * Bc[0][s];
+      acc[2][s] = ku * Bc[2][s] - gdt * Bc[0][s];
+      acc[3][s] = ku * Bc[3][s] - gdt * (Bc[1][s] + Bc[2][s]);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v][2] = acc[v][3] = 0.f;
+
+    // volume: fluxes at nodes (x_r, y_gq), fed back as k-slot g
+    const float* eyr = a.eyn + 3 * j;
+    const float eyg = g == 0 ? eyr[0] : (g == 1 ? eyr[1] : eyr[2]);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      const int node = g4 + 4 * r;
-      if (node < 9) {
-        const int qa = node % 3, qb = node / 3;
-        const double e = Exn[3 * c16 + qa] * Eyn[3 * jl + qb], pe = a.p0 * e, ee = pe / a.gm1d;
-        FaceEq<float> q;
-        q.rho = (float)(a.rho0 * e); q.p = (float)pe; q.e = (float)ee; q.ep = (float)(ee + pe);
-        Vec4<float> d = {dn[0][r], dn[1][r], dn[2][r], dn[3][r]};
-        Vec4<float> fx, fy;
-        point_terms(P, q, d, fx, fy);
-        X[0][r][0] = fx.r * a.odx; X[0][r][1] = fx.mx * a.odx; X[0][r][2] = fx.my * a.odx; X[0][r][3] = fx.e * a.odx;
-        X[1][r][0] = fy.r * a.ody; X[1][r][1] = fy.mx * a.ody; X[1][r][2] = fy.my * a.ody; X[1][r][3] = fy.e * a.ody;
-        X[2][r][0] = 0.f; X[2][r][1] = -a.g * d.r; X[2][r][2] = -a.g * d.r; X[2][r][3] = -a.g * (d.mx + d.my);
-      } else {
+      const FaceEq<float> q = eq_at(a, exr[r] * eyg);
+      Vec4<float> fx, fy;
+      point_terms(P, q, Vec4<float>{dn[0][r], dn[1][r], dn[2][r], dn[3][r]}, fx, fy);
+      const float X[4] = {fx.r * a.odx, fx.mx * a.odx, fx.my * a.odx, fx.e * a.odx};
+      const float Y[4] = {fy.r * a.ody, fy.mx * a.ody, fy.my * a.ody, fy.e * a.ody};
 #pragma unroll
-        for (int q = 0; q < 3; ++q)
+      for (int v = 0; v < 4; ++v) acc[v] = mfma(Abx[r], X[v], acc[v]);
 #pragma unroll
-          for (int v = 0; v < 4; ++v) X[q][r][v] = 0.f;
-      }
+      for (int v = 0; v < 4; ++v) acc[v] = mfma(Aby[r], Y[v], acc[v]);
     }
+
+    // x face on the left of column i at point gq: right trace of column i-1 | left trace of i
+    Vec4<float> xl = {from_col_left(df[0][1]), from_col_left(df[1][1]), from_col_left(df[2][1]),
+                      from_col_left(df[3][1])};
+    Vec4<float> xr = {df[0][0], df[1][0], df[2][0], df[3][0]};
+    if (i == 0) xl = mirror_x(xr);
+    if (i == nx) xr = mirror_x(xl);
+    const Vec4<float> Fx = face_flux<float, FLUX, 0, FORM_DEV>(P, eq_at(a, exf_i * eyg), xl, xr);
+    const float fl[4] = {Fx.r * a.odx, Fx.mx * a.odx, Fx.my * a.odx, Fx.e * a.odx};
+    // y face below row j at point gq: top trace of row j-1 | bottom trace of row j
+    const Vec4<float> yr = {df[0][2], df[1][2], df[2][2], df[3][2]};
+    const Vec4<float> Fb = flux_y(j, j == 0 ? mirror_y(yr) : Tt, yr);
+    const float fb[4] = {Fb.r, Fb.mx, Fb.my, Fb.e};
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      v4f z = {0.f, 0.f, 0.f, 0.f};
-      acc[rr][v] = z;
+      acc[v] = mfma(Abf[0], fl[v], acc[v]);
+      acc[v] = mfma(Abf[1], from_col_right(fl[v]), acc[v]);   // right face = left face of column i+1
+      acc[v] = mfma(Abf[2], fb[v], acc[v]);
+    }
+    // the face below row j is the top face of row j-1
+    if (j > y0) finish(j - 1, accp, Fb, N);
+    Tt = {df[0][3], df[1][3], df[2][3], df[3][3]};
 #pragma unroll
-      for (int q = 0; q < 3; ++q)
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-          acc[rr][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Abv[q][r], X[q][r][v], acc[rr][v], 0, 0, 0);
+    for (int v = 0; v < 4; ++v) {
+      accp[v] = acc[v];
+      Bc[v][0] = Bn[v][0];
+      Bc[v][1] = Bn[v][1];
     }
   }
 
-  // ---- phase 1b: halo traces ------------------------------------------------------
-  {
+  // ---- epilogue: the top face of the segment's last row --------------------------
+  if (with_un) load_own(a.un, y1 - 1, N);
+  Vec4<float> yr;
+  if (y1 < ny) {
     v4f df[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      v4f z = {0.f, 0.f, 0.f, 0.f};
-      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af0, H0[v], z, 0, 0, 0);
-      df[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af1, H1[v], df[v], 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int pt = 4 * g4 + r;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float val = df[v][r];
-        if (wave == 0 && pt >= 9 && pt < 12) YT[0][v][0][pt - 9][c16] = val;       // top trace of the row below
-        if (wave == 1 && pt >= 6 && pt < 9) YT[1][v][TILE][pt - 6][c16] = val;     // bottom trace of the row above
-        if (wave == 2 && pt >= 3 && pt < 6) XT[0][v][c16][pt - 3][0] = val;        // right trace of the left column
-        if (wave == 3 && pt < 3) XT[1][v][c16][pt][TILE] = val;                     // left trace of the right column
-      }
-    }
+    fwd_face(Bc, df);      // Bc holds row y1
+    yr = {df[0][2], df[1][2], df[2][2], df[3][2]};
+  } else {
+    yr = mirror_y(Tt);
   }
-  __syncthreads();
-
-  // ---- phase 2: face fluxes (minus equilibrium flux), scaled by 1/dx or 1/dy ----------
-  for (int it = threadIdx.x; it < (TILE + 1) * TILE * NQ; it += 256) {
-    // x face f (innermost), row, point
-    {
-      const int f = it % (TILE + 1), pt = (it / (TILE + 1)) % NQ, row = it / ((TILE + 1) * NQ);
-      const int fi = tx0 + f, j = ty0 + row;
-      if (fi <= a.nx && j < a.ny) {
-        const bool hasl = fi >= 1, hasr = fi <= a.nx - 1;
-        Vec4<float> dl = {XT[0][0][row][pt][f], XT[0][1][row][pt][f], XT[0][2][row][pt][f], XT[0][3][row][pt][f]};
-        Vec4<float> dr = {XT[1][0][row][pt][f], XT[1][1][row][pt][f], XT[1][2][row][pt][f], XT[1][3][row][pt][f]};
-        if (!hasl) dl = {dr.r, -dr.mx, dr.my, dr.e};
-        if (!hasr) dr = {dl.r, -dl.mx, dl.my, dl.e};
-        const double e = Exf[f] * Eyn[3 * row + pt], pe = a.p0 * e, ee = pe / a.gm1d;
-        FaceEq<float> q;
-        q.rho = (float)(a.rho0 * e); q.p = (float)pe; q.e = (float)ee; q.ep = (float)(ee + pe);
-        Vec4<float> F = face_flux<float, FLUX, 0, FORM_DEV>(P, q, dl, dr);
-        XT[0][0][row][pt][f] = F.r * a.odx; XT[0][1][row][pt][f] = F.mx * a.odx;
-        XT[0][2][row][pt][f] = F.my * a.odx; XT[0][3][row][pt][f] = F.e * a.odx;
-      }
-    }
-    // y face f, point, column (innermost)
-    {
-      const int col = it % TILE, pt = (it / TILE) % NQ, f = it / (TILE * NQ);
-      const int fj = ty0 + f, ii = tx0 + col;
-      if (fj <= a.ny && ii < a.nx) {
-        const bool hasb = fj >= 1, hast = fj <= a.ny - 1;
-        Vec4<float> dl = {YT[0][0][f][pt][col], YT[0][1][f][pt][col], YT[0][2][f][pt][col], YT[0][3][f][pt][col]};
-        Vec4<float> dr = {YT[1][0][f][pt][col], YT[1][1][f][pt][col], YT[1][2][f][pt][col], YT[1][3][f][pt][col]};
-        if (!hasb) dl = {dr.r, dr.mx, -dr.my, dr.e};
-        if (!hast) dr = {dl.r, dl.mx, -dl.my, dl.e};
-        const double e = Exn[3 * col + pt] * Eyf[f], pe = a.p0 * e, ee = pe / a.gm1d;
-        FaceEq<float> q;
-        q.rho = (float)(a.rho0 * e); q.p = (float)pe; q.e = (float)ee; q.ep = (float)(ee + pe);
-        Vec4<float> F = face_flux<float, FLUX, 1, FORM_DEV>(P, q, dl, dr);
-        YT[0][0][f][pt][col] = F.r * a.ody; YT[0][1][f][pt][col] = F.mx * a.ody;
-        YT[0][2][f][pt][col] = F.my * a.ody; YT[0][3][f][pt][col] = F.e * a.ody;
-      }
-    }
-  }
-  __syncthreads();
-
-  // ---- phase 3: face integrals, stage combination, store -----------------------------
-#pragma unroll
-  for (int rr = 0; rr < 4; ++rr) {
-    const int jl = 4 * wave + rr, j = ty0 + jl;
-    const bool ok = j < a.ny && i < a.nx;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-#pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        const int k = 4 * s + g4;   // face point fed by this lane in step s
-        float b;
-        if (k < 3) b = XT[0][v][jl][k][c16];
-        else if (k < 6) b = XT[0][v][jl][k - 3][c16 + 1];
-        else if (k < 9) b = YT[0][v][jl][k - 6][c16];
-        else b = YT[0][v][jl + 1][k - 9][c16];
-        if (!ok) b = 0.f;
-        acc[rr][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(Abf[s], b, acc[rr][v], 0, 0, 0);
-      }
-    }
-    // U and U^n in the output layout: mode 4*g4 + r of cell c16 sits in lane
-    // 16*r + c16, as its first (g4 == 0) or second (g4 == 1) loaded mode
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int src = 16 * r + c16;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float u0 = __shfl(B0[rr][v], src), u1 = __shfl(B1[rr][v], src);
-        const float n0 = __shfl(N0[rr][v], src), n1 = __shfl(N1[rr][v], src);
-        const int m = 4 * g4 + r;
-        if (ok && g4 < 2 && m < NM) {
-          const float u = g4 == 0 ? u0 : u1, un = g4 == 0 ? n0 : n1;
-          const float L = acc[rr][v][r];
-          float res;
-          if (a.residual_only) {
-            res = L;
-          } else {
-            res = a.beta * (u + dtf * L);
-            if (a.alpha != 0.f) res = a.alpha * un + res;
-          }
-          a.out[(long long)(v * NM + m) * a.plane + (long long)j * a.pitch + i] = res;
-        }
-      }
-    }
-  }
+  finish(y1 - 1, accp, flux_y(y1, Tt, yr), N);
 }
 
 // TVB limiter (in place; means are never written, so neighbours may be read
@@ -397,9 +391,9 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
   const bool active = a.init || (a.ctrl->t[par] < a.t_end);
   if (!active) {
     // instance done: carry the clock and rate to the next parity once
-    if (a.bookkeeping && blockIdx.x == 0 && threadIdx.x == 0) {
-      a.ctrl->t[par ^ 1] = a.ctrl->t[par];
-      a.ctrl->rate[par ^ 1] = a.ctrl->rate[par];
+    if (a.bookkeeping && blockIdx.x == 0 && threadIdx.x < RSLOTS) {
+      if (threadIdx.x == 0) a.ctrl->t[par ^ 1] = a.ctrl->t[par];
+      a.ctrl->rate[par ^ 1][threadIdx.x] = a.ctrl->rate[par][threadIdx.x];
     }
     return;
   }
@@ -467,17 +461,21 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
     }
   }
   if (a.cfl) {
+    // workgroup maximum, then one atomic per workgroup into its slot
+    __shared__ float wmax[4];
     for (int off = 32; off > 0; off >>= 1) rate = fmaxf(rate, __shfl_xor(rate, off));
-    const int anybad = __any(bad);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = rate;
+    const int anybad = __syncthreads_or(bad);
     const int slot = a.init ? par : (par ^ 1);
-    if ((threadIdx.x & 63) == 0) {
-      atomicMax(&a.ctrl->rate[slot], __float_as_uint(rate));
+    if (threadIdx.x == 0) {
+      const float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+      atomicMax(&a.ctrl->rate[slot][blockIdx.x % RSLOTS], __float_as_uint(m));
       if (anybad) atomicOr(&a.ctrl->err, 1u);
     }
     if (a.bookkeeping && blockIdx.x == 0 && threadIdx.x == 0) {
       // the step that just ended: same dt as its stages computed
       const double t = a.ctrl->t[par];
-      double dt = a.cfl_over / (double)__uint_as_float(a.ctrl->rate[par]);
+      double dt = a.cfl_over / (double)ctrl_rate(a.ctrl, par);
       if (t + dt > a.t_end) dt = a.t_end - t;
       a.ctrl->t[par ^ 1] = t + dt;
       a.ctrl->dt_last = dt;
@@ -535,9 +533,13 @@ struct fvwb_dg {
   hipStream_t stream = nullptr;
   int pitch = 0;
   long long plane = 0;
+  long long org = 0;              // offset of cell (0,0) inside the ghost frame
+  int ntx = 0, nseg = 0, seg_rows = 0;
   float* buf[3] = {nullptr, nullptr, nullptr};
   double* host_stage = nullptr;   // device FP64 AoS staging
   double *exn = nullptr, *eyn = nullptr, *exf = nullptr, *eyf = nullptr;
+  float *exn32 = nullptr, *eyn32 = nullptr, *exf32 = nullptr, *eyf32 = nullptr;
+  float* cell(int b) const { return buf[b] + org; }
   Consts* C = nullptr;
   Ctrl* ctrl = nullptr;
   int parity = 0;
@@ -561,7 +563,8 @@ extern "C" const char* fvwb_dg_last_error(fvwb_dg_handle h) { return h ? h->err.
 
 extern "C" int fvwb_dg_destroy(fvwb_dg_handle h) {
   if (!h) return FVWB_OK;
-  void* ps[] = {h->buf[0], h->buf[1], h->buf[2], h->host_stage, h->exn, h->eyn, h->exf, h->eyf, h->C, h->ctrl};
+  void* ps[] = {h->buf[0], h->buf[1], h->buf[2], h->host_stage, h->exn, h->eyn, h->exf, h->eyf,
+                h->exn32, h->eyn32, h->exf32, h->eyf32, h->C, h->ctrl};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -587,8 +590,17 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   if (!h) return dg_fail(nullptr, FVWB_ERR_OOM, "host allocation failed");
   h->p = *p;
   const int nx = p->nx, ny = p->ny;
-  h->pitch = (nx + TILE - 1) / TILE * TILE;
-  h->plane = (long long)h->pitch * ((ny + TILE - 1) / TILE * TILE);
+  // ghost frame: one column left of column 0 and enough on the right for the
+  // last strip's lanes, one row below row 0 and one above row ny-1; the ghost
+  // cells are zero for the life of the handle and never written
+  h->ntx = (nx + SW - 1) / SW;
+  h->pitch = (SW * h->ntx + 2 + 15) / 16 * 16;
+  h->plane = ((long long)h->pitch * (ny + GROWS) + 15) / 16 * 16;
+  h->org = h->pitch + 1;
+  if ((5 * h->plane + h->pitch) * 4 >= (1LL << 31)) {
+    delete h;
+    return dg_fail(nullptr, FVWB_ERR_ARG, "grid too large for the 2 GiB mode-plane group of one variable");
+  }
   auto bail = [&](int rc, const char* m) { h->err = m; g_dg_err = m; fvwb_dg_destroy(h); return rc; };
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(FVWB_ERR_HIP, "hipStreamCreate failed");
   const size_t sbytes = sizeof(float) * 4 * NM * (size_t)h->plane;
@@ -598,6 +610,8 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
       hipMalloc(&h->exn, sizeof(double) * 3 * nx) != hipSuccess || hipMalloc(&h->eyn, sizeof(double) * 3 * ny) != hipSuccess ||
       hipMalloc(&h->exf, sizeof(double) * (nx + 1)) != hipSuccess ||
       hipMalloc(&h->eyf, sizeof(double) * (ny + 1)) != hipSuccess || hipMalloc(&h->C, sizeof(Consts)) != hipSuccess ||
+      hipMalloc(&h->exn32, sizeof(float) * 3 * nx) != hipSuccess || hipMalloc(&h->eyn32, sizeof(float) * 3 * ny) != hipSuccess ||
+      hipMalloc(&h->exf32, sizeof(float) * (nx + 1)) != hipSuccess || hipMalloc(&h->eyf32, sizeof(float) * (ny + 1)) != hipSuccess ||
       hipMalloc(&h->ctrl, sizeof(Ctrl)) != hipSuccess)
     return bail(FVWB_ERR_OOM, "device allocation failed");
   // padding columns and rows stay zero for the whole life of the handle
@@ -620,34 +634,62 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   for (int j = 0; j <= ny; ++j) fy[j] = std::exp(-(kk * (j * dy)));
   Consts C;
   std::memset(&C, 0, sizeof C);
-  for (int m = 0; m < NM; ++m) {
-    for (int row = 0; row < 16; ++row) {
-      const int node = node_of_row(row);
-      if (node >= 9) continue;
-      const int a = node % 3, b = node / 3;
-      const double w = h->wq[a] * h->wq[b];
-      const double pa = leg(h->xq[a], MI[m]), pb = leg(h->xq[b], MJ[m]);
-      C.fwd_node[row][m] = (float)(pa * pb);
-      C.back_vol[0][m][row] = (float)(0.5 * w * legp(h->xq[a], MI[m]) * pb);
-      C.back_vol[1][m][row] = (float)(0.5 * w * pa * legp(h->xq[b], MJ[m]));
-      C.back_vol[2][m][row] = (float)(0.25 * w * pa * pb);
+  const double* xq = h->xq;
+  const double* wq = h->wq;
+  for (int row = 0; row < 16; ++row) {
+    const int g = row >> 2, r = row & 3;
+    for (int m = 0; m < NM; ++m) {
+      if (g < 3 && r < 3) C.fwd_node[row][m] = (float)(leg(xq[r], MI[m]) * leg(xq[g], MJ[m]));   // node (x_r, y_g)
+      if (g < 3) {
+        const int q = g;
+        double v;
+        if (r == 0) v = leg(-1.0, MI[m]) * leg(xq[q], MJ[m]);        // left trace
+        else if (r == 1) v = leg(1.0, MI[m]) * leg(xq[q], MJ[m]);    // right
+        else if (r == 2) v = leg(xq[q], MI[m]) * leg(-1.0, MJ[m]);   // bottom
+        else v = leg(xq[q], MI[m]) * leg(1.0, MJ[m]);                // top
+        C.fwd_face[row][m] = (float)v;
+      }
     }
-    for (int q = 0; q < 3; ++q) {
-      C.fwd_face[q][m] = (float)(leg(-1.0, MI[m]) * leg(h->xq[q], MJ[m]));       // left
-      C.fwd_face[3 + q][m] = (float)(leg(1.0, MI[m]) * leg(h->xq[q], MJ[m]));    // right
-      C.fwd_face[6 + q][m] = (float)(leg(h->xq[q], MI[m]) * leg(-1.0, MJ[m]));   // bottom
-      C.fwd_face[9 + q][m] = (float)(leg(h->xq[q], MI[m]) * leg(1.0, MJ[m]));    // top
-      C.back_face[m][q] = (float)(0.5 * h->wq[q] * leg(-1.0, MI[m]) * leg(h->xq[q], MJ[m]));
-      C.back_face[m][3 + q] = (float)(-0.5 * h->wq[q] * leg(1.0, MI[m]) * leg(h->xq[q], MJ[m]));
-      C.back_face[m][6 + q] = (float)(0.5 * h->wq[q] * leg(h->xq[q], MI[m]) * leg(-1.0, MJ[m]));
-      C.back_face[m][9 + q] = (float)(-0.5 * h->wq[q] * leg(h->xq[q], MI[m]) * leg(1.0, MJ[m]));
+    // output row -> mode: row 4g is mode g, row 4g+1 mode g+4 (g < 2), others none
+    const int m = r == 0 ? g : (r == 1 && g < 2 ? g + 4 : -1);
+    if (m < 0) continue;
+    for (int k = 0; k < 3; ++k) {
+      for (int st = 0; st < 3; ++st) {     // backward volume step st feeds node (x_st, y_k) as k-slot k
+        const double w = wq[st] * wq[k];
+        C.back_vol[0][st][row][k] = (float)(0.5 * w * legp(xq[st], MI[m]) * leg(xq[k], MJ[m]));
+        C.back_vol[1][st][row][k] = (float)(0.5 * w * leg(xq[st], MI[m]) * legp(xq[k], MJ[m]));
+      }
+      // face fluxes (pre-scaled by 1/dx or 1/dy) at point k: +-0.5 w phi at the face
+      C.back_face[0][row][k] = (float)(0.5 * wq[k] * leg(-1.0, MI[m]) * leg(xq[k], MJ[m]));
+      C.back_face[1][row][k] = (float)(-0.5 * wq[k] * leg(1.0, MI[m]) * leg(xq[k], MJ[m]));
+      C.back_face[2][row][k] = (float)(0.5 * wq[k] * leg(xq[k], MI[m]) * leg(-1.0, MJ[m]));
+      C.back_face[3][row][k] = (float)(-0.5 * wq[k] * leg(xq[k], MI[m]) * leg(1.0, MJ[m]));
     }
+  }
+  std::vector<float> ex32(ex.begin(), ex.end()), ey32(ey.begin(), ey.end()), fx32(fx.begin(), fx.end()),
+      fy32(fy.begin(), fy.end());
+  // segments along y: as many waves as the device holds at once, at least 4 rows each
+  {
+    int dev = 0, cus = 0, blocks_per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, dg_stage_kernel<fvwb::FLUX_HLL>, 256, 0) !=
+            hipSuccess)
+      return bail(FVWB_ERR_HIP, "device query failed");
+    const long long slots = 4LL * std::max(1, cus) * std::max(1, blocks_per_cu);
+    const long long nseg = std::max(1LL, std::min<long long>(ny, slots / h->ntx));
+    h->seg_rows = std::max(4, (int)((ny + nseg - 1) / nseg));
+    h->nseg = (ny + h->seg_rows - 1) / h->seg_rows;
   }
   if (hipMemcpy(h->exn, ex.data(), sizeof(double) * ex.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->eyn, ey.data(), sizeof(double) * ey.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->exf, fx.data(), sizeof(double) * fx.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->eyf, fy.data(), sizeof(double) * fy.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->C, &C, sizeof C, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->exn32, ex32.data(), sizeof(float) * ex32.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->eyn32, ey32.data(), sizeof(float) * ey32.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->exf32, fx32.data(), sizeof(float) * fx32.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->eyf32, fy32.data(), sizeof(float) * fy32.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemsetAsync(h->ctrl, 0, sizeof(Ctrl), h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess)
     return bail(FVWB_ERR_HIP, "copying tables failed");
   *out = h;
@@ -657,12 +699,13 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
 static Args stage_args(fvwb_dg* h) {
   const fvwb_dg_params& p = h->p;
   Args a{};
-  a.C = h->C; a.exn = h->exn; a.eyn = h->eyn; a.exf = h->exf; a.eyf = h->eyf; a.ctrl = h->ctrl;
+  a.C = h->C; a.exn = h->exn32; a.eyn = h->eyn32; a.exf = h->exf32; a.eyf = h->eyf32; a.ctrl = h->ctrl;
   a.plane = h->plane; a.pitch = h->pitch; a.nx = p.nx; a.ny = p.ny;
+  a.ntx = h->ntx; a.nseg = h->nseg; a.seg_rows = h->seg_rows;
   a.cfl_over = p.cfl / 5.0;
   a.gamma = (float)p.gamma; a.g = (float)p.g;
   a.odx = (float)(p.nx / p.boxlen_x); a.ody = (float)(p.ny / p.boxlen_y);
-  a.rho0 = p.rho0; a.p0 = p.p0; a.gm1d = p.gamma - 1.0;
+  a.rho0 = (float)p.rho0; a.p0 = (float)p.p0; a.ep0 = (float)(p.p0 * p.gamma / (p.gamma - 1.0));
   a.t_end = INFINITY;
   return a;
 }
@@ -686,7 +729,7 @@ static LimArgs lim_args(fvwb_dg* h, float* st) {
 }
 
 static void launch_stage(fvwb_dg* h, const Args& a) {
-  const dim3 grid((h->p.nx + TILE - 1) / TILE, (h->p.ny + TILE - 1) / TILE), block(256);
+  const dim3 grid((unsigned)((h->ntx * h->nseg + 3) / 4)), block(256);
   if (h->p.flux == FVWB_FLUX_HLL)
     hipLaunchKernelGGL(dg_stage_kernel<fvwb::FLUX_HLL>, grid, block, 0, h->stream, a);
   else
@@ -710,12 +753,12 @@ static int put_modes(fvwb_dg* h, const double* modes, float* dst) {
 
 extern "C" int fvwb_dg_upload(fvwb_dg_handle h, const double* modes) {
   if (!h || !modes) return dg_fail(h, FVWB_ERR_ARG, "NULL argument");
-  int rc = put_modes(h, modes, h->buf[0]);
+  int rc = put_modes(h, modes, h->cell(0));
   if (rc) return rc;
   Ctrl z;
   std::memset(&z, 0, sizeof z);
   DGHIP(h, hipMemcpyAsync(h->ctrl, &z, sizeof z, hipMemcpyHostToDevice, h->stream));
-  LimArgs la = lim_args(h, h->buf[0]);
+  LimArgs la = lim_args(h, h->cell(0));
   la.parity = 0; la.cfl = 1; la.init = 1; la.bookkeeping = 0;
   launch_limit(h, la);
   DGHIP(h, hipGetLastError());
@@ -757,7 +800,7 @@ extern "C" int fvwb_dg_init_fourier(fvwb_dg_handle h, int32_t nterms, const doub
 
 extern "C" int fvwb_dg_compute_residual(fvwb_dg_handle h, const double* modes, double* dudt) {
   if (!h || !modes || !dudt) return dg_fail(h, FVWB_ERR_ARG, "NULL argument");
-  int rc = put_modes(h, modes, h->buf[1]);
+  int rc = put_modes(h, modes, h->cell(1));
   if (rc) return rc;
   Args a = stage_args(h);
   a.in = h->buf[1]; a.un = nullptr; a.out = h->buf[2]; a.residual_only = 1; a.parity = h->parity;
@@ -765,7 +808,7 @@ extern "C" int fvwb_dg_compute_residual(fvwb_dg_handle h, const double* modes, d
   DGHIP(h, hipGetLastError());
   const fvwb_dg_params& p = h->p;
   const long long n = 4LL * NM * p.nx * p.ny;
-  hipLaunchKernelGGL(dg_from_planes_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->buf[2],
+  hipLaunchKernelGGL(dg_from_planes_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->cell(2),
                      h->host_stage, p.nx, p.ny, h->pitch, h->plane);
   DGHIP(h, hipGetLastError());
   DGHIP(h, hipMemcpyAsync(dudt, h->host_stage, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
@@ -785,17 +828,17 @@ static int enqueue_step(fvwb_dg* h, double t_end) {
   // stage 1
   a.in = A; a.un = nullptr; a.out = B; a.alpha = 0.f; a.beta = 1.f; a.first = 1;
   launch_stage(h, a);
-  la.st = B; la.cfl = 0; la.bookkeeping = 0;
+  la.st = h->cell(1); la.cfl = 0; la.bookkeeping = 0;
   if (la.limit) launch_limit(h, la);
   // stage 2
   a.in = B; a.un = A; a.out = Cc; a.alpha = 0.75f; a.beta = 0.25f; a.first = 0;
   launch_stage(h, a);
-  la.st = Cc;
+  la.st = h->cell(2);
   if (la.limit) launch_limit(h, la);
-  // stage 3 (in place over U^n: each lane reads its own U^n entry before writing it)
+  // stage 3 (in place over U^n: each owned cell's U^n is read by the lane that writes it, before the write)
   a.in = Cc; a.un = A; a.out = A; a.alpha = (float)(1.0 / 3.0); a.beta = (float)(2.0 / 3.0);
   launch_stage(h, a);
-  la.st = A; la.cfl = 1; la.bookkeeping = 1;
+  la.st = h->cell(0); la.cfl = 1; la.bookkeeping = 1;
   launch_limit(h, la);
   DGHIP(h, hipGetLastError());
   h->parity ^= 1;
@@ -828,7 +871,7 @@ extern "C" int fvwb_dg_download(fvwb_dg_handle h, double* modes) {
   if (!h->ready) return dg_fail(h, FVWB_ERR_STATE, "no state uploaded");
   const fvwb_dg_params& p = h->p;
   const long long n = 4LL * NM * p.nx * p.ny;
-  hipLaunchKernelGGL(dg_from_planes_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->buf[0],
+  hipLaunchKernelGGL(dg_from_planes_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->cell(0),
                      h->host_stage, p.nx, p.ny, h->pitch, h->plane);
   DGHIP(h, hipGetLastError());
   DGHIP(h, hipMemcpyAsync(modes, h->host_stage, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
