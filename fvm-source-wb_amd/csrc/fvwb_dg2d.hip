@@ -14,7 +14,8 @@
 //   forward:  [16 points x 8 modes] x [8 modes x 16 cells], once for the 9
 //             volume nodes and once for the 12 face points;
 //   backward: [16 outputs x 4 inputs] x [4 inputs x 16 cells] per step, 6 steps
-//             for the flux terms at the nodes and 4 for the face fluxes.
+//             for the flux terms at the nodes and 4 for the face fluxes; the
+//             backward operands carry dt/dx or dt/dy.
 // Lane l = 16 g + c handles cell c; in the forward products it receives node
 // (r, g) (r = 0..2) and the left, right, bottom and top traces at Gauss point g
 // (g = 0..2); in the backward products it feeds exactly those values as k-slot
@@ -223,9 +224,12 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   const float Af0 = C.fwd_face[c][g], Af1 = C.fwd_face[c][g + 4];
   float Abx[3], Aby[3], Abf[4];
 #pragma unroll
-  for (int r = 0; r < 3; ++r) { Abx[r] = sdt * C.back_vol[0][r][c][g]; Aby[r] = sdt * C.back_vol[1][r][c][g]; }
+  for (int r = 0; r < 3; ++r) {
+    Abx[r] = (sdt * a.odx) * C.back_vol[0][r][c][g];
+    Aby[r] = (sdt * a.ody) * C.back_vol[1][r][c][g];
+  }
 #pragma unroll
-  for (int s = 0; s < 4; ++s) Abf[s] = sdt * C.back_face[s][c][g];
+  for (int s = 0; s < 4; ++s) Abf[s] = (sdt * (s < 2 ? a.odx : a.ody)) * C.back_face[s][c][g];
 
   // equilibrium factors of this column: x nodes, the left x face, x node gq
   const int ic = min(max(i, 0), nx - 1), ifc = min(max(i, 0), nx);
@@ -245,11 +249,9 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   };
   auto mirror_x = [](Vec4<float> d) { return Vec4<float>{d.r, -d.mx, d.my, d.e}; };
   auto mirror_y = [](Vec4<float> d) { return Vec4<float>{d.r, d.mx, -d.my, d.e}; };
-  // y-face flux at Gauss point gq below row j (lower state dl, upper dr), times 1/dy
+  // y-face flux at Gauss point gq below row j (lower state dl, upper dr)
   auto flux_y = [&](int j, Vec4<float> dl, Vec4<float> dr) {
-    const FaceEq<float> q = eq_at(a, exg * a.eyf[j]);
-    const Vec4<float> F = face_flux<float, FLUX, 1, FORM_DEV>(P, q, dl, dr);
-    return Vec4<float>{F.r * a.ody, F.mx * a.ody, F.my * a.ody, F.e * a.ody};
+    return face_flux<float, FLUX, 1, FORM_DEV>(P, eq_at(a, exg * a.eyf[j]), dl, dr);
   };
   // finish row j: add the flux through its top face, combine and store
   auto finish = [&](int j, v4f (&acc)[4], const Vec4<float>& Ft, const float (&N)[4][2]) {
@@ -265,9 +267,9 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   };
 
   // ---- prologue: top traces of the row below the segment, first row's modes ------
-  float Bc[4][2], Bn[4][2], N[4][2];
+  float B0[4][2], B1[4][2], N[4][2];
   Vec4<float> Tt = {0.f, 0.f, 0.f, 0.f};   // top trace (point gq) of the row below the current one
-  load_row(a.in, y0, Bc);
+  load_row(a.in, y0, B0);
   if (y0 > 0) {
     float Bh[4][2];
     load_row(a.in, y0 - 1, Bh);
@@ -277,14 +279,16 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v) N[v][0] = N[v][1] = 0.f;
-  v4f accp[4];
+  v4f accA[4], accB[4];
 
-  for (int j = y0; j < y1; ++j) {
+  // one row: Bc holds its modes, Bn receives the next row's, accp holds the
+  // accumulator of the row below (finished here), acc receives this row's
+  auto row = [&](int j, const float (&Bc)[4][2], float (&Bn)[4][2], v4f (&accp)[4], v4f (&acc)[4]) {
     // loads in flight during this row: the next row, and U^n of the row finished here
     load_row(a.in, j + 1, Bn);
     if (with_un && j > y0) load_own(a.un, j - 1, N);
 
-    v4f dn[4], df[4], acc[4];
+    v4f dn[4], df[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const v4f z = {0.f, 0.f, 0.f, 0.f};
@@ -311,8 +315,8 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
       const FaceEq<float> q = eq_at(a, exr[r] * eyg);
       Vec4<float> fx, fy;
       point_terms(P, q, Vec4<float>{dn[0][r], dn[1][r], dn[2][r], dn[3][r]}, fx, fy);
-      const float X[4] = {fx.r * a.odx, fx.mx * a.odx, fx.my * a.odx, fx.e * a.odx};
-      const float Y[4] = {fy.r * a.ody, fy.mx * a.ody, fy.my * a.ody, fy.e * a.ody};
+      const float X[4] = {fx.r, fx.mx, fx.my, fx.e};
+      const float Y[4] = {fy.r, fy.mx, fy.my, fy.e};
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[v] = mfma(Abx[r], X[v], acc[v]);
 #pragma unroll
@@ -326,7 +330,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
     if (i == 0) xl = mirror_x(xr);
     if (i == nx) xr = mirror_x(xl);
     const Vec4<float> Fx = face_flux<float, FLUX, 0, FORM_DEV>(P, eq_at(a, exf_i * eyg), xl, xr);
-    const float fl[4] = {Fx.r * a.odx, Fx.mx * a.odx, Fx.my * a.odx, Fx.e * a.odx};
+    const float fl[4] = {Fx.r, Fx.mx, Fx.my, Fx.e};
     // y face below row j at point gq: top trace of row j-1 | bottom trace of row j
     const Vec4<float> yr = {df[0][2], df[1][2], df[2][2], df[3][2]};
     const Vec4<float> Fb = flux_y(j, j == 0 ? mirror_y(yr) : Tt, yr);
@@ -340,25 +344,34 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
     // the face below row j is the top face of row j-1
     if (j > y0) finish(j - 1, accp, Fb, N);
     Tt = {df[0][3], df[1][3], df[2][3], df[3][3]};
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      accp[v] = acc[v];
-      Bc[v][0] = Bn[v][0];
-      Bc[v][1] = Bn[v][1];
-    }
-  }
+  };
 
-  // ---- epilogue: the top face of the segment's last row --------------------------
-  if (with_un) load_own(a.un, y1 - 1, N);
-  Vec4<float> yr;
-  if (y1 < ny) {
-    v4f df[4];
-    fwd_face(Bc, df);      // Bc holds row y1
-    yr = {df[0][2], df[1][2], df[2][2], df[3][2]};
-  } else {
-    yr = mirror_y(Tt);
+  // ---- epilogue: the top face of the segment's last row (Bc holds row y1) ----------
+  auto last = [&](const float (&Bc)[4][2], v4f (&accp)[4]) {
+    if (with_un) load_own(a.un, y1 - 1, N);
+    Vec4<float> yr;
+    if (y1 < ny) {
+      v4f df[4];
+      fwd_face(Bc, df);
+      yr = {df[0][2], df[1][2], df[2][2], df[3][2]};
+    } else {
+      yr = mirror_y(Tt);
+    }
+    finish(y1 - 1, accp, flux_y(y1, Tt, yr), N);
+  };
+
+  // rows in pairs with the register sets swapped, so nothing is copied between rows
+  int j = y0;
+  for (; j + 1 < y1; j += 2) {
+    row(j, B0, B1, accA, accB);
+    row(j + 1, B1, B0, accB, accA);
   }
-  finish(y1 - 1, accp, flux_y(y1, Tt, yr), N);
+  if (j < y1) {
+    row(j, B0, B1, accA, accB);
+    last(B1, accB);
+  } else {
+    last(B0, accA);
+  }
 }
 
 // TVB limiter (in place; means are never written, so neighbours may be read
