@@ -60,12 +60,17 @@ struct Consts {
 };
 
 // The CFL rate max((|u|+c)/dx + (|v|+c)/dy) of parity p is the maximum of
-// rate[p][0..RSLOTS-1]: workgroup b reduces into slot b % RSLOTS, so the atomics
-// of the grid spread over RSLOTS addresses instead of queueing on one.
-constexpr int RSLOTS = 8;
+// rate[p][0..RSLOTS-1]: workgroup b reduces into slot b % RSLOTS, each slot on a
+// 128-byte line of its own, so the grid's atomics spread over RSLOTS lines
+// instead of queueing on one.
+constexpr int RSLOTS = 16;
+struct alignas(128) RateSlot {
+  uint32_t v;                 // FP32 bit pattern; positive floats order like uints
+  uint32_t pad[31];
+};
 
 struct Ctrl {
-  uint32_t rate[2][RSLOTS];   // FP32 bit patterns; positive floats order like uints
+  RateSlot rate[2][RSLOTS];
   double t[2];
   double dt_last;
   int32_t steps;
@@ -95,7 +100,7 @@ struct Args {
 __device__ inline float ctrl_rate(const Ctrl* c, int par) {
   uint32_t m = 0u;
 #pragma unroll
-  for (int k = 0; k < RSLOTS; ++k) m = max(m, c->rate[par][k]);
+  for (int k = 0; k < RSLOTS; ++k) m = max(m, c->rate[par][k].v);
   return __uint_as_float(m);
 }
 
@@ -165,7 +170,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
     if (!(t < a.t_end)) return;       // this instance is done
     dt = a.cfl_over / (double)ctrl_rate(ck, par);
     if (t + dt > a.t_end) dt = a.t_end - t;
-    if (a.first && blockIdx.x == 0 && threadIdx.x < RSLOTS) a.ctrl->rate[par ^ 1][threadIdx.x] = 0u;
+    if (a.first && blockIdx.x == 0 && threadIdx.x < RSLOTS) a.ctrl->rate[par ^ 1][threadIdx.x].v = 0u;
   }
   const int w = xcd_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (w >= a.ntx * a.nseg) return;
@@ -376,19 +381,22 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
 
 // TVB limiter (in place; means are never written, so neighbours may be read
 // concurrently) and, when requested, the CFL rate of the limited state plus the
-// step bookkeeping of the instance.
+// step bookkeeping of the instance.  One thread per cell, a row of cells per
+// grid row; every mode of variable v is read through one buffer resource, so
+// out-of-range neighbours and masked lanes cost no address arithmetic.
 struct LimArgs {
-  float* st;
-  const double* exn;
-  const double* eyn;
+  float* st;           // cell (0,0) of the 24 mode planes
+  const float* exn;    // e^{-k x} at x nodes [3*nx] (FP32)
+  const float* eyn;    // e^{-k y} at y nodes [3*ny] (FP32)
   Ctrl* ctrl;
   long long plane;
   int pitch, nx, ny, parity;
+  int nrec;            // bytes from a variable's mode-0 cell (0,0) to the end of its six planes
   int limit, cfl, bookkeeping, init;
   double t_end, cfl_over;
   float gamma, odx, ody, mdx2, mdy2;
-  double rho0, p0;
-  float phi[3][3];   // phi_n(xi_a)
+  float rho0, eover;   // equilibrium rho and E per e^{-k(x+y)}
+  float phi1[3], phi2[3];   // phi_1, phi_2 at the Gauss nodes
 };
 
 __device__ inline float gminmod(float x, float y, float z, float mh2) {
@@ -401,91 +409,107 @@ __device__ inline float gminmod(float x, float y, float z, float mh2) {
 
 __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
   const int par = a.parity;
+  const int blin = blockIdx.y * gridDim.x + blockIdx.x;
   const bool active = a.init || (a.ctrl->t[par] < a.t_end);
   if (!active) {
     // instance done: carry the clock and rate to the next parity once
-    if (a.bookkeeping && blockIdx.x == 0 && threadIdx.x < RSLOTS) {
+    if (a.bookkeeping && blin == 0 && threadIdx.x < RSLOTS) {
       if (threadIdx.x == 0) a.ctrl->t[par ^ 1] = a.ctrl->t[par];
-      a.ctrl->rate[par ^ 1][threadIdx.x] = a.ctrl->rate[par][threadIdx.x];
+      a.ctrl->rate[par ^ 1][threadIdx.x].v = a.ctrl->rate[par][threadIdx.x].v;
     }
     return;
   }
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nc = a.nx * a.ny;
+  const int nx = a.nx, ny = a.ny;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  const bool in = i < nx;
+  const unsigned pb = (unsigned)a.plane * 4u;
+  const unsigned ob = (unsigned)(j * a.pitch + i) * 4u;
+  auto off = [&](bool ok, unsigned o, int m) { return ok ? o + (unsigned)m * pb : MASKED; };
+  __amdgpu_buffer_rsrc_t rv[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) rv[v] = mode_rsrc(a.st + (long long)(v * NM) * a.plane, a.nrec);
+  // the limiter needs modes 0-2 only; the CFL rate needs all six
+  float md[4][NM];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < NM; ++m) md[v][m] = (m < 3 || a.cfl) ? ldm(rv[v], off(in, ob, m)) : 0.f;
+
+  if (a.limit) {
+    const float s3 = 1.7320508075688772f;
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    const unsigned ow = off(in && hw, ob - 4u, 0), oe = off(in && he, ob + 4u, 0);
+    const unsigned os = off(in && hs, ob - (unsigned)a.pitch * 4u, 0), on = off(in && hn, ob + (unsigned)a.pitch * 4u, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float u0 = md[v][0];
+      // walls mirror the mean, with the normal momentum negated
+      const float mx = v == 1 ? -u0 : u0, my = v == 2 ? -u0 : u0;
+      const float lw = ldm(rv[v], ow), le = ldm(rv[v], oe), ls = ldm(rv[v], os), ln = ldm(rv[v], on);
+      const float uw = hw ? lw : mx, ue = he ? le : mx, us = hs ? ls : my, un = hn ? ln : my;
+      const float x = s3 * md[v][1], y = s3 * md[v][2];
+      const float xt = gminmod(x, ue - u0, u0 - uw, a.mdx2);
+      const float yt = gminmod(y, un - u0, u0 - us, a.mdy2);
+      const bool lim = in && (xt != x || yt != y);
+      if (lim) {
+        md[v][1] = xt * (1.0f / 1.7320508075688772f);
+        md[v][2] = yt * (1.0f / 1.7320508075688772f);
+        md[v][3] = md[v][4] = md[v][5] = 0.f;
+      }
+#pragma unroll
+      for (int m = 1; m < NM; ++m) stm(rv[v], off(lim, ob, m), md[v][m]);
+    }
+  }
+
   float rate = 0.f;
   bool bad = false;
-  if (c < nc) {
-    const int i = c % a.nx, j = c / a.nx;
-    const long long o = (long long)j * a.pitch + i;
-    float md[4][NM];
+  if (a.cfl && in) {
+    const float gm1 = a.gamma - 1.0f;
+    const int ic = i;
+    float ex[NQ], ey[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) { ex[q] = a.exn[3 * ic + q]; ey[q] = a.eyn[3 * j + q]; }
+    // node values by sum factorisation: d(x_a, y_b) = A0(a) + phi_1(y_b) A1(a) + phi_2(y_b) m5,
+    // A0 = m0 + phi_1 m1 + phi_2 m3, A1 = m2 + phi_1 m4 (modes (0,0),(1,0),(0,1),(2,0),(1,1),(0,2))
+    float A0[4][NQ], A1[4][NQ];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-      for (int m = 0; m < NM; ++m) md[v][m] = a.st[(long long)(v * NM + m) * a.plane + o];
-    if (a.limit) {
-      const float s3 = 1.7320508075688772f;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float* base = a.st + (long long)(v * NM) * a.plane;
-        const float u0 = md[v][0];
-        const float uw = i > 0 ? base[o - 1] : (v == 1 ? -u0 : u0);
-        const float ue = i < a.nx - 1 ? base[o + 1] : (v == 1 ? -u0 : u0);
-        const float us = j > 0 ? base[o - a.pitch] : (v == 2 ? -u0 : u0);
-        const float un = j < a.ny - 1 ? base[o + a.pitch] : (v == 2 ? -u0 : u0);
-        const float x = s3 * md[v][1], y = s3 * md[v][2];
-        const float xt = gminmod(x, ue - u0, u0 - uw, a.mdx2);
-        const float yt = gminmod(y, un - u0, u0 - us, a.mdy2);
-        if (xt != x || yt != y) {
-          md[v][1] = xt * (1.0f / 1.7320508075688772f);
-          md[v][2] = yt * (1.0f / 1.7320508075688772f);
-          md[v][3] = md[v][4] = md[v][5] = 0.f;
-          for (int m = 1; m < NM; ++m) a.st[(long long)(v * NM + m) * a.plane + o] = md[v][m];
-        }
+      for (int q = 0; q < NQ; ++q) {
+        A0[v][q] = md[v][0] + a.phi1[q] * md[v][1] + a.phi2[q] * md[v][3];
+        A1[v][q] = md[v][2] + a.phi1[q] * md[v][4];
       }
-    }
-    if (a.cfl) {
-      const float gm1 = a.gamma - 1.0f;
-      const double e_over = a.p0 / (double)gm1;
-      double ex[NQ], ey[NQ];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) { ex[q] = a.exn[3 * i + q]; ey[q] = a.eyn[3 * j + q]; }
+    for (int qb = 0; qb < NQ; ++qb)
 #pragma unroll
-      for (int qb = 0; qb < NQ; ++qb)
+      for (int qa = 0; qa < NQ; ++qa) {
+        const float e = ex[qa] * ey[qb];
+        float d[4];
 #pragma unroll
-        for (int qa = 0; qa < NQ; ++qa) {
-          const double e = ex[qa] * ey[qb];
-          const float re = (float)(a.rho0 * e), ee = (float)(e_over * e);
-          float d[4];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const float* mm = md[v];
-            d[v] = mm[0] + a.phi[1][qa] * mm[1] + a.phi[1][qb] * mm[2] + a.phi[2][qa] * mm[3] +
-                   a.phi[1][qa] * a.phi[1][qb] * mm[4] + a.phi[2][qb] * mm[5];
-          }
-          const float rho = re + d[0];
-          const float inv = trcp(fmaxf(rho, 1e-10f));
-          const float u = d[1] * inv, w = d[2] * inv;
-          const float p = gm1 * ((ee + d[3]) - 0.5f * rho * (u * u + w * w));
-          const float cs = tsqrt(a.gamma * fmaxf(p, 1e-10f) * inv);
-          const float rt = (fabsf(u) + cs) * a.odx + (fabsf(w) + cs) * a.ody;
-          bad |= !(rho > 0.f) || !(p > 0.f) || !(rt < 1e30f);
-          rate = fmaxf(rate, rt);
-        }
-    }
+        for (int v = 0; v < 4; ++v) d[v] = A0[v][qa] + a.phi1[qb] * A1[v][qa] + a.phi2[qb] * md[v][5];
+        const float rho = a.rho0 * e + d[0];
+        const float inv = trcp(fmaxf(rho, 1e-10f));
+        const float u = d[1] * inv, w = d[2] * inv;
+        const float p = gm1 * ((a.eover * e + d[3]) - 0.5f * rho * (u * u + w * w));
+        const float cs = tsqrt(a.gamma * fmaxf(p, 1e-10f) * inv);
+        const float rt = (fabsf(u) + cs) * a.odx + (fabsf(w) + cs) * a.ody;
+        bad |= !(rho > 0.f) || !(p > 0.f) || !(rt < 1e30f);
+        rate = fmaxf(rate, rt);
+      }
   }
   if (a.cfl) {
     // workgroup maximum, then one atomic per workgroup into its slot
     __shared__ float wmax[4];
-    for (int off = 32; off > 0; off >>= 1) rate = fmaxf(rate, __shfl_xor(rate, off));
+    for (int o = 32; o > 0; o >>= 1) rate = fmaxf(rate, __shfl_xor(rate, o));
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = rate;
     const int anybad = __syncthreads_or(bad);
     const int slot = a.init ? par : (par ^ 1);
     if (threadIdx.x == 0) {
       const float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-      atomicMax(&a.ctrl->rate[slot][blockIdx.x % RSLOTS], __float_as_uint(m));
+      atomicMax(&a.ctrl->rate[slot][blin % RSLOTS].v, __float_as_uint(m));
       if (anybad) atomicOr(&a.ctrl->err, 1u);
     }
-    if (a.bookkeeping && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (a.bookkeeping && blin == 0 && threadIdx.x == 0) {
       // the step that just ended: same dt as its stages computed
       const double t = a.ctrl->t[par];
       double dt = a.cfl_over / (double)ctrl_rate(a.ctrl, par);
@@ -610,7 +634,7 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   h->pitch = (SW * h->ntx + 2 + 15) / 16 * 16;
   h->plane = ((long long)h->pitch * (ny + GROWS) + 15) / 16 * 16;
   h->org = h->pitch + 1;
-  if ((5 * h->plane + h->pitch) * 4 >= (1LL << 31)) {
+  if (NM * h->plane * 4 >= (1LL << 31)) {
     delete h;
     return dg_fail(nullptr, FVWB_ERR_ARG, "grid too large for the 2 GiB mode-plane group of one variable");
   }
@@ -726,18 +750,21 @@ static Args stage_args(fvwb_dg* h) {
 static LimArgs lim_args(fvwb_dg* h, float* st) {
   const fvwb_dg_params& p = h->p;
   LimArgs a{};
-  a.st = st; a.exn = h->exn; a.eyn = h->eyn; a.ctrl = h->ctrl;
+  a.st = st; a.exn = h->exn32; a.eyn = h->eyn32; a.ctrl = h->ctrl;
   a.plane = h->plane; a.pitch = h->pitch; a.nx = p.nx; a.ny = p.ny;
+  a.nrec = (int)((NM * h->plane - h->org) * 4);
   a.cfl_over = p.cfl / 5.0;
   a.gamma = (float)p.gamma;
   a.odx = (float)(p.nx / p.boxlen_x); a.ody = (float)(p.ny / p.boxlen_y);
   const double dx = p.boxlen_x / p.nx, dy = p.boxlen_y / p.ny;
   a.mdx2 = (float)(p.tvb_m * (dx * dx)); a.mdy2 = (float)(p.tvb_m * (dy * dy));
-  a.rho0 = p.rho0; a.p0 = p.p0;
+  a.rho0 = (float)p.rho0; a.eover = (float)(p.p0 / (p.gamma - 1.0));
   a.limit = p.limiter == FVWB_DG_LIMITER_TVB;
   a.t_end = INFINITY;
-  for (int n = 0; n < 3; ++n)
-    for (int q = 0; q < 3; ++q) a.phi[n][q] = (float)leg(h->xq[q], n);
+  for (int q = 0; q < 3; ++q) {
+    a.phi1[q] = (float)leg(h->xq[q], 1);
+    a.phi2[q] = (float)leg(h->xq[q], 2);
+  }
   return a;
 }
 
@@ -750,8 +777,8 @@ static void launch_stage(fvwb_dg* h, const Args& a) {
 }
 
 static void launch_limit(fvwb_dg* h, const LimArgs& a) {
-  const int nc = h->p.nx * h->p.ny;
-  hipLaunchKernelGGL(dg_limit_kernel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(dg_limit_kernel, dim3((unsigned)((h->p.nx + 255) / 256), (unsigned)h->p.ny), dim3(256), 0,
+                     h->stream, a);
 }
 
 static int put_modes(fvwb_dg* h, const double* modes, float* dst) {

@@ -91,12 +91,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
     const bool active = t < a.t_end;
     unsigned long long m = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) m = c->cmax[p][k] > m ? c->cmax[p][k] : m;
+    for (int k = 0; k < 8; ++k) m = c->cmax[p][k].v > m ? c->cmax[p][k].v : m;
     // dt = 0.5*dx/cmax*cfl (benchmark_2d.f90:242)
     dt = 0.5 * a.h / __longlong_as_double((long long)m) * a.cfl;
     if (tile == 0 && lane < 8) {
       Ctrl* cw = a.ctrl;
-      if (a.first) cw->cmax[q][lane] = 0ull;
+      if (a.first) cw->cmax[q][lane].v = 0ull;
       if (a.last && lane == 0) {
         cw->t[q] = active ? t + dt : t;
         if (active) { cw->steps_done += 1; cw->dt_last = dt; }
@@ -347,7 +347,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
     if (bad) atomicOr(&a.ctrl->err, 1u);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
-    if (lane == 0) atomicMax(&a.ctrl->cmax[q][slot], (unsigned long long)__double_as_longlong(smax));
+    if (lane == 0) atomicMax(&a.ctrl->cmax[q][slot].v, (unsigned long long)__double_as_longlong(smax));
   }
 }
 
@@ -480,7 +480,7 @@ __global__ void cfl_init_kernel(const T* __restrict__ in, const double* __restri
   if (bad) atomicOr(&ctrl->err, 1u);
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
   if ((threadIdx.x & 63) == 0)
-    atomicMax(&ctrl->cmax[0][blockIdx.x & 7], (unsigned long long)__double_as_longlong(m));
+    atomicMax(&ctrl->cmax[0][blockIdx.x & 7].v, (unsigned long long)__double_as_longlong(m));
 }
 
 // Device-side initial condition: w_eq = closed-form equilibrium at centres

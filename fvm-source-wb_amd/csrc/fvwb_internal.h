@@ -9,8 +9,13 @@ namespace fvwb {
 // Device-side step control of one instance.  Each step n uses slot p = n & 1;
 // its last stage accumulates max(|v|+c) of the new state into cmax[1-p] and
 // writes t[1-p], so no host synchronisation is needed between steps.
+// Atomics on one cache line serialise, so every slot has a 128-byte line of its own.
+struct alignas(128) SpeedSlot {
+  unsigned long long v;           // bit pattern of a non-negative double
+  unsigned long long pad[15];
+};
 struct Ctrl {
-  unsigned long long cmax[2][8];  // bit patterns of non-negative doubles, one per XCD
+  SpeedSlot cmax[2][8];           // one slot per XCD
   double t[2];
   double dt_last;
   int32_t steps_done;
