@@ -522,7 +522,10 @@ extern "C" int fvwb_batch_autotune(fvwb_batch_handle h, int32_t trial_steps, con
     BATCH_HIP(h, hipEventCreate(&e0));
     BATCH_HIP(h, hipEventCreate(&e1));
     const int row_opts[] = {4, 8, 12, 16, 24, 32};
-    const int cpl_opts[] = {2, 4};
+    // FP64 lanes always own 2 columns; FP32 lanes own 2 or 4 (one column per
+    // lane measured no faster on MI355X)
+    const int cpl_opts32[] = {2, 4}, cpl_opts64[] = {2};
+    const int* cpl_opts = h->shared.precision == FVWB_FP32 ? cpl_opts32 : cpl_opts64;
     const int ncpl = h->shared.precision == FVWB_FP32 ? 2 : 1;
     double best = 1e300;
     int rc = FVWB_OK;

@@ -14,6 +14,8 @@
 // from the window; each row is one coalesced 64-wide load per variable plane.
 // Face fluxes are computed once: the x face on a lane's left, the y face above
 // the current row (carried to the next row).
+#include <type_traits>
+
 #include "fvwb_device.h"
 #include "fvwb_internal.h"
 
@@ -46,7 +48,7 @@ __device__ __forceinline__ T ld1(const T* row, unsigned boff, int row_bytes) {
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(row, row_bytes), boff, 0, 0));
 }
 
-// A lane's block of C consecutive columns of one variable: one 8- or 16-byte load.
+// A lane's block of C consecutive columns of one variable: one 4-, 8- or 16-byte load.
 template <typename T, int C>
 struct Lanes { T v[C]; };
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
@@ -55,18 +57,22 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 // policies were measured and did not help the stage kernels.
 template <typename T, int C, int AUX = 0>
 __device__ __forceinline__ Lanes<T, C> ldv(const T* row, unsigned boff, int row_bytes) {
-  static_assert(C * sizeof(T) == 8 || C * sizeof(T) == 16, "block of 8 or 16 bytes");
+  static_assert(C * sizeof(T) == 4 || C * sizeof(T) == 8 || C * sizeof(T) == 16, "block of 4, 8 or 16 bytes");
   if constexpr (C * sizeof(T) == 16)
     return __builtin_bit_cast(Lanes<T, C>, __builtin_amdgcn_raw_buffer_load_b128(row_rsrc(row, row_bytes), boff, 0, AUX));
-  else
+  else if constexpr (C * sizeof(T) == 8)
     return __builtin_bit_cast(Lanes<T, C>, __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(row, row_bytes), boff, 0, AUX));
+  else
+    return __builtin_bit_cast(Lanes<T, C>, __builtin_amdgcn_raw_buffer_load_b32(row_rsrc(row, row_bytes), boff, 0, AUX));
 }
 template <typename T, int C, int AUX = 0>
 __device__ __forceinline__ void stv(T* row, unsigned boff, int row_bytes, Lanes<T, C> x) {
   if constexpr (C * sizeof(T) == 16)
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), row_rsrc(row, row_bytes), boff, 0, AUX);
-  else
+  else if constexpr (C * sizeof(T) == 8)
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), row_rsrc(row, row_bytes), boff, 0, AUX);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), row_rsrc(row, row_bytes), boff, 0, AUX);
 }
 
 // Deviation state of a lane's C columns on one row.
@@ -162,7 +168,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
   };
   // cell-centred rows (clamped into the grid; rows past the strip are masked)
   auto load_cell = [&](const T* base, int j) -> Blk<T, CPL> {
-    const T* r = base + (long long)min(j, ny - 1) * pitch;
+    const T* r = base + (long long)clampi(j, 0, ny - 1) * pitch;
     Lanes<T, CPL> v0 = ldv<T, CPL>(r, vboff, rowb), v1 = ldv<T, CPL>(r + plane, vboff, rowb),
                   v2 = ldv<T, CPL>(r + 2 * plane, vboff, rowb), v3 = ldv<T, CPL>(r + 3 * plane, vboff, rowb);
     Blk<T, CPL> b;
@@ -202,146 +208,170 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
   double smax = 0.0;
   bool bad = false;
 
-  // ---- prologue -----------------------------------------------------------------
-  // A ring of RING rows holds rows j..j+2 while row j is computed and receives
-  // row j+RING-1 (one row in flight ahead of use); U^n uses a ring of URING.
-  // The body is unrolled RING times with rotated roles, so the rings never move
-  // data.  (A deeper ring measured no faster: the strip sweep is limited by its
-  // memory access pattern, not by loads in flight per wave.)
-  constexpr int RING = 4;
-  constexpr int URING = 2;
-  Blk<T, CPL> R[RING], Ur[URING], sy, Gp;
-  {
-    const double eyf0 = a.eyf[clampi(y0, 0, ny)];
-    Blk<T, CPL> Dm1 = load_row(a.in, y0 - 1);
+  // ---- the sweep ------------------------------------------------------------------
+  // Even tile rows sweep upward, odd ones downward, so two vertically adjacent
+  // tiles reach their shared halo rows at the same time and one of them finds
+  // the rows in L2.  Both directions compute every face flux from the same two
+  // states in the same order (lower, upper), so the result does not depend on
+  // the direction.
+  // A ring of RING rows holds rows j, j+DIR, j+2DIR while row j is computed and
+  // receives row j+(RING-1)DIR (one row in flight ahead of use); U^n uses a ring
+  // of URING.  The body is unrolled RING times with rotated roles, so the rings
+  // never move data.  (A deeper ring measured no faster: the strip sweep is
+  // limited by its memory access pattern, not by loads in flight per wave.)
+  auto sweep = [&](auto dirc) {
+    constexpr int DIR = decltype(dirc)::value;
+    constexpr int RING = 4;
+    constexpr int URING = 2;
+    // slopes are differences along the sweep (minmod2(-a,-b) = -minmod2(a,b)), so a
+    // row's state toward the next row is D + sy/2 and toward the previous D - sy/2
+    const T hd = P.half;
+    const int js = DIR > 0 ? y0 : y1 - 1;      // first row of the sweep
+    Blk<T, CPL> R[RING], Ur[URING], sy, Gp;
+    // flux through the face between row jb (behind, state Db) and row ja (ahead, Da)
+    auto yface = [&](const FaceEq<T>& qy, const Vec4<T>& Db, const Vec4<T>& Da) {
+      if constexpr (DIR > 0) return face_flux<T, FLUX, 1, FORM>(P, qy, Db, Da);
+      else return face_flux<T, FLUX, 1, FORM>(P, qy, Da, Db);
+    };
+    {
+      const double eyf0 = a.eyf[clampi(js + (DIR < 0), 0, ny)];
+      Blk<T, CPL> Dm1 = load_row(a.in, js - DIR);
 #pragma unroll
-    for (int k = 0; k < RING - 1; ++k) R[k] = load_row(a.in, y0 + k);
-    Blk<T, CPL> Dm2;
-    if (RECON) Dm2 = load_row(a.in, y0 - 2);
-#pragma unroll
-    for (int k = 0; k < CPL; ++k) {
-      FaceEq<T> qy = face_eq(rxc[k], eyf0);
-      if (RECON) {
-        Vec4<T> sym1 = minmod2(Dm1.c[k] - Dm2.c[k], R[0].c[k] - Dm1.c[k]);
-        sy.c[k] = minmod2(R[0].c[k] - Dm1.c[k], R[1].c[k] - R[0].c[k]);
-        Gp.c[k] = face_flux<T, FLUX, 1, FORM>(P, qy, Dm1.c[k] + scale(P.half, sym1),
-                                              R[0].c[k] - scale(P.half, sy.c[k]));
-      } else {
-        Gp.c[k] = face_flux<T, FLUX, 1, FORM>(P, qy, Dm1.c[k], R[0].c[k]);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < URING; ++k) Ur[k] = Blk<T, CPL>{};
-    if (MODE == MODE_STAGE) {
-#pragma unroll
-      for (int k = 0; k < URING - 1; ++k) Ur[k] = load_cell(a.un, y0 + k);
-    }
-  }
-
-  // one row: A = row j, B = row j+1, C = row j+2; S receives row j+RING-1; UA
-  // holds U^n(j) and US receives U^n(j+URING-1).
-  auto row_step = [&](int j, const Blk<T, CPL>& A, const Blk<T, CPL>& B, const Blk<T, CPL>& C,
-                      Blk<T, CPL>& S, const Blk<T, CPL>& UA, Blk<T, CPL>& US) {
-    S = load_row(a.in, j + RING - 1);
-    if (MODE == MODE_STAGE) US = load_cell(a.un, j + URING - 1);
-    const int jt = min(j, ny - 1);
-    const double eyc = a.eyc[jt];
-    const double eyf = a.eyf[jt + 1];
-
-    // y faces above row j
-    Blk<T, CPL> Gn;
-#pragma unroll
-    for (int k = 0; k < CPL; ++k) {
-      FaceEq<T> qy = face_eq(rxc[k], eyf);
-      if (RECON) {
-        Vec4<T> syn = minmod2(B.c[k] - A.c[k], C.c[k] - B.c[k]);
-        Gn.c[k] = face_flux<T, FLUX, 1, FORM>(P, qy, A.c[k] + scale(P.half, sy.c[k]), B.c[k] - scale(P.half, syn));
-        sy.c[k] = syn;
-      } else {
-        Gn.c[k] = face_flux<T, FLUX, 1, FORM>(P, qy, A.c[k], B.c[k]);
-      }
-    }
-    // x faces on the left of each column; the first takes its left state from
-    // the neighbouring lane's last column, the last column's right face is the
-    // right neighbour's first
-    Vec4<T> xr[CPL], xl[CPL];
-    if (RECON) {
-      Vec4<T> lm = from_left(A.c[CPL - 1]), rp = from_right(A.c[0]);
+      for (int k = 0; k < RING - 1; ++k) R[k] = load_row(a.in, js + k * DIR);
+      Blk<T, CPL> Dm2;
+      if (RECON) Dm2 = load_row(a.in, js - 2 * DIR);
 #pragma unroll
       for (int k = 0; k < CPL; ++k) {
-        Vec4<T> am = k == 0 ? lm : A.c[k - 1];
-        Vec4<T> ap = k == CPL - 1 ? rp : A.c[k + 1];
-        Vec4<T> sx = minmod2(A.c[k] - am, ap - A.c[k]);
-        xr[k] = A.c[k] + scale(P.half, sx);
-        xl[k] = A.c[k] - scale(P.half, sx);
+        FaceEq<T> qy = face_eq(rxc[k], eyf0);
+        if (RECON) {
+          Vec4<T> sym1 = minmod2(Dm1.c[k] - Dm2.c[k], R[0].c[k] - Dm1.c[k]);
+          sy.c[k] = minmod2(R[0].c[k] - Dm1.c[k], R[1].c[k] - R[0].c[k]);
+          Gp.c[k] = yface(qy, Dm1.c[k] + scale(hd, sym1), R[0].c[k] - scale(hd, sy.c[k]));
+        } else {
+          Gp.c[k] = yface(qy, Dm1.c[k], R[0].c[k]);
+        }
       }
-    } else {
 #pragma unroll
-      for (int k = 0; k < CPL; ++k) xr[k] = xl[k] = A.c[k];
+      for (int k = 0; k < URING; ++k) Ur[k] = Blk<T, CPL>{};
+      if (MODE == MODE_STAGE) {
+#pragma unroll
+        for (int k = 0; k < URING - 1; ++k) Ur[k] = load_cell(a.un, js + k * DIR);
+      }
     }
-    Vec4<T> F[CPL + 1];
-    const Vec4<T> xr_left = from_left(xr[CPL - 1]);
-#pragma unroll
-    for (int k = 0; k < CPL; ++k)
-      F[k] = face_flux<T, FLUX, 0, FORM>(P, face_eq(rxf[k], eyc), k == 0 ? xr_left : xr[k - 1], xl[k]);
-    F[CPL] = from_right(F[0]);
 
-    Blk<T, CPL> O;
-#pragma unroll
-    for (int k = 0; k < CPL; ++k) {
-      const Vec4<T>& D = A.c[k];
-      // residual: -dF/dx - dG/dy + (s(U) - s(U_eq)), s = (0, -rho g, -rho g, -g (m_x+m_y))
-      Vec4<T> L;
-      L.r = -(F[k + 1].r - F[k].r) * odx - (Gn.c[k].r - Gp.c[k].r) * ody;
-      L.mx = (-(F[k + 1].mx - F[k].mx) * odx - (Gn.c[k].mx - Gp.c[k].mx) * ody) - P.g * D.r;
-      L.my = (-(F[k + 1].my - F[k].my) * odx - (Gn.c[k].my - Gp.c[k].my) * ody) - P.g * D.r;
-      L.e = (-(F[k + 1].e - F[k].e) * odx - (Gn.c[k].e - Gp.c[k].e) * ody) - P.g * (D.mx + D.my);
-      if (BC == BC_FROZEN) {
-        const int col = x0 + k;
-        if (col == 0 || col == nx - 1 || j == 0 || j == ny - 1) L = {T(0), T(0), T(0), T(0)};
-      }
-      if (MODE == MODE_RESIDUAL) {
-        O.c[k] = L;
-      } else if (MODE == MODE_STAGE1) {
-        O.c[k] = D + scale(bdt, L);
-      } else {
-        O.c[k] = scale(alpha, UA.c[k]) + (scale(beta, D) + scale(bdt, L));
-      }
-    }
-    Gp = Gn;
-    if (own && j < y1) {
-      T* d = a.out + (long long)j * pitch;
-      Lanes<T, CPL> o0, o1, o2, o3;
+    // one row: A = row j, B = row j+DIR, C = row j+2DIR; S receives row
+    // j+(RING-1)DIR; UA holds U^n(j) and US receives U^n(j+(URING-1)DIR).
+    auto row_step = [&](int j, const Blk<T, CPL>& A, const Blk<T, CPL>& B, const Blk<T, CPL>& C,
+                        Blk<T, CPL>& S, const Blk<T, CPL>& UA, Blk<T, CPL>& US) {
+      S = load_row(a.in, j + (RING - 1) * DIR);
+      if (MODE == MODE_STAGE) US = load_cell(a.un, j + (URING - 1) * DIR);
+      const int jt = clampi(j, 0, ny - 1);
+      const double eyc = a.eyc[jt];
+      const double eyf = a.eyf[jt + (DIR > 0)];
+
+      // y faces between row j and row j+DIR
+      Blk<T, CPL> Gn;
 #pragma unroll
       for (int k = 0; k < CPL; ++k) {
-        o0.v[k] = O.c[k].r; o1.v[k] = O.c[k].mx; o2.v[k] = O.c[k].my; o3.v[k] = O.c[k].e;
+        FaceEq<T> qy = face_eq(rxc[k], eyf);
+        if (RECON) {
+          Vec4<T> syn = minmod2(B.c[k] - A.c[k], C.c[k] - B.c[k]);
+          Gn.c[k] = yface(qy, A.c[k] + scale(hd, sy.c[k]), B.c[k] - scale(hd, syn));
+          sy.c[k] = syn;
+        } else {
+          Gn.c[k] = yface(qy, A.c[k], B.c[k]);
+        }
       }
-      // a block may run past nx into the row padding, which is never read as data
-      stv<T, CPL>(d, vboff, rowb, o0);
-      stv<T, CPL>(d + plane, vboff, rowb, o1);
-      stv<T, CPL>(d + 2 * plane, vboff, rowb, o2);
-      stv<T, CPL>(d + 3 * plane, vboff, rowb, o3);
-      if (MODE != MODE_RESIDUAL && a.last) {
+      // x faces on the left of each column; the first takes its left state from
+      // the neighbouring lane's last column, the last column's right face is the
+      // right neighbour's first
+      Vec4<T> xr[CPL], xl[CPL];
+      if (RECON) {
+        Vec4<T> lm = from_left(A.c[CPL - 1]), rp = from_right(A.c[0]);
 #pragma unroll
         for (int k = 0; k < CPL; ++k) {
-          if (x0 + k < nx) {
-            bool ok;
-            T rct = T(rxc[k] * eyc);
-            T sp = cell_speed(P, rct, rct * eor, O.c[k], ok);
-            smax = fmax(smax, (double)sp);
-            bad |= !ok;
+          Vec4<T> am = k == 0 ? lm : A.c[k - 1];
+          Vec4<T> ap = k == CPL - 1 ? rp : A.c[k + 1];
+          Vec4<T> sx = minmod2(A.c[k] - am, ap - A.c[k]);
+          xr[k] = A.c[k] + scale(P.half, sx);
+          xl[k] = A.c[k] - scale(P.half, sx);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) xr[k] = xl[k] = A.c[k];
+      }
+      Vec4<T> F[CPL + 1];
+      const Vec4<T> xr_left = from_left(xr[CPL - 1]);
+#pragma unroll
+      for (int k = 0; k < CPL; ++k)
+        F[k] = face_flux<T, FLUX, 0, FORM>(P, face_eq(rxf[k], eyc), k == 0 ? xr_left : xr[k - 1], xl[k]);
+      F[CPL] = from_right(F[0]);
+
+      Blk<T, CPL> O;
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        const Vec4<T>& D = A.c[k];
+        // residual: -dF/dx - dG/dy + (s(U) - s(U_eq)), s = (0, -rho g, -rho g, -g (m_x+m_y));
+        // the same expression in both directions, on the flux above (Ga) and below (Gb)
+        const Vec4<T>& Ga = DIR > 0 ? Gn.c[k] : Gp.c[k];
+        const Vec4<T>& Gb = DIR > 0 ? Gp.c[k] : Gn.c[k];
+        Vec4<T> L;
+        L.r = -(F[k + 1].r - F[k].r) * odx - (Ga.r - Gb.r) * ody;
+        L.mx = (-(F[k + 1].mx - F[k].mx) * odx - (Ga.mx - Gb.mx) * ody) - P.g * D.r;
+        L.my = (-(F[k + 1].my - F[k].my) * odx - (Ga.my - Gb.my) * ody) - P.g * D.r;
+        L.e = (-(F[k + 1].e - F[k].e) * odx - (Ga.e - Gb.e) * ody) - P.g * (D.mx + D.my);
+        if (BC == BC_FROZEN) {
+          const int col = x0 + k;
+          if (col == 0 || col == nx - 1 || j == 0 || j == ny - 1) L = {T(0), T(0), T(0), T(0)};
+        }
+        if (MODE == MODE_RESIDUAL) {
+          O.c[k] = L;
+        } else if (MODE == MODE_STAGE1) {
+          O.c[k] = D + scale(bdt, L);
+        } else {
+          O.c[k] = scale(alpha, UA.c[k]) + (scale(beta, D) + scale(bdt, L));
+        }
+      }
+      Gp = Gn;
+      if (own && j >= y0 && j < y1) {
+        T* d = a.out + (long long)j * pitch;
+        Lanes<T, CPL> o0, o1, o2, o3;
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+          o0.v[k] = O.c[k].r; o1.v[k] = O.c[k].mx; o2.v[k] = O.c[k].my; o3.v[k] = O.c[k].e;
+        }
+        // a block may run past nx into the row padding, which is never read as data
+        stv<T, CPL>(d, vboff, rowb, o0);
+        stv<T, CPL>(d + plane, vboff, rowb, o1);
+        stv<T, CPL>(d + 2 * plane, vboff, rowb, o2);
+        stv<T, CPL>(d + 3 * plane, vboff, rowb, o3);
+        if (MODE != MODE_RESIDUAL && a.last) {
+#pragma unroll
+          for (int k = 0; k < CPL; ++k) {
+            if (x0 + k < nx) {
+              bool ok;
+              T rct = T(rxc[k] * eyc);
+              T sp = cell_speed(P, rct, rct * eor, O.c[k], ok);
+              smax = fmax(smax, (double)sp);
+              bad |= !ok;
+            }
           }
         }
       }
+    };
+
+    const int nrows = y1 - y0;
+    for (int n = 0; n < nrows; n += RING) {
+#pragma unroll
+      for (int k = 0; k < RING; ++k)
+        row_step(js + (n + k) * DIR, R[k % RING], R[(k + 1) % RING], R[(k + 2) % RING], R[(k + RING - 1) % RING],
+                 Ur[k % URING], Ur[(k + URING - 1) % URING]);
     }
   };
-
-  for (int j = y0; j < y1; j += RING) {
-#pragma unroll
-    for (int k = 0; k < RING; ++k)
-      row_step(j + k, R[k % RING], R[(k + 1) % RING], R[(k + 2) % RING], R[(k + RING - 1) % RING],
-               Ur[k % URING], Ur[(k + URING - 1) % URING]);
-  }
+  if (ty & 1)
+    sweep(std::integral_constant<int, -1>{});
+  else
+    sweep(std::integral_constant<int, 1>{});
 
   if (MODE != MODE_RESIDUAL && a.last) {
     if (bad) atomicOr(&a.ctrl->err, 1u);
