@@ -49,6 +49,50 @@ def test_residual_matches_oracle(shape, flux):
     assert err <= 2e-5 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("shape", [(1000, 1030), (2000, 301)])
+def test_residual_matches_oracle_large(shape):
+    """Grids large enough that the sweep runs segments of many rows (25 and 15
+    rows on a 256-CU device at 3 waves per SIMD) with short last segments, and
+    strips whose last lanes fall past nx."""
+    nx, ny = shape
+    say(f"case: DG residual {nx}x{ny} (multi-row segments)")
+    op = po.make_dg_params(nx, ny, g=1.3, rho0=1.1, p0=0.9)
+    d = np.asfortranarray(po.dg2d_fourier_state(op, amp=1e-3))
+    ref = po.dg2d_residual(op, d)
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        got = s.compute_residual(d)
+    e = np.abs(got - ref)
+    err = float(e.max())
+    where = np.unravel_index(np.argmax(e), e.shape)
+    row_err = e.max(axis=(0, 1, 3))
+    say(f"   max |L| {np.max(np.abs(ref)):.3e}, max err {err:.3e} at (var, i, j, mode) {where}, "
+        f"median row max {np.median(row_err):.3e}")
+    # FP32 flux rounding (about 2^-24 of the deviation flux) is divided by dx and
+    # dy, so the bound grows with the resolution; a wrong trace or face anywhere
+    # (a segment or strip edge) gives an error of the order of |L| itself
+    dmax = float(np.max(np.abs(d)))
+    assert err <= 2e-5 * np.max(np.abs(ref)) + 2e-6 * dmax * (nx / op.boxlen_x + ny / op.boxlen_y)
+    # no row stands out: segment boundaries are as accurate as interior rows
+    assert row_err.max() <= 50 * np.median(row_err)
+
+
+def test_run_matches_oracle_multirow_segments():
+    say("case: DG 3 SSP-RK3 steps on 1000x400 (10-row segments)")
+    op = po.make_dg_params(1000, 400, limiter=po.DG_LIMITER_TVB, g=1.0)
+    d = po.dg2d_limit(op, po.dg2d_fourier_state(op, amp=1e-4))
+    ref, t, n, dt = po.dg2d_advance(op, d, max_steps=3)
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        s.upload(d)
+        s.advance(3)
+        tg, dtg, ng = s.time()
+        got = s.download()
+    assert ng == n == 3 and abs(tg - t) <= 1e-6 * t
+    scale = np.max(np.abs(ref))
+    err = np.max(np.abs(got - ref))
+    say(f"   max |d| {scale:.3e}, max err {err:.3e}")
+    assert err <= 1e-5 * scale
+
+
 def test_rest_is_exact():
     say("case: DG rest")
     op = po.make_dg_params(37, 29, g=2.0)
