@@ -92,7 +92,13 @@ struct Args {
   int ntx, nseg, seg_rows;   // strips across x, segments along y, rows per segment
   int first, residual_only;
   double t_end, cfl_over;    // cfl / (2k+1)
+  double dtc;                // this stage's multiple of the step's dt
   float alpha, beta;
+  // the two-output stage (DUAL): out = alpha U^n + beta U + gam dt L(U) and
+  // xout = pu U^n + px X + pin U + pl dt L(U), X read from xin (own cells only)
+  const float* xin;
+  float* xout;
+  float gam, pu, px, pin, pl;
   float gamma, g, odx, ody;
   float rho0, p0, ep0;       // rho0, p0 and p0 gamma/(gamma-1): equilibrium rho, p and E+p per e^{-k(x+y)}
 };
@@ -160,7 +166,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return xcd * base + (xcd < rem ? xcd : rem) + idx;
 }
 
-template <int FLUX>
+template <int FLUX, bool DUAL>
 __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   const Ctrl* ck = a.ctrl;
   const int par = a.parity;
@@ -183,10 +189,11 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   const int nx = a.nx, ny = a.ny;
   const long long plane = a.plane;
   const int pitch = a.pitch;
-  // accumulator acc = U + dt L (the backward operands carry the factor dt),
-  // out = alpha U^n + beta acc; the residual alone is acc with dt = 1 and no U
-  const float sdt = a.residual_only ? 1.f : (float)dt;
-  const float ku = a.residual_only ? 0.f : 1.f;
+  // accumulator acc = U + dtc dt L (the backward operands carry the factor
+  // dtc dt), out = alpha U^n + beta acc; the residual alone is acc with dt = 1
+  // and no U; the two-output stage accumulates dt L alone
+  const float sdt = a.residual_only ? 1.f : (float)(DUAL ? dt : a.dtc * dt);
+  const float ku = (a.residual_only || DUAL) ? 0.f : 1.f;
   const float gdt = a.g * sdt;
   const float alpha = a.residual_only ? 0.f : a.alpha;
   const float beta = a.residual_only ? 1.f : a.beta;
@@ -259,6 +266,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
     return face_flux<float, FLUX, 1, FORM_DEV>(P, eq_at(a, exg * a.eyf[j]), dl, dr);
   };
   // finish row j: add the flux through its top face, combine and store
+  float Uo[4][2], Xo[4][2];   // the two-output stage: U and X of the row being finished
   auto finish = [&](int j, v4f (&acc)[4], const Vec4<float>& Ft, const float (&N)[4][2]) {
     const float ft[4] = {Ft.r, Ft.mx, Ft.my, Ft.e};
 #pragma unroll
@@ -266,8 +274,16 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const __amdgpu_buffer_rsrc_t r = rs(a.out, v, j);
-      stm(r, st0, alpha * N[v][0] + beta * acc[v][0]);
-      stm(r, st1, alpha * N[v][1] + beta * acc[v][1]);
+      if constexpr (!DUAL) {
+        stm(r, st0, alpha * N[v][0] + beta * acc[v][0]);
+        stm(r, st1, alpha * N[v][1] + beta * acc[v][1]);
+      } else {
+        stm(r, st0, alpha * N[v][0] + beta * Uo[v][0] + a.gam * acc[v][0]);
+        stm(r, st1, alpha * N[v][1] + beta * Uo[v][1] + a.gam * acc[v][1]);
+        const __amdgpu_buffer_rsrc_t rx = rs(a.xout, v, j);
+        stm(rx, st0, a.pu * N[v][0] + a.px * Xo[v][0] + a.pin * Uo[v][0] + a.pl * acc[v][0]);
+        stm(rx, st1, a.pu * N[v][1] + a.px * Xo[v][1] + a.pin * Uo[v][1] + a.pl * acc[v][1]);
+      }
     }
   };
 
@@ -292,6 +308,10 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
     // loads in flight during this row: the next row, and U^n of the row finished here
     load_row(a.in, j + 1, Bn);
     if (with_un && j > y0) load_own(a.un, j - 1, N);
+    if (DUAL && j > y0) {
+      load_own(a.in, j - 1, Uo);
+      load_own(a.xin, j - 1, Xo);
+    }
 
     v4f dn[4], df[4];
 #pragma unroll
@@ -354,6 +374,10 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   // ---- epilogue: the top face of the segment's last row (Bc holds row y1) ----------
   auto last = [&](const float (&Bc)[4][2], v4f (&accp)[4]) {
     if (with_un) load_own(a.un, y1 - 1, N);
+    if (DUAL) {
+      load_own(a.in, y1 - 1, Uo);
+      load_own(a.xin, y1 - 1, Xo);
+    }
     Vec4<float> yr;
     if (y1 < ny) {
       v4f df[4];
@@ -572,7 +596,8 @@ struct fvwb_dg {
   long long plane = 0;
   long long org = 0;              // offset of cell (0,0) inside the ghost frame
   int ntx = 0, nseg = 0, seg_rows = 0;
-  float* buf[3] = {nullptr, nullptr, nullptr};
+  float* buf[4] = {nullptr, nullptr, nullptr, nullptr};   // SSP(5,4) uses the fourth
+  int nbuf = 3;
   double* host_stage = nullptr;   // device FP64 AoS staging
   double *exn = nullptr, *eyn = nullptr, *exf = nullptr, *eyf = nullptr;
   float *exn32 = nullptr, *eyn32 = nullptr, *exf32 = nullptr, *eyf32 = nullptr;
@@ -600,7 +625,7 @@ extern "C" const char* fvwb_dg_last_error(fvwb_dg_handle h) { return h ? h->err.
 
 extern "C" int fvwb_dg_destroy(fvwb_dg_handle h) {
   if (!h) return FVWB_OK;
-  void* ps[] = {h->buf[0], h->buf[1], h->buf[2], h->host_stage, h->exn, h->eyn, h->exf, h->eyf,
+  void* ps[] = {h->buf[0], h->buf[1], h->buf[2], h->buf[3], h->host_stage, h->exn, h->eyn, h->exf, h->eyf,
                 h->exn32, h->eyn32, h->exf32, h->eyf32, h->C, h->ctrl};
   for (void* q : ps)
     if (q) (void)hipFree(q);
@@ -619,6 +644,8 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   if (p->limiter != FVWB_DG_LIMITER_NONE && p->limiter != FVWB_DG_LIMITER_TVB)
     return dg_fail(nullptr, FVWB_ERR_ARG, "bad limiter");
   if (p->bc != FVWB_BC_REFLECT) return dg_fail(nullptr, FVWB_ERR_ARG, "the DG path implements walls (FVWB_BC_REFLECT)");
+  if (p->integrator != FVWB_DG_SSPRK3 && p->integrator != FVWB_DG_SSPRK54)
+    return dg_fail(nullptr, FVWB_ERR_ARG, "integrator must be FVWB_DG_SSPRK3 or FVWB_DG_SSPRK54");
   if (!(p->gamma > 1.0) || !(p->boxlen_x > 0) || !(p->boxlen_y > 0) || !(p->cfl > 0) || !(p->rho0 > 0) ||
       !(p->p0 > 0) || !(p->g >= 0) || !(p->tvb_m >= 0))
     return dg_fail(nullptr, FVWB_ERR_ARG, "gamma > 1, positive box, cfl, rho0, p0 and g, M >= 0 required");
@@ -641,7 +668,8 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   auto bail = [&](int rc, const char* m) { h->err = m; g_dg_err = m; fvwb_dg_destroy(h); return rc; };
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(FVWB_ERR_HIP, "hipStreamCreate failed");
   const size_t sbytes = sizeof(float) * 4 * NM * (size_t)h->plane;
-  for (int b = 0; b < 3; ++b)
+  h->nbuf = p->integrator == FVWB_DG_SSPRK54 ? 4 : 3;
+  for (int b = 0; b < h->nbuf; ++b)
     if (hipMalloc(&h->buf[b], sbytes) != hipSuccess) return bail(FVWB_ERR_OOM, "device allocation failed");
   if (hipMalloc(&h->host_stage, sizeof(double) * 4 * NM * (size_t)nx * ny) != hipSuccess ||
       hipMalloc(&h->exn, sizeof(double) * 3 * nx) != hipSuccess || hipMalloc(&h->eyn, sizeof(double) * 3 * ny) != hipSuccess ||
@@ -652,7 +680,7 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
       hipMalloc(&h->ctrl, sizeof(Ctrl)) != hipSuccess)
     return bail(FVWB_ERR_OOM, "device allocation failed");
   // padding columns and rows stay zero for the whole life of the handle
-  for (int b = 0; b < 3; ++b) {
+  for (int b = 0; b < h->nbuf; ++b) {
     const long long n = 4LL * NM * h->plane;
     hipLaunchKernelGGL(dg_fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->buf[b], n);
   }
@@ -710,7 +738,7 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
     int dev = 0, cus = 0, blocks_per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, dg_stage_kernel<fvwb::FLUX_HLL>, 256, 0) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, dg_stage_kernel<fvwb::FLUX_HLL, false>, 256, 0) !=
             hipSuccess)
       return bail(FVWB_ERR_HIP, "device query failed");
     const long long slots = 4LL * std::max(1, cus) * std::max(1, blocks_per_cu);
@@ -744,6 +772,8 @@ static Args stage_args(fvwb_dg* h) {
   a.odx = (float)(p.nx / p.boxlen_x); a.ody = (float)(p.ny / p.boxlen_y);
   a.rho0 = (float)p.rho0; a.p0 = (float)p.p0; a.ep0 = (float)(p.p0 * p.gamma / (p.gamma - 1.0));
   a.t_end = INFINITY;
+  a.dtc = 1.0;
+  a.xin = nullptr; a.xout = nullptr;
   return a;
 }
 
@@ -770,10 +800,16 @@ static LimArgs lim_args(fvwb_dg* h, float* st) {
 
 static void launch_stage(fvwb_dg* h, const Args& a) {
   const dim3 grid((unsigned)((h->ntx * h->nseg + 3) / 4)), block(256);
-  if (h->p.flux == FVWB_FLUX_HLL)
-    hipLaunchKernelGGL(dg_stage_kernel<fvwb::FLUX_HLL>, grid, block, 0, h->stream, a);
-  else
-    hipLaunchKernelGGL(dg_stage_kernel<fvwb::FLUX_RUSANOV>, grid, block, 0, h->stream, a);
+  const bool hll = h->p.flux == FVWB_FLUX_HLL, dual = a.xout != nullptr;
+  if (hll && dual) {
+    hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_HLL, true>), grid, block, 0, h->stream, a);
+  } else if (hll) {
+    hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_HLL, false>), grid, block, 0, h->stream, a);
+  } else if (dual) {
+    hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_RUSANOV, true>), grid, block, 0, h->stream, a);
+  } else {
+    hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_RUSANOV, false>), grid, block, 0, h->stream, a);
+  }
 }
 
 static void launch_limit(fvwb_dg* h, const LimArgs& a) {
@@ -856,8 +892,20 @@ extern "C" int fvwb_dg_compute_residual(fvwb_dg_handle h, const double* modes, d
   return FVWB_OK;
 }
 
-// Enqueue one SSP-RK3 step: A -> B, (B,A) -> C, (C,A) -> A, each stage followed
-// by the limiter; the last also reduces the CFL rate and advances the clock.
+// Coefficients of the SSP(5,4) of 2d/benchmark_2d_dg.f90:686-707, as the
+// reference's default-real literals round them: c10, a20, a21, c21, a30, a32,
+// c32, a40, a43, c43, a50, a52, a53, a54, c53, c54.
+static const float SSP54[16] = {0.391752226571890f, 0.444370493651235f, 0.555629506348765f, 0.368410593050371f,
+                                0.620101851488403f, 0.379898148511597f, 0.251891774271694f, 0.178079954393132f,
+                                0.821920045606868f, 0.544974750228521f, 0.00683325884039f,  0.51723167208978f,
+                                0.12759831133288f,  0.34833675773694f,  0.08460416338212f,  0.22600748319395f};
+
+// Enqueue one step.  SSP-RK3: A -> B, (B,A) -> C, (C,A) -> A, each stage
+// followed by the limiter; the last also reduces the CFL rate and advances the
+// clock.  SSP(5,4) on four buffers: w1 = B, w2 = C, w3 = B, w4 = D; the fourth
+// stage also forms w5 - c54 dt L(w4) = a50 u + a52 w2 + a53 w3 + a54 w4 + c53 dt
+// L(w3) in C over w2 (each owned cell read before it is written), and the fifth
+// adds a54 w4 + c54 dt L(w4) to it in A.
 static int enqueue_step(fvwb_dg* h, double t_end) {
   float *A = h->buf[0], *B = h->buf[1], *Cc = h->buf[2];
   const int par = h->parity;
@@ -865,6 +913,40 @@ static int enqueue_step(fvwb_dg* h, double t_end) {
   a.parity = par; a.t_end = t_end;
   LimArgs la = lim_args(h, nullptr);
   la.parity = par; la.t_end = t_end;
+  if (h->p.integrator == FVWB_DG_SSPRK54) {
+    const float* c = SSP54;
+    float* D = h->buf[3];
+    // w1 = u + c10 dt L(u)
+    a.in = A; a.un = nullptr; a.out = B; a.alpha = 0.f; a.beta = 1.f; a.dtc = c[0]; a.first = 1;
+    launch_stage(h, a);
+    la.st = h->cell(1); la.cfl = 0; la.bookkeeping = 0;
+    if (la.limit) launch_limit(h, la);
+    // w2 = a20 u + a21 (w1 + (c21/a21) dt L(w1))
+    a.in = B; a.un = A; a.out = Cc; a.alpha = c[1]; a.beta = c[2]; a.dtc = (double)c[3] / c[2]; a.first = 0;
+    launch_stage(h, a);
+    la.st = h->cell(2);
+    if (la.limit) launch_limit(h, la);
+    // w3 = a30 u + a32 (w2 + (c32/a32) dt L(w2)), over w1
+    a.in = Cc; a.un = A; a.out = B; a.alpha = c[4]; a.beta = c[5]; a.dtc = (double)c[6] / c[5];
+    launch_stage(h, a);
+    la.st = h->cell(1);
+    if (la.limit) launch_limit(h, la);
+    // w4 = a40 u + a43 w3 + c43 dt L(w3) and P = a50 u + a52 w2 + a53 w3 + c53 dt L(w3)
+    a.in = B; a.un = A; a.out = D; a.alpha = c[7]; a.beta = c[8]; a.dtc = 1.0;
+    a.xin = Cc; a.xout = Cc; a.gam = c[9]; a.pu = c[10]; a.px = c[11]; a.pin = c[12]; a.pl = c[14];
+    launch_stage(h, a);
+    a.xin = nullptr; a.xout = nullptr;
+    la.st = h->cell(3);
+    if (la.limit) launch_limit(h, la);
+    // u = P + a54 (w4 + (c54/a54) dt L(w4))
+    a.in = D; a.un = Cc; a.out = A; a.alpha = 1.f; a.beta = c[13]; a.dtc = (double)c[15] / c[13];
+    launch_stage(h, a);
+    la.st = h->cell(0); la.cfl = 1; la.bookkeeping = 1;
+    launch_limit(h, la);
+    DGHIP(h, hipGetLastError());
+    h->parity ^= 1;
+    return FVWB_OK;
+  }
   // stage 1
   a.in = A; a.un = nullptr; a.out = B; a.alpha = 0.f; a.beta = 1.f; a.first = 1;
   launch_stage(h, a);

@@ -80,6 +80,7 @@ def make_params_1d(nx, *, bc=BC1D_WALL, recon=RECON_MUSCL_MINMOD, integrator=RK2
 
 
 DG_LIMITER_NONE, DG_LIMITER_TVB = 0, 1
+DG_SSPRK3, DG_SSPRK54 = 0, 1
 
 
 class DGParams(ctypes.Structure):
@@ -88,7 +89,7 @@ class DGParams(ctypes.Structure):
     _fields_ = [
         ("nvar", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
         ("degree", ctypes.c_int32), ("flux", ctypes.c_int32), ("limiter", ctypes.c_int32),
-        ("bc", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("bc", ctypes.c_int32), ("integrator", ctypes.c_int32),
         ("gamma", ctypes.c_double), ("boxlen_x", ctypes.c_double), ("boxlen_y", ctypes.c_double),
         ("cfl", ctypes.c_double), ("rho0", ctypes.c_double), ("p0", ctypes.c_double),
         ("g", ctypes.c_double), ("tvb_m", ctypes.c_double),
@@ -96,10 +97,10 @@ class DGParams(ctypes.Structure):
 
 
 def make_dg_params(nx, ny, *, degree=2, flux=FLUX_HLL, limiter=DG_LIMITER_TVB, gamma=1.4, boxlen_x=1.0,
-                   boxlen_y=1.0, cfl=0.8, rho0=1.0, p0=1.0, g=1.0, tvb_m=50.0) -> DGParams:
+                   boxlen_y=1.0, cfl=0.8, rho0=1.0, p0=1.0, g=1.0, tvb_m=50.0, integrator=0) -> DGParams:
     """Defaults: the DG gravity workload (P2, HLL, TVB with M = 50, walls, CFL 0.8)."""
-    return DGParams(4, nx, ny, degree, flux, limiter, BC_REFLECT, 0, gamma, boxlen_x, boxlen_y, cfl, rho0, p0,
-                    g, tvb_m)
+    return DGParams(4, nx, ny, degree, flux, limiter, BC_REFLECT, integrator, gamma, boxlen_x, boxlen_y, cfl,
+                    rho0, p0, g, tvb_m)
 
 
 _lib = None

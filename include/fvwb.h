@@ -279,6 +279,10 @@ int fvwb1d_get_stream(fvwb1d_handle h, void** stream);
 #define FVWB_DG_LIMITER_TVB   1   /* 2d/limiters.f90:30-54 generalized_minmod on the
                                      linear modes of each conservative deviation;
                                      a limited cell drops its quadratic modes   */
+#define FVWB_DG_SSPRK3        0   /* Shu-Osher SSP-RK3 (dg_with_source.f90:194-204)  */
+#define FVWB_DG_SSPRK54       1   /* five-stage fourth-order SSP Runge-Kutta of
+                                     2d/benchmark_2d_dg.f90:683-710 (solver 'RK4',
+                                     the reference's shipped DG integrator)     */
 
 typedef struct fvwb_dg_params {
   int32_t nvar;        /* must be 4                                             */
@@ -287,7 +291,7 @@ typedef struct fvwb_dg_params {
   int32_t flux;        /* FVWB_FLUX_RUSANOV / FVWB_FLUX_HLL                     */
   int32_t limiter;     /* FVWB_DG_LIMITER_*                                     */
   int32_t bc;          /* FVWB_BC_REFLECT (walls)                               */
-  int32_t reserved;    /* 0                                                     */
+  int32_t integrator;  /* FVWB_DG_SSPRK3 / FVWB_DG_SSPRK54                      */
   double  gamma, boxlen_x, boxlen_y;
   double  cfl;         /* dt = cfl / ((2k+1) max((|u|+c)/dx + (|v|+c)/dy))      */
   double  rho0, p0, g; /* equilibrium rho0*exp(-k(x+y)), k = rho0*g/p0          */
@@ -295,7 +299,7 @@ typedef struct fvwb_dg_params {
 } fvwb_dg_params;
 
 typedef struct fvwb_dg* fvwb_dg_handle;
-/* The device path: FP32 modes and arithmetic (FP64 equilibrium tables), the
+/* The device path: FP32 modes, arithmetic and equilibrium tables, the
  * projections on the matrix cores, degree 2 only. */
 int fvwb_dg_create(fvwb_dg_handle* h, const fvwb_dg_params* p);
 int fvwb_dg_destroy(fvwb_dg_handle h);
@@ -311,8 +315,9 @@ int fvwb_dg_compute_residual(fvwb_dg_handle h, const double* modes, double* dudt
 int fvwb_dg_upload(fvwb_dg_handle h, const double* modes);
 int fvwb_dg_init_fourier(fvwb_dg_handle h, int32_t nterms, const double* a, const double* mk, const double* nk,
                          const double* theta, double amp);
-/* nsteps SSP-RK3 steps (limiter after every stage; dt = cfl/(5 max rate) from
- * the device, the last step clipped to t_end; no host synchronisation inside). */
+/* nsteps steps of p.integrator (limiter after every stage; dt = cfl/(5 max rate)
+ * from the device for either integrator, the last step clipped to t_end; no host
+ * synchronisation inside). */
 int fvwb_dg_advance(fvwb_dg_handle h, int32_t nsteps, double t_end);
 int fvwb_dg_advance_async(fvwb_dg_handle h, int32_t nsteps, double t_end);
 int fvwb_dg_download(fvwb_dg_handle h, double* modes);

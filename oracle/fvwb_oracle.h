@@ -116,6 +116,7 @@ int    orc_dg2d_nmodes(int degree);
 void   orc_dg2d_residual(const fvwb_dg_params* p, const double* d, double* L);
 void   orc_dg2d_limit(const fvwb_dg_params* p, double* d);
 double orc_dg2d_max_rate(const fvwb_dg_params* p, const double* d);
+void   orc_dg2d_ssp54_coeffs(double c[16]);
 void   orc_dg2d_advance(const fvwb_dg_params* p, double* d, double t_end, int max_steps, double* t_out,
                         int* steps_out, double* dt_last);
 /* deviation at the (k+1)^2 volume points, AoS(4,nx,ny,(k+1)^2), point a + (k+1)*b */

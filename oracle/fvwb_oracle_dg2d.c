@@ -10,7 +10,8 @@
  * (the template of dg_with_source.f90:1749-2031 carried to 2D), Gauss-Legendre
  * volume and face quadrature with the reference's assembly
  * (2d/benchmark_2d_dg.f90:1455-1462), the configured FV face flux, walls, the
- * TVB limiter of 2d/limiters.f90:30-54 on the linear modes, SSP-RK3.  Pinned by
+ * TVB limiter of 2d/limiters.f90:30-54 on the linear modes, SSP-RK3 or the
+ * reference's SSP(5,4) (2d/benchmark_2d_dg.f90:683-708).  Pinned by
  * invariants in tests/ (exact rest, degree 0 == first-order FV, convergence
  * orders, mass identity).  Built with -ffp-contract=off.  Not linked by the
  * product.
@@ -334,35 +335,88 @@ double orc_dg2d_max_rate(const fvwb_dg_params* p, const double* d)
 
 /* SSP-RK3 with the limiter after every stage; the state is taken as limited
  * on entry.  dt = cfl/((2k+1) max rate), the last step clipped to t_end. */
+/* The five-stage fourth-order SSP Runge-Kutta of 2d/benchmark_2d_dg.f90:683-710
+ * (solver 'RK4'), in its Shu-Osher form.  The reference writes the coefficients
+ * as default-real literals, so they enter the FP64 arithmetic rounded to FP32;
+ * they are kept that way here:
+ *   w1 = u + c10 dt L(u)
+ *   w2 = a20 u + a21 w1 + c21 dt L(w1)
+ *   w3 = a30 u + a32 w2 + c32 dt L(w2)
+ *   w4 = a40 u + a43 w3 + c43 dt L(w3)
+ *   u' = a50 u + a52 w2 + a53 w3 + a54 w4 + c53 dt L(w3) + c54 dt L(w4)
+ * in the order c10, a20, a21, c21, a30, a32, c32, a40, a43, c43, a50, a52, a53,
+ * a54, c53, c54. */
+void orc_dg2d_ssp54_coeffs(double c[16])
+{
+  static const float lit[16] = {
+      0.391752226571890f,                                              /* :686 */
+      0.444370493651235f, 0.555629506348765f, 0.368410593050371f,      /* :690 */
+      0.620101851488403f, 0.379898148511597f, 0.251891774271694f,      /* :693 */
+      0.178079954393132f, 0.821920045606868f, 0.544974750228521f,      /* :696 */
+      0.00683325884039f, 0.51723167208978f, 0.12759831133288f, 0.34833675773694f,
+      0.08460416338212f,                                               /* :703-704 */
+      0.22600748319395f};                                              /* :707 */
+  for (int k = 0; k < 16; ++k) c[k] = (double)lit[k];
+}
+
 void orc_dg2d_advance(const fvwb_dg_params* p, double* d, double t_end, int max_steps, double* t_out,
                       int* steps_out, double* dt_last)
 {
   const size_t n = 4u * (size_t)p->nx * p->ny * orc_dg2d_nmodes(p->degree);
+  const int ssp54 = p->integrator == FVWB_DG_SSPRK54;
   double* L = malloc(sizeof(double) * n);
   double* d1 = malloc(sizeof(double) * n);
   double* d2 = malloc(sizeof(double) * n);
+  double* d3 = ssp54 ? malloc(sizeof(double) * n) : NULL;
+  double* d4 = ssp54 ? malloc(sizeof(double) * n) : NULL;
+  double* d5 = ssp54 ? malloc(sizeof(double) * n) : NULL;
+  double c[16];
+  orc_dg2d_ssp54_coeffs(c);
   double t = 0.0, dt = 0.0;
   int iter = 0;
   while (t < t_end && iter < max_steps) {
     const double rate = orc_dg2d_max_rate(p, d);
     dt = p->cfl / ((2.0 * p->degree + 1.0) * rate);
     if (t + dt > t_end) dt = t_end - t;
-    orc_dg2d_residual(p, d, L);
-    for (size_t q = 0; q < n; ++q) d1[q] = d[q] + dt * L[q];
-    orc_dg2d_limit(p, d1);
-    orc_dg2d_residual(p, d1, L);
-    for (size_t q = 0; q < n; ++q) d2[q] = 0.75 * d[q] + 0.25 * d1[q] + 0.25 * dt * L[q];
-    orc_dg2d_limit(p, d2);
-    orc_dg2d_residual(p, d2, L);
-    for (size_t q = 0; q < n; ++q) d[q] = (1.0 / 3.0) * d[q] + (2.0 / 3.0) * d2[q] + (2.0 / 3.0) * dt * L[q];
-    orc_dg2d_limit(p, d);
+    if (!ssp54) {
+      orc_dg2d_residual(p, d, L);
+      for (size_t q = 0; q < n; ++q) d1[q] = d[q] + dt * L[q];
+      orc_dg2d_limit(p, d1);
+      orc_dg2d_residual(p, d1, L);
+      for (size_t q = 0; q < n; ++q) d2[q] = 0.75 * d[q] + 0.25 * d1[q] + 0.25 * dt * L[q];
+      orc_dg2d_limit(p, d2);
+      orc_dg2d_residual(p, d2, L);
+      for (size_t q = 0; q < n; ++q) d[q] = (1.0 / 3.0) * d[q] + (2.0 / 3.0) * d2[q] + (2.0 / 3.0) * dt * L[q];
+      orc_dg2d_limit(p, d);
+    } else {
+      /* :685-708; L(w3) is evaluated twice there (:695, :700) with the same
+       * argument, once here */
+      orc_dg2d_residual(p, d, L);
+      for (size_t q = 0; q < n; ++q) d1[q] = d[q] + c[0] * dt * L[q];
+      orc_dg2d_limit(p, d1);
+      orc_dg2d_residual(p, d1, L);
+      for (size_t q = 0; q < n; ++q) d2[q] = c[1] * d[q] + c[2] * d1[q] + c[3] * dt * L[q];
+      orc_dg2d_limit(p, d2);
+      orc_dg2d_residual(p, d2, L);
+      for (size_t q = 0; q < n; ++q) d3[q] = c[4] * d[q] + c[5] * d2[q] + c[6] * dt * L[q];
+      orc_dg2d_limit(p, d3);
+      orc_dg2d_residual(p, d3, L);
+      for (size_t q = 0; q < n; ++q) d4[q] = c[7] * d[q] + c[8] * d3[q] + c[9] * dt * L[q];
+      orc_dg2d_limit(p, d4);
+      /* w5 takes the limited w4 (apply_limiter(w4) precedes it) and L(w3) */
+      for (size_t q = 0; q < n; ++q)
+        d5[q] = c[10] * d[q] + c[11] * d2[q] + c[12] * d3[q] + c[13] * d4[q] + c[14] * dt * L[q];
+      orc_dg2d_residual(p, d4, L);
+      for (size_t q = 0; q < n; ++q) d[q] = d5[q] + c[15] * dt * L[q];
+      orc_dg2d_limit(p, d);
+    }
     t = t + dt;
     iter = iter + 1;
   }
   if (t_out) *t_out = t;
   if (steps_out) *steps_out = iter;
   if (dt_last) *dt_last = dt;
-  free(L); free(d1); free(d2);
+  free(L); free(d1); free(d2); free(d3); free(d4); free(d5);
 }
 
 /* values of the deviation at the (k+1)^2 volume points, AoS(4,nx,ny,nq*nq)

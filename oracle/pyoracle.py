@@ -62,6 +62,7 @@ def make_params_1d(nx, *, bc=BC1D_ZERO_GRADIENT, recon=RECON_FIRST_ORDER, integr
 
 
 DG_LIMITER_NONE, DG_LIMITER_TVB = 0, 1
+DG_SSPRK3, DG_SSPRK54 = 0, 1
 
 
 class DGParams(ctypes.Structure):
@@ -70,7 +71,7 @@ class DGParams(ctypes.Structure):
     _fields_ = [
         ("nvar", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
         ("degree", ctypes.c_int32), ("flux", ctypes.c_int32), ("limiter", ctypes.c_int32),
-        ("bc", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("bc", ctypes.c_int32), ("integrator", ctypes.c_int32),
         ("gamma", ctypes.c_double), ("boxlen_x", ctypes.c_double), ("boxlen_y", ctypes.c_double),
         ("cfl", ctypes.c_double), ("rho0", ctypes.c_double), ("p0", ctypes.c_double),
         ("g", ctypes.c_double), ("tvb_m", ctypes.c_double),
@@ -78,9 +79,9 @@ class DGParams(ctypes.Structure):
 
 
 def make_dg_params(nx, ny, *, degree=2, flux=FLUX_HLL, limiter=DG_LIMITER_TVB, gamma=1.4, boxlen_x=1.0,
-                   boxlen_y=1.0, cfl=0.8, rho0=1.0, p0=1.0, g=1.0, tvb_m=50.0) -> DGParams:
-    return DGParams(4, nx, ny, degree, flux, limiter, BC_REFLECT, 0, gamma, boxlen_x, boxlen_y, cfl, rho0, p0,
-                    g, tvb_m)
+                   boxlen_y=1.0, cfl=0.8, rho0=1.0, p0=1.0, g=1.0, tvb_m=50.0, integrator=0) -> DGParams:
+    return DGParams(4, nx, ny, degree, flux, limiter, BC_REFLECT, integrator, gamma, boxlen_x, boxlen_y, cfl,
+                    rho0, p0, g, tvb_m)
 
 
 def make_params(nx, ny, *, bc=BC_FROZEN_RING, recon=RECON_FIRST_ORDER, flux=FLUX_RUSANOV,
@@ -152,6 +153,7 @@ def lib():
         L.orc_dg2d_max_rate.argtypes = [pd, dp]
         L.orc_dg2d_max_rate.restype = ctypes.c_double
         L.orc_dg2d_advance.argtypes = [pd, dp, ctypes.c_double, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_int), dp]
+        L.orc_dg2d_ssp54_coeffs.argtypes = [dp]
         L.orc_dg2d_nodal.argtypes = [pd, dp, dp]
         L.orc_dg2d_project.argtypes = [pd, dp, dp]
         _lib = L
@@ -455,6 +457,15 @@ def dg2d_advance(p: DGParams, d, t_end=float("inf"), max_steps=1 << 30):
     lib().orc_dg2d_advance(ctypes.byref(p), _p(d), t_end, max_steps, ctypes.byref(t), ctypes.byref(n),
                            ctypes.byref(dt))
     return d, t.value, n.value, dt.value
+
+
+def dg2d_ssp54_coeffs():
+    """Coefficients of the reference's SSP(5,4) (2d/benchmark_2d_dg.f90:686-707) as
+    the oracle uses them: c10, a20, a21, c21, a30, a32, c32, a40, a43, c43, a50, a52,
+    a53, a54, c53, c54."""
+    c = np.zeros(16)
+    lib().orc_dg2d_ssp54_coeffs(_p(c))
+    return c
 
 
 def dg2d_nodal(p: DGParams, d):

@@ -23,8 +23,8 @@ def say(*a):
 
 
 def gpu_p(op):
-    return fvwb.DGParams(op.nvar, op.nx, op.ny, op.degree, op.flux, op.limiter, op.bc, 0, op.gamma, op.boxlen_x,
-                         op.boxlen_y, op.cfl, op.rho0, op.p0, op.g, op.tvb_m)
+    return fvwb.DGParams(op.nvar, op.nx, op.ny, op.degree, op.flux, op.limiter, op.bc, op.integrator, op.gamma,
+                         op.boxlen_x, op.boxlen_y, op.cfl, op.rho0, op.p0, op.g, op.tvb_m)
 
 
 def smooth_noisy(op, amp=1e-3, noise=1e-4, seed=1):
@@ -93,9 +93,10 @@ def test_run_matches_oracle_multirow_segments():
     assert err <= 1e-5 * scale
 
 
-def test_rest_is_exact():
-    say("case: DG rest")
-    op = po.make_dg_params(37, 29, g=2.0)
+@pytest.mark.parametrize("integrator", (po.DG_SSPRK3, po.DG_SSPRK54))
+def test_rest_is_exact(integrator):
+    say(f"case: DG rest integrator={integrator}")
+    op = po.make_dg_params(37, 29, g=2.0, integrator=integrator)
     z = np.zeros((4, 37, 29, 6), order="F")
     with fvwb.SolverDG(gpu_p(op)) as s:
         assert np.count_nonzero(s.compute_residual(z)) == 0
@@ -121,10 +122,11 @@ def test_limiter_matches_oracle():
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-7 * np.max(np.abs(d)))
 
 
+@pytest.mark.parametrize("integrator", (po.DG_SSPRK3, po.DG_SSPRK54))
 @pytest.mark.parametrize("limiter", (po.DG_LIMITER_NONE, po.DG_LIMITER_TVB))
-def test_run_matches_oracle(limiter):
-    say(f"case: DG 30 SSP-RK3 steps limiter={limiter}")
-    op = po.make_dg_params(48, 40, limiter=limiter, g=1.0)
+def test_run_matches_oracle(limiter, integrator):
+    say(f"case: DG 30 steps limiter={limiter} integrator={integrator}")
+    op = po.make_dg_params(48, 40, limiter=limiter, g=1.0, integrator=integrator)
     d = po.dg2d_limit(op, po.dg2d_fourier_state(op, amp=1e-4))
     ref, t, n, dt = po.dg2d_advance(op, d, max_steps=30)
     with fvwb.SolverDG(gpu_p(op)) as s:
@@ -159,10 +161,33 @@ def test_fourier_initial_state_and_t_end():
     assert t == t_end and n == no
 
 
+def test_ssp54_t_end_and_limited_steps():
+    say("case: DG SSP(5,4) with the limiter active, to t_end")
+    op = po.make_dg_params(32, 24, tvb_m=50.0, integrator=po.DG_SSPRK54)
+    X, Y = po.dg2d_points(op)
+    nodal = np.zeros((4, 32, 24, 9), order="F")
+    nodal[3] = 0.05 * np.where(X < 0.37, 1.0, 0.0)      # a pressure step the limiter acts on
+    d = po.dg2d_limit(op, po.dg2d_project(op, nodal))
+    t_end = 4e-3
+    ref, t, n, _ = po.dg2d_advance(op, d, t_end=t_end)
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        s.upload(d)
+        s.advance(1000, t_end)
+        tg, _, ng = s.time()
+        got = s.download()
+    assert tg == t_end and ng == n
+    scale = np.max(np.abs(ref))
+    err = np.max(np.abs(got - ref))
+    say(f"   {n} steps, max |d| {scale:.3e}, max err {err:.3e}")
+    assert err <= 1e-5 * scale
+
+
 def test_error_paths():
     say("case: DG error paths")
     with pytest.raises(fvwb.FvwbError):
         fvwb.SolverDG(gpu_p(po.make_dg_params(8, 8, degree=1)))
+    with pytest.raises(fvwb.FvwbError):
+        fvwb.SolverDG(gpu_p(po.make_dg_params(8, 8, integrator=2)))
     op = po.make_dg_params(8, 8)
     with fvwb.SolverDG(gpu_p(op)) as s:
         with pytest.raises(fvwb.FvwbError) as e:

@@ -172,3 +172,67 @@ def test_tvb_limiter_keeps_means_and_smooth_data():
     dxp = np.diff(m, axis=0)
     a = s3 * lim0[3, 1:-1, :, 1]
     assert np.all(np.abs(a) <= np.maximum(np.abs(dxp[1:]), np.abs(dxp[:-1])) + 1e-15)
+
+
+# ---- the reference's SSP(5,4) (2d/benchmark_2d_dg.f90:683-708) ------------------------
+
+def _ssp54_butcher():
+    """Butcher tableau (A, b) of the Shu-Osher stages of the oracle's SSP(5,4):
+    every stage is u + dt sum_j A[i, j] L(Y_j)."""
+    c10, a20, a21, c21, a30, a32, c32, a40, a43, c43, a50, a52, a53, a54, c53, c54 = po.dg2d_ssp54_coeffs()
+    A = np.zeros((5, 5))
+    A[1, 0] = c10                                       # Y1 = w1
+    A[2] = a21 * A[1]; A[2, 1] += c21                   # Y2 = w2
+    A[3] = a32 * A[2]; A[3, 2] += c32                   # Y3 = w3
+    A[4] = a43 * A[3]; A[4, 3] += c43                   # Y4 = w4
+    b = a52 * A[2] + a53 * A[3] + a54 * A[4]
+    b[3] += c53
+    b[4] += c54
+    return A, b
+
+
+def test_ssp54_coefficients_satisfy_the_fourth_order_conditions():
+    """The eight order conditions of a fourth-order Runge-Kutta method, on the
+    tableau the Shu-Osher stages define; they hold to the FP32 rounding of the
+    reference's default-real literals (about 1e-7), which a wrong digit or a
+    dropped term anywhere breaks."""
+    c = po.dg2d_ssp54_coeffs()
+    assert np.array_equal(c, c.astype(np.float32).astype(np.float64))   # FP32 literals
+    c10, a20, a21, c21, a30, a32, c32, a40, a43, c43, a50, a52, a53, a54, c53, c54 = c
+    for s in (a20 + a21, a30 + a32, a40 + a43, a50 + a52 + a53 + a54):   # consistency
+        assert abs(s - 1.0) < 1e-7
+    A, b = _ssp54_butcher()
+    cc = A.sum(axis=1)
+    conds = [(b.sum(), 1.0), (b @ cc, 1 / 2), (b @ cc**2, 1 / 3), (b @ (A @ cc), 1 / 6),
+             (b @ cc**3, 1 / 4), (b @ (cc * (A @ cc)), 1 / 8), (b @ (A @ cc**2), 1 / 12),
+             (b @ (A @ (A @ cc)), 1 / 24)]
+    for got, want in conds:
+        assert abs(got - want) < 1e-6, conds
+    # and not fifth order: the method is the five-stage fourth-order one
+    assert abs(b @ cc**4 - 1 / 5) > 1e-4 or abs(b @ (A @ (A @ (A @ cc))) - 1 / 120) > 1e-4
+
+
+def _ssp54_run(cfl, t_end=0.04, n=12):
+    p = po.make_dg_params(n, n, degree=2, limiter=po.DG_LIMITER_NONE, integrator=po.DG_SSPRK54, cfl=cfl)
+    d = po.dg2d_fourier_state(p, amp=1e-3)
+    out, t, steps, _ = po.dg2d_advance(p, d, t_end=t_end)
+    assert t == t_end
+    return out, steps
+
+
+def test_ssp54_runs_converge_at_fourth_order_in_time():
+    """Same grid, dt halved twice: successive differences shrink by about 2^4
+    (SSP-RK3 gives 2^3)."""
+    a, na = _ssp54_run(0.8)
+    b, nb = _ssp54_run(0.4)
+    c, nc = _ssp54_run(0.2)
+    assert na < nb < nc
+    r = np.max(np.abs(a - b)) / np.max(np.abs(b - c))
+    assert r > 12.0, r
+
+
+def test_ssp54_keeps_rest_exact():
+    p = po.make_dg_params(10, 9, degree=2, integrator=po.DG_SSPRK54, g=2.0)
+    z = np.zeros((4, 10, 9, 6), order="F")
+    out, t, steps, _ = po.dg2d_advance(p, z, max_steps=7)
+    assert steps == 7 and np.count_nonzero(out) == 0
