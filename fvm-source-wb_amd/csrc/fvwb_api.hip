@@ -84,7 +84,10 @@ static int validate(const fvwb_params* p, std::string& why) {
   if ((long long)p->nx * p->ny > (1ll << 31)) { why = "nx*ny exceeds 2^31 cells"; return FVWB_ERR_ARG; }
   if (p->bc != FVWB_BC_FROZEN_RING && p->bc != FVWB_BC_REFLECT) { why = "unknown bc"; return FVWB_ERR_ARG; }
   if (p->recon != FVWB_RECON_FIRST_ORDER && p->recon != FVWB_RECON_MUSCL_MINMOD) { why = "unknown recon"; return FVWB_ERR_ARG; }
-  if (p->flux != FVWB_FLUX_RUSANOV && p->flux != FVWB_FLUX_HLL) { why = "unknown flux"; return FVWB_ERR_ARG; }
+  if (p->flux != FVWB_FLUX_RUSANOV && p->flux != FVWB_FLUX_HLL) {
+    why = "flux must be FVWB_FLUX_RUSANOV or FVWB_FLUX_HLL (HLLC is a DG flux)";
+    return FVWB_ERR_ARG;
+  }
   if (p->integrator != FVWB_RK2_HEUN && p->integrator != FVWB_SSPRK3) { why = "unknown integrator"; return FVWB_ERR_ARG; }
   if (p->precision != FVWB_FP64 && p->precision != FVWB_FP32) { why = "unknown precision"; return FVWB_ERR_ARG; }
   if (!(p->gamma > 1.0) || !(p->boxlen_x > 0) || !(p->boxlen_y > 0) || !(p->cfl > 0) || !(p->rho0 > 0) ||

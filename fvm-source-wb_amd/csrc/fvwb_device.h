@@ -127,7 +127,7 @@ struct Phys {
 template <typename T>
 struct FaceEq { T rho, p, e, ep; };
 
-enum { FLUX_RUSANOV = 0, FLUX_HLL = 1 };
+enum { FLUX_RUSANOV = 0, FLUX_HLL = 1, FLUX_HLLC = 2 };
 enum { FORM_PLAIN = 0, FORM_DEV = 1 };
 
 // Per-side quantities.
@@ -188,12 +188,70 @@ __device__ __forceinline__ Side<T> side_dev(const Phys<T>& P, const FaceEq<T>& q
   return s;
 }
 
+// HLLC minus the equilibrium flux, in deviation form (DEV): the three-wave
+// riemann_hllc of dg_with_source.f90:1318-1374 carried to 2D.  Wave speeds
+// S_L = min(u_L,u_R) - max(c_L,c_R), S_R = max(u_L,u_R) + max(c_L,c_R) on the
+// normal velocity; Lagrangian sound speeds D_L = rho_L (u_L - S_L),
+// D_R = rho_R (S_R - u_R); the acoustic star state u* and p* - p_eq from the
+// pressure deviations; the sampled state's flux with the tangential velocity of
+// the contact's upwind side.  Every term of a star flux carries u*, and u* and
+// p* - p_eq vanish exactly when both deviations do, so a resting atmosphere
+// gives exactly zero.  q.e must hold the equilibrium energy.
+template <typename T, int DIR>
+__device__ __forceinline__ Vec4<T> hllc_dev(const Phys<T>& P, const FaceEq<T>& q, Vec4<T> dl, Vec4<T> dr) {
+  struct HSide { T rho, un, ut, dp, c, e; Vec4<T> f; };
+  auto side = [&](Vec4<T> d) {
+    HSide s;
+    s.rho = q.rho + d.r;
+    const T inv = trcp(tmax(s.rho, T(1e-10)));
+    const T u = d.mx * inv, v = d.my * inv;
+    s.dp = P.gm1 * (d.e - P.half * (d.mx * u + d.my * v));
+    const T ep = (q.ep + d.e) + s.dp;
+    s.e = q.e + d.e;
+    s.c = tsqrt(P.gamma * tmax(q.p + s.dp, T(1e-10)) * inv);
+    if (DIR == 0) {
+      s.un = u; s.ut = v;
+      s.f = {d.mx, d.mx * u + s.dp, d.my * u, ep * u};
+    } else {
+      s.un = v; s.ut = u;
+      s.f = {d.my, d.mx * v, d.my * v + s.dp, ep * v};
+    }
+    return s;
+  };
+  const HSide L = side(dl), R = side(dr);
+  const T cm = tmax(L.c, R.c);
+  const T sl = tmin(L.un, R.un) - cm, sr = tmax(L.un, R.un) + cm;
+  const T DL = L.rho * (L.un - sl), DR = R.rho * (sr - R.un);
+  const T inv = trcp(DL + DR);
+  const T us = (DR * R.un + DL * L.un + (L.dp - R.dp)) * inv;
+  const T dps = L.dp + (DL * (R.dp - L.dp) + DL * DR * (L.un - R.un)) * inv;   // p* - p_eq
+  // star state of the contact's upwind side
+  const bool lft = us > T(0);
+  const T sk = lft ? sl : sr, rk = lft ? L.rho : R.rho, unk = lft ? L.un : R.un, utk = lft ? L.ut : R.ut;
+  const T ek = lft ? L.e : R.e, pk = q.p + (lft ? L.dp : R.dp), ps = q.p + dps;
+  const T den = trcp(sk - us);
+  const T rs = rk * (sk - unk) * den;
+  const T es = ((sk - unk) * ek - pk * unk + ps * us) * den;
+  const T mf = rs * us;
+  Vec4<T> fs;
+  if (DIR == 0) fs = {mf, mf * us + dps, mf * utk, us * (es + ps)};
+  else fs = {mf, mf * utk, mf * us + dps, us * (es + ps)};
+  // sampled region: left state, a star state, or the right state
+  const bool star = !(sl > T(0)) && (lft || sr > T(0));
+  const Vec4<T>& side_f = sl > T(0) ? L.f : R.f;
+  return star ? fs : side_f;
+}
+
 // Numerical flux minus the equilibrium flux at one face, given the deviations
 // on the left (dl) and right (dr).  At dl == dr the Rusanov and HLL forms used
 // here return the one-sided flux exactly (0.5*(f+f) = f; f + w*0 - q*0 = f), so
 // the deviation is exactly zero at equilibrium.
 template <typename T, int FLUX, int DIR, int FORM>
 __device__ __forceinline__ Vec4<T> face_flux(const Phys<T>& P, const FaceEq<T>& q, Vec4<T> dl, Vec4<T> dr) {
+  if constexpr (FLUX == FLUX_HLLC) {
+    static_assert(FORM == FORM_DEV, "HLLC is implemented in deviation form");
+    return hllc_dev<T, DIR>(P, q, dl, dr);
+  }
   Side<T> L, R;
   Vec4<T> fe = {T(0), T(0), T(0), T(0)};
   Vec4<T> jl = dl, jr = dr;  // states entering the jump terms

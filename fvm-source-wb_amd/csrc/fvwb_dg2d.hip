@@ -100,7 +100,7 @@ struct Args {
   float* xout;
   float gam, pu, px, pin, pl;
   float gamma, g, odx, ody;
-  float rho0, p0, ep0;       // rho0, p0 and p0 gamma/(gamma-1): equilibrium rho, p and E+p per e^{-k(x+y)}
+  float rho0, p0, e0, ep0;   // rho0, p0, p0/(gamma-1), p0 gamma/(gamma-1): equilibrium rho, p, E, E+p per e^{-k(x+y)}
 };
 
 __device__ inline float ctrl_rate(const Ctrl* c, int par) {
@@ -127,7 +127,7 @@ __device__ inline FaceEq<float> eq_at(const Args& a, float e) {
   FaceEq<float> q;
   q.rho = a.rho0 * e;
   q.p = a.p0 * e;
-  q.e = 0.f;           // not used by the deviation fluxes
+  q.e = a.e0 * e;      // used by HLLC only
   q.ep = a.ep0 * e;
   return q;
 }
@@ -640,7 +640,8 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   if (!p) return dg_fail(nullptr, FVWB_ERR_ARG, "params is NULL");
   if (p->nvar != 4 || p->nx < 2 || p->ny < 2) return dg_fail(nullptr, FVWB_ERR_ARG, "nvar must be 4, nx, ny >= 2");
   if (p->degree != 2) return dg_fail(nullptr, FVWB_ERR_ARG, "the device path implements degree 2 (P2)");
-  if (p->flux != FVWB_FLUX_RUSANOV && p->flux != FVWB_FLUX_HLL) return dg_fail(nullptr, FVWB_ERR_ARG, "bad flux");
+  if (p->flux != FVWB_FLUX_RUSANOV && p->flux != FVWB_FLUX_HLL && p->flux != FVWB_FLUX_HLLC)
+    return dg_fail(nullptr, FVWB_ERR_ARG, "bad flux");
   if (p->limiter != FVWB_DG_LIMITER_NONE && p->limiter != FVWB_DG_LIMITER_TVB)
     return dg_fail(nullptr, FVWB_ERR_ARG, "bad limiter");
   if (p->bc != FVWB_BC_REFLECT) return dg_fail(nullptr, FVWB_ERR_ARG, "the DG path implements walls (FVWB_BC_REFLECT)");
@@ -770,7 +771,8 @@ static Args stage_args(fvwb_dg* h) {
   a.cfl_over = p.cfl / 5.0;
   a.gamma = (float)p.gamma; a.g = (float)p.g;
   a.odx = (float)(p.nx / p.boxlen_x); a.ody = (float)(p.ny / p.boxlen_y);
-  a.rho0 = (float)p.rho0; a.p0 = (float)p.p0; a.ep0 = (float)(p.p0 * p.gamma / (p.gamma - 1.0));
+  a.rho0 = (float)p.rho0; a.p0 = (float)p.p0; a.e0 = (float)(p.p0 / (p.gamma - 1.0));
+  a.ep0 = (float)(p.p0 * p.gamma / (p.gamma - 1.0));
   a.t_end = INFINITY;
   a.dtc = 1.0;
   a.xin = nullptr; a.xout = nullptr;
@@ -800,15 +802,19 @@ static LimArgs lim_args(fvwb_dg* h, float* st) {
 
 static void launch_stage(fvwb_dg* h, const Args& a) {
   const dim3 grid((unsigned)((h->ntx * h->nseg + 3) / 4)), block(256);
-  const bool hll = h->p.flux == FVWB_FLUX_HLL, dual = a.xout != nullptr;
-  if (hll && dual) {
-    hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_HLL, true>), grid, block, 0, h->stream, a);
-  } else if (hll) {
-    hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_HLL, false>), grid, block, 0, h->stream, a);
-  } else if (dual) {
-    hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_RUSANOV, true>), grid, block, 0, h->stream, a);
-  } else {
-    hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_RUSANOV, false>), grid, block, 0, h->stream, a);
+  const bool dual = a.xout != nullptr;
+  switch (h->p.flux) {
+    case FVWB_FLUX_HLL:
+      if (dual) hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_HLL, true>), grid, block, 0, h->stream, a);
+      else hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_HLL, false>), grid, block, 0, h->stream, a);
+      break;
+    case FVWB_FLUX_HLLC:
+      if (dual) hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_HLLC, true>), grid, block, 0, h->stream, a);
+      else hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_HLLC, false>), grid, block, 0, h->stream, a);
+      break;
+    default:
+      if (dual) hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_RUSANOV, true>), grid, block, 0, h->stream, a);
+      else hipLaunchKernelGGL((dg_stage_kernel<fvwb::FLUX_RUSANOV, false>), grid, block, 0, h->stream, a);
   }
 }
 

@@ -25,7 +25,7 @@ HEADER = os.path.join(os.path.dirname(PKG_DIR), "include", "fvwb.h")
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_NONPHYSICAL, ERR_STATE = 0, -1, -2, -3, -4, -5
 BC_FROZEN_RING, BC_REFLECT = 0, 1
 RECON_FIRST_ORDER, RECON_MUSCL_MINMOD = 0, 1
-FLUX_RUSANOV, FLUX_HLL = 0, 1
+FLUX_RUSANOV, FLUX_HLL, FLUX_HLLC = 0, 1, 2
 RK2_HEUN, SSPRK3 = 0, 1
 FP64, FP32 = 0, 1
 

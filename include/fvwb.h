@@ -47,9 +47,12 @@ extern "C" {
 #define FVWB_RECON_MUSCL_MINMOD  1
 /* numerical flux: RUSANOV is compute_llflux (benchmark_2d.f90:353-367, wave
  * speed |v|+c with |v| the velocity magnitude); HLL is the two-wave HLL flux with
- * Davis speed bounds on the signed normal velocity. */
+ * Davis speed bounds on the signed normal velocity; HLLC (DG only) is the
+ * three-wave riemann_hllc of dg_with_source.f90:1318-1374 carried to 2D, the
+ * tangential velocity advected by the contact. */
 #define FVWB_FLUX_RUSANOV     0
 #define FVWB_FLUX_HLL         1
+#define FVWB_FLUX_HLLC        2
 /* time integrator: RK2_HEUN is the reference's evolve (benchmark_2d.f90:246-250);
  * SSPRK3 is the Shu-Osher scheme of dg_with_source.f90:194-204. */
 #define FVWB_RK2_HEUN         0
@@ -288,7 +291,7 @@ typedef struct fvwb_dg_params {
   int32_t nvar;        /* must be 4                                             */
   int32_t nx, ny;      /* cells, >= 2                                           */
   int32_t degree;      /* polynomial degree k (the GPU path implements k = 2)   */
-  int32_t flux;        /* FVWB_FLUX_RUSANOV / FVWB_FLUX_HLL                     */
+  int32_t flux;        /* FVWB_FLUX_RUSANOV / FVWB_FLUX_HLL / FVWB_FLUX_HLLC    */
   int32_t limiter;     /* FVWB_DG_LIMITER_*                                     */
   int32_t bc;          /* FVWB_BC_REFLECT (walls)                               */
   int32_t integrator;  /* FVWB_DG_SSPRK3 / FVWB_DG_SSPRK54                      */

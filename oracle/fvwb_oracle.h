@@ -47,6 +47,13 @@ double orc_generalized_minmod(double x, double y, double z, double M, double dx)
 void   orc_ref_hllflux(double gamma, const double ul[4], const double ur[4],
                        const double fl[4], const double fr[4], double out[4]);
 
+/* dg_with_source.f90:1318-1374 riemann_hllc (1D, u = (rho, rho v, E)), as written */
+void   orc_ref_riemann_hllc_1d(double gamma, const double ul[3], const double ur[3], double f[3]);
+/* its 2D extension used by the configured schemes (flux FVWB_FLUX_HLLC): the same
+ * wave speeds and acoustic star state along direction dir (0 x, 1 y), the
+ * tangential velocity taken from the upwind side of the contact */
+void   orc_hllc_2d(double gamma, int dir, const double ul[4], const double ur[4], double f[4]);
+
 /* ---- benchmark_2d.f90 restatements --------------------------------------- */
 /* get_equilibrium_solution (174-218) at nx*ny points (x,y given per point). */
 void   orc_ref2d_equilibrium(int nequilibrium, int n, const double* x, const double* y, double* w);

@@ -322,6 +322,70 @@ static void face_devs(const dev_view* v, int dir, int ia, int ja, double dl[4], 
   }
 }
 
+/* riemann_hllc of dg_with_source.f90:1318-1374, statement for statement (its
+ * compute_primitive :1213-1223 inlined). */
+void orc_ref_riemann_hllc_1d(double gamma, const double ul[3], const double ur[3], double f[3])
+{
+  double wl[3], wr[3];
+  wl[0] = ul[0]; wl[1] = ul[1] / wl[0]; wl[2] = (gamma - 1.0) * (ul[2] - 0.5 * wl[0] * (wl[1] * wl[1]));
+  wr[0] = ur[0]; wr[1] = ur[1] / wr[0]; wr[2] = (gamma - 1.0) * (ur[2] - 0.5 * wr[0] * (wr[1] * wr[1]));
+  const double cl = sqrt(gamma * fmax(wl[2], 1e-10) / fmax(wl[0], 1e-10));
+  const double cr = sqrt(gamma * fmax(wr[2], 1e-10) / fmax(wr[0], 1e-10));
+  const double sl = fmin(wl[1], wr[1]) - fmax(cl, cr);
+  const double sr = fmax(wl[1], wr[1]) + fmax(cl, cr);
+  const double dl = wl[0] * (wl[1] - sl);
+  const double dr = wr[0] * (sr - wr[1]);
+  const double us = (dr * wr[1] + dl * wl[1] + (wl[2] - wr[2])) / (dl + dr);
+  const double ps = (dr * wl[2] + dl * wr[2] + dl * dr * (wl[1] - wr[1])) / (dl + dr);
+  const double rsl = wl[0] * (sl - wl[1]) / (sl - us);
+  const double esl = ((sl - wl[1]) * ul[2] - wl[2] * wl[1] + ps * us) / (sl - us);
+  const double rsr = wr[0] * (sr - wr[1]) / (sr - us);
+  const double esr = ((sr - wr[1]) * ur[2] - wr[2] * wr[1] + ps * us) / (sr - us);
+  double g0, g1, g2, e;
+  if (sl > 0.0) { g0 = wl[0]; g1 = wl[1]; g2 = wl[2]; e = ul[2]; }
+  else if (us > 0.0) { g0 = rsl; g1 = us; g2 = ps; e = esl; }
+  else if (sr > 0.0) { g0 = rsr; g1 = us; g2 = ps; e = esr; }
+  else { g0 = wr[0]; g1 = wr[1]; g2 = wr[2]; e = ur[2]; }
+  f[0] = g0 * g1;
+  f[1] = g0 * g1 * g1 + g2;
+  f[2] = g1 * (e + g2);
+}
+
+/* The 2D extension of riemann_hllc: normal velocity un = u[1+dir]/rho, the
+ * tangential one carried by the contact from its upwind side.  The star pressure
+ * is written p_L + (D_L (p_R - p_L) + D_L D_R (u_L - u_R))/(D_L + D_R), equal to
+ * the reference's (D_R p_L + D_L p_R + D_L D_R (u_L - u_R))/(D_L + D_R) but exactly
+ * p at a resting state, so the flux of equal resting states is the one-sided
+ * flux bit for bit. */
+void orc_hllc_2d(double gamma, int dir, const double ul[4], const double ur[4], double f[4])
+{
+  const int n = 1 + dir, t = 2 - dir;
+  double wl[4], wr[4];
+  orc_ref_primitive(gamma, ul, wl);
+  orc_ref_primitive(gamma, ur, wr);
+  const double cl = sqrt(gamma * fmax(wl[3], 1e-10) / fmax(wl[0], 1e-10));
+  const double cr = sqrt(gamma * fmax(wr[3], 1e-10) / fmax(wr[0], 1e-10));
+  const double sl = fmin(wl[n], wr[n]) - fmax(cl, cr);
+  const double sr = fmax(wl[n], wr[n]) + fmax(cl, cr);
+  const double dl = wl[0] * (wl[n] - sl);
+  const double dr = wr[0] * (sr - wr[n]);
+  const double us = (dr * wr[n] + dl * wl[n] + (wl[3] - wr[3])) / (dl + dr);
+  const double ps = wl[3] + (dl * (wr[3] - wl[3]) + dl * dr * (wl[n] - wr[n])) / (dl + dr);
+  double r, un, ut, p, e;
+  if (sl > 0.0) { r = wl[0]; un = wl[n]; ut = wl[t]; p = wl[3]; e = ul[3]; }
+  else if (us > 0.0) {
+    r = wl[0] * (sl - wl[n]) / (sl - us); un = us; ut = wl[t]; p = ps;
+    e = ((sl - wl[n]) * ul[3] - wl[3] * wl[n] + ps * us) / (sl - us);
+  } else if (sr > 0.0) {
+    r = wr[0] * (sr - wr[n]) / (sr - us); un = us; ut = wr[t]; p = ps;
+    e = ((sr - wr[n]) * ur[3] - wr[3] * wr[n] + ps * us) / (sr - us);
+  } else { r = wr[0]; un = wr[n]; ut = wr[t]; p = wr[3]; e = ur[3]; }
+  f[0] = r * un;
+  f[n] = r * un * un + p;
+  f[t] = r * un * ut;
+  f[3] = un * (e + p);
+}
+
 /* Numerical flux minus the equilibrium flux at the face (plain FP64 form). */
 static void face_flux_dev(const fvwb_params* p, int dir, const double ueq[4],
                           const double dl[4], const double dr[4], double out[4])
@@ -338,6 +402,8 @@ static void face_flux_dev(const fvwb_params* p, int dir, const double ueq[4],
   double fh[4];
   if (p->flux == FVWB_FLUX_RUSANOV) {
     orc_ref_llflux(gamma, ul, ur, fl, fr, fh);
+  } else if (p->flux == FVWB_FLUX_HLLC) {
+    orc_hllc_2d(gamma, dir, ul, ur, fh);
   } else {
     /* HLL with Davis bounds S_L = min(vL-cL, vR-cR), S_R = max(vL+cL, vR+cR) on
      * the signed normal velocity, written as F_L + w (F_R-F_L) - q (U_R-U_L)

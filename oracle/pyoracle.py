@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_fvwb.so")
 
 BC_FROZEN_RING, BC_REFLECT = 0, 1
 RECON_FIRST_ORDER, RECON_MUSCL_MINMOD = 0, 1
-FLUX_RUSANOV, FLUX_HLL = 0, 1
+FLUX_RUSANOV, FLUX_HLL, FLUX_HLLC = 0, 1, 2
 RK2_HEUN, SSPRK3 = 0, 1
 FP64, FP32 = 0, 1
 
@@ -124,6 +124,8 @@ def lib():
         L.orc_generalized_minmod.restype = ctypes.c_double
         for name in ("orc_ref_llflux", "orc_ref_hllflux"):
             getattr(L, name).argtypes = [ctypes.c_double, dp, dp, dp, dp, dp]
+        L.orc_ref_riemann_hllc_1d.argtypes = [ctypes.c_double, dp, dp, dp]
+        L.orc_hllc_2d.argtypes = [ctypes.c_double, ctypes.c_int, dp, dp, dp]
         L.orc_ref_flux.argtypes = [ctypes.c_double, dp, dp, dp]
         L.orc_ref_speed.argtypes = [ctypes.c_double, dp]
         L.orc_ref_speed.restype = ctypes.c_double
@@ -247,6 +249,22 @@ def ref_llflux(gamma, ul, ur, fl, fr):
     out = np.zeros(4)
     ul, ur, fl, fr = map(_vec, (ul, ur, fl, fr))
     lib().orc_ref_llflux(gamma, _p(ul), _p(ur), _p(fl), _p(fr), _p(out))
+    return out
+
+
+def ref_riemann_hllc_1d(gamma, ul, ur):
+    """riemann_hllc of dg_with_source.f90:1318-1374 (1D state (rho, rho v, E))."""
+    out = np.zeros(3)
+    ul, ur = map(_vec, (ul, ur))
+    lib().orc_ref_riemann_hllc_1d(gamma, _p(ul), _p(ur), _p(out))
+    return out
+
+
+def hllc_2d(gamma, direction, ul, ur):
+    """The 2D HLLC of the configured schemes along x (0) or y (1), full states."""
+    out = np.zeros(4)
+    ul, ur = map(_vec, (ul, ur))
+    lib().orc_hllc_2d(gamma, direction, _p(ul), _p(ur), _p(out))
     return out
 
 

@@ -68,9 +68,40 @@ def summarize(a):
     }
 
 
-def main():
+def hllc_fixture():
+    """riemann_hllc (dg_with_source.f90:1318-1374) on 1D state pairs chosen to
+    reach all four sampled regions (supersonic either way, star states either
+    side of the contact) plus equal states and stationary contacts."""
+    rng = np.random.default_rng(1318)
+    gamma = float(np.float32(1.4))          # the reference's gamma (an FP32 literal)
+    m = 384
+    ul = np.zeros((m, 3)); ur = np.zeros((m, 3))
+    for a in (ul, ur):
+        a[:, 0] = rng.uniform(0.2, 2.0, m)
+        a[:, 1] = rng.uniform(-3, 3, m) * a[:, 0]
+        a[:, 2] = rng.uniform(0.2, 3.0, m) / (gamma - 1) + 0.5 * a[:, 1] ** 2 / a[:, 0]
+    ul[:64, 1] = 4.0 * ul[:64, 0]; ur[:64, 1] = 4.0 * ur[:64, 0]            # supersonic to the right
+    ul[64:128, 1] = -4.0 * ul[64:128, 0]; ur[64:128, 1] = -4.0 * ur[64:128, 0]  # to the left
+    ur[128:160] = ul[128:160]                                              # equal states
+    ur[160:192, 1] = 0.0; ul[160:192, 1] = 0.0                             # stationary contacts
+    ur[160:192, 2] = ul[160:192, 2]
+    for a in (ul, ur):
+        a[:, 2] = np.maximum(a[:, 2], 0.2 / (gamma - 1) + 0.5 * a[:, 1] ** 2 / a[:, 0])
+    f = rr.run("dg1d_riemann_hllc", ul=ul, ur=ur, gamma=np.array(gamma))["f"]
+    return {"ul": ul, "ur": ur, "f": f, "gamma": np.array(gamma)}
+
+
+def main(only=None):
     assert rr.available("bench2d") and rr.available("dg2d"), "build oracle/_ref first"
     out = {}
+    if only == "dg1d_riemann_hllc":
+        assert rr.available("dg1d"), "build oracle/_ref first"
+        out["dg1d_riemann_hllc"] = hllc_fixture()
+        for name, arrs in out.items():
+            path = os.path.join(HERE, name + ".npz")
+            np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrs.items()})
+            print(f"wrote {path} ({os.path.getsize(path)} bytes)")
+        return
 
     # A1 compute_update_exact on a perturbed state and at equilibrium
     p, u, w = bench2d_inputs()
@@ -142,6 +173,9 @@ def main():
     f1 = rr.run("bench1d_llflux", ul=ul, ur=ur, fl=fl1, fr=fr1)["f"]
     out["bench1d_llflux"] = {"ul": ul, "ur": ur, "fl": fl1, "fr": fr1, "f": f1, "gamma": np.array(p1.gamma)}
 
+    # dg_with_source.f90 riemann_hllc (the HLLC of the DG codes)
+    out["dg1d_riemann_hllc"] = hllc_fixture()
+
     # A17 tensor-basis projections of the reference's 2D DG (8x8 cells, order 2)
     a = np.asfortranarray(np.random.default_rng(17).normal(size=(4, 8, 8, 2, 2)))
     out["dg2d_projections"] = {"a": a, "modes_from_nodes": rr.run("dg2d_modes_from_nodes", a=a)["r"],
@@ -154,4 +188,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

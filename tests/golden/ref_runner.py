@@ -112,6 +112,21 @@ def _child(op, fin, fout):
             out["f"] = res
         else:
             raise ValueError(op)
+    elif op.startswith("dg1d_"):
+        L = ctypes.CDLL(os.path.join(REF_DIR, LIBS["dg1d"]))
+        if op == "dg1d_riemann_hllc":
+            ul, ur = (np.ascontiguousarray(z[k]) for k in ("ul", "ur"))
+            gam = ctypes.c_double(float(z["gamma"]))
+            nv = ctypes.c_int(3)
+            res = np.zeros_like(ul)
+            for q in range(ul.shape[0]):
+                o = np.zeros(3)
+                a, b = np.ascontiguousarray(ul[q]), np.ascontiguousarray(ur[q])
+                L.riemann_hllc_(_dp(a), _dp(b), _dp(o), ctypes.byref(gam), ctypes.byref(nv))
+                res[q] = o
+            out["f"] = res
+        else:
+            raise ValueError(op)
     elif op.startswith("dg2d_"):
         L = ctypes.CDLL(os.path.join(REF_DIR, LIBS["dg2d"]))
         d = ctypes.c_double

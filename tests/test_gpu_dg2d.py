@@ -35,7 +35,7 @@ def smooth_noisy(op, amp=1e-3, noise=1e-4, seed=1):
 
 
 @pytest.mark.parametrize("shape", [(16, 16), (40, 37), (17, 5), (64, 64), (2, 33)])
-@pytest.mark.parametrize("flux", (po.FLUX_HLL, po.FLUX_RUSANOV))
+@pytest.mark.parametrize("flux", (po.FLUX_HLL, po.FLUX_RUSANOV, po.FLUX_HLLC))
 def test_residual_matches_oracle(shape, flux):
     nx, ny = shape
     say(f"case: DG residual {nx}x{ny} flux={flux}")
@@ -93,10 +93,11 @@ def test_run_matches_oracle_multirow_segments():
     assert err <= 1e-5 * scale
 
 
-@pytest.mark.parametrize("integrator", (po.DG_SSPRK3, po.DG_SSPRK54))
-def test_rest_is_exact(integrator):
-    say(f"case: DG rest integrator={integrator}")
-    op = po.make_dg_params(37, 29, g=2.0, integrator=integrator)
+@pytest.mark.parametrize("integrator,flux", [(po.DG_SSPRK3, po.FLUX_HLL), (po.DG_SSPRK54, po.FLUX_HLL),
+                                             (po.DG_SSPRK3, po.FLUX_HLLC)])
+def test_rest_is_exact(integrator, flux):
+    say(f"case: DG rest integrator={integrator} flux={flux}")
+    op = po.make_dg_params(37, 29, g=2.0, integrator=integrator, flux=flux)
     z = np.zeros((4, 37, 29, 6), order="F")
     with fvwb.SolverDG(gpu_p(op)) as s:
         assert np.count_nonzero(s.compute_residual(z)) == 0
@@ -159,6 +160,50 @@ def test_fourier_initial_state_and_t_end():
         assert s.time() == (t, dt, n)
     _, to, no, _ = po.dg2d_advance(op, ref, t_end=t_end)
     assert t == t_end and n == no
+
+
+@pytest.mark.parametrize("integrator", (po.DG_SSPRK3, po.DG_SSPRK54))
+def test_hllc_run_matches_oracle(integrator):
+    say(f"case: DG HLLC 30 steps with TVB, integrator={integrator}")
+    op = po.make_dg_params(48, 40, flux=po.FLUX_HLLC, g=1.0, integrator=integrator)
+    d = po.dg2d_limit(op, po.dg2d_fourier_state(op, amp=1e-4))
+    ref, t, n, _ = po.dg2d_advance(op, d, max_steps=30)
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        s.upload(d)
+        s.advance(30)
+        tg, _, ng = s.time()
+        got = s.download()
+    assert ng == n == 30 and abs(tg - t) <= 1e-6 * t
+    scale = np.max(np.abs(ref))
+    err = np.max(np.abs(got - ref))
+    say(f"   max |d| {scale:.3e}, max err {err:.3e}")
+    assert err <= 1e-5 * scale
+
+
+def test_hllc_contact_and_limiter_match_oracle():
+    """A density and shear contact plus a pressure step: every HLLC region and the
+    limiter are exercised."""
+    say("case: DG HLLC across a contact and a step")
+    op = po.make_dg_params(32, 24, flux=po.FLUX_HLLC, tvb_m=50.0)
+    X, Y = po.dg2d_points(op)
+    nodal = np.zeros((4, 32, 24, 9), order="F")
+    nodal[0] = np.where(Y < 0.45, 0.2, -0.1)                 # density jump
+    nodal[1] = np.where(Y < 0.45, 0.05, -0.05)               # shear
+    nodal[3] = 0.05 * np.where(X < 0.37, 1.0, 0.0)
+    d = po.dg2d_limit(op, po.dg2d_project(op, nodal))
+    ref = po.dg2d_residual(op, d)
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        got = s.compute_residual(d)
+        r, t, n, _ = po.dg2d_advance(op, d, max_steps=5)
+        s.upload(d)
+        s.advance(5)
+        g5 = s.download()
+    err = np.max(np.abs(got - ref))
+    say(f"   residual: max |L| {np.max(np.abs(ref)):.3e}, max err {err:.3e}")
+    assert err <= 2e-5 * np.max(np.abs(ref))
+    err5 = np.max(np.abs(g5 - r))
+    say(f"   5 steps: max |d| {np.max(np.abs(r)):.3e}, max err {err5:.3e}")
+    assert err5 <= 1e-5 * np.max(np.abs(r))
 
 
 def test_ssp54_t_end_and_limited_steps():

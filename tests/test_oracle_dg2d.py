@@ -77,7 +77,7 @@ def test_projection_of_nodal_values_is_identity(degree):
 
 
 @pytest.mark.parametrize("degree", (0, 1, 2))
-@pytest.mark.parametrize("flux", (po.FLUX_RUSANOV, po.FLUX_HLL))
+@pytest.mark.parametrize("flux", (po.FLUX_RUSANOV, po.FLUX_HLL, po.FLUX_HLLC))
 def test_rest_is_exact(degree, flux):
     p = po.make_dg_params(7, 6, degree=degree, flux=flux, g=1.7, rho0=1.2, p0=0.9)
     d = np.zeros((4, 7, 6, po.dg_nmodes(degree)), order="F")
@@ -86,7 +86,7 @@ def test_rest_is_exact(degree, flux):
     assert n == 5 and np.count_nonzero(d2) == 0
 
 
-@pytest.mark.parametrize("flux", (po.FLUX_RUSANOV, po.FLUX_HLL))
+@pytest.mark.parametrize("flux", (po.FLUX_RUSANOV, po.FLUX_HLL, po.FLUX_HLLC))
 def test_degree_zero_is_first_order_fv(flux):
     """P0 with one Gauss point per direction is the first-order FV scheme with
     walls: the same face fluxes at the face midpoints and the same source."""

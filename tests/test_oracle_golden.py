@@ -78,6 +78,44 @@ def test_hllflux_as_written_matches_reference_golden():
     np.testing.assert_allclose(out, g["f"], rtol=1e-14, atol=1e-14)
 
 
+def test_riemann_hllc_matches_reference_golden_bitwise():
+    """riemann_hllc (dg_with_source.f90:1318-1374) restated statement for
+    statement: equal to the flang build's output bit for bit on states that reach
+    all four sampled regions."""
+    g = load("dg1d_riemann_hllc")
+    gam = float(g["gamma"])
+    out = np.array([po.ref_riemann_hllc_1d(gam, a, b) for a, b in zip(g["ul"], g["ur"])])
+    np.testing.assert_array_equal(out, g["f"])
+    # the fixture covers every region: supersonic both ways and both star states
+    regions = set()
+    for a, b in zip(g["ul"], g["ur"]):
+        wl, wr = a[1] / a[0], b[1] / b[0]
+        pl = (gam - 1) * (a[2] - 0.5 * a[0] * wl * wl)
+        pr = (gam - 1) * (b[2] - 0.5 * b[0] * wr * wr)
+        cl, cr = np.sqrt(gam * pl / a[0]), np.sqrt(gam * pr / b[0])
+        sl, sr = min(wl, wr) - max(cl, cr), max(wl, wr) + max(cl, cr)
+        regions.add("L" if sl > 0 else ("R" if sr <= 0 else "star"))
+    assert regions == {"L", "R", "star"}
+
+
+def test_hllc_2d_reduces_to_reference_riemann_hllc():
+    """The 2D HLLC of the configured schemes, on states without tangential
+    velocity, along x and along y (rotated), against the reference's own 1D HLLC:
+    the same fluxes to rounding (its star pressure is rearranged), zero tangential
+    flux."""
+    g = load("dg1d_riemann_hllc")
+    gam = float(g["gamma"])
+    for a, b, f in zip(g["ul"], g["ur"], g["f"]):
+        for d in (0, 1):
+            ua = np.zeros(4); ub = np.zeros(4)
+            ua[0], ua[1 + d], ua[3] = a
+            ub[0], ub[1 + d], ub[3] = b
+            f2 = po.hllc_2d(gam, d, ua, ub)
+            want = np.array([f[0], 0.0, 0.0, f[2]])
+            want[1 + d] = f[1]
+            np.testing.assert_allclose(f2, want, rtol=1e-12, atol=1e-12 * np.abs(f).max())
+
+
 @pytest.mark.skipif(not rr.available("bench2d"), reason="oracle/_ref not built")
 def test_update_exact_live_against_reference_other_input():
     """A second input (other noise salt, larger bump) straight from the flang

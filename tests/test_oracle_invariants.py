@@ -288,3 +288,60 @@ def test_sod_shock_tube_converges_to_exact_solution(flux):
     assert e_mu < 0.02 and e_mu < 0.8 * e_fo and e_mu2 < 0.7 * e_mu, (e_fo, e_mu, e_mu2)
     # the 1D problem stays 1D: rows identical, no y momentum
     assert np.max(np.abs(u[0, :, 0] - u[0, :, 3])) < 1e-13 and np.max(np.abs(u[2])) < 1e-13
+
+
+# ---- the 2D HLLC of the configured DG flux (its tangential part) ---------------------
+
+def _cons(gam, r, u, v, p):
+    return np.array([r, r * u, r * v, p / (gam - 1) + 0.5 * r * (u * u + v * v)])
+
+
+def _flux(gam, U, d):
+    fx, fy = po.ref_flux(gam, U)
+    return fx if d == 0 else fy
+
+
+def test_hllc_2d_contacts_and_consistency():
+    """Equal states give the physical flux; a contact with a jump in density and
+    tangential velocity is resolved without diffusion: at rest the flux is the
+    pressure alone, moving it is the upwind side's flux, whichever side that is."""
+    gam = 1.4
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        for d in (0, 1):
+            U = _cons(gam, rng.uniform(0.3, 2), rng.normal(), rng.normal(), rng.uniform(0.3, 2))
+            np.testing.assert_allclose(po.hllc_2d(gam, d, U, U), _flux(gam, U, d), rtol=1e-14, atol=1e-14)
+            p, r1, r2, t1, t2 = rng.uniform(0.3, 2), rng.uniform(0.3, 2), rng.uniform(0.3, 2), rng.normal(), rng.normal()
+            vel = [0.0, 0.0]
+            vel[1 - d] = t1
+            UL = _cons(gam, r1, *vel, p)
+            vel[1 - d] = t2
+            UR = _cons(gam, r2, *vel, p)
+            f = po.hllc_2d(gam, d, UL, UR)
+            want = np.zeros(4)
+            want[1 + d] = p
+            np.testing.assert_allclose(f, want, rtol=0, atol=1e-14)
+            for u0 in (0.3, -0.3):
+                vel = [0.0, 0.0]
+                vel[d], vel[1 - d] = u0, t1
+                UL = _cons(gam, r1, *vel, p)
+                vel[1 - d] = t2
+                UR = _cons(gam, r2, *vel, p)
+                up = UL if u0 > 0 else UR
+                np.testing.assert_allclose(po.hllc_2d(gam, d, UL, UR), _flux(gam, up, d), rtol=1e-13, atol=1e-13)
+
+
+def test_hllc_2d_mirror_antisymmetry():
+    """Reflecting the normal axis swaps the sides and negates the normal velocity:
+    F(mirror U_R, mirror U_L) = -mirror(F(U_L, U_R))."""
+    gam = 1.4
+    rng = np.random.default_rng(6)
+    for _ in range(100):
+        UL = _cons(gam, rng.uniform(0.3, 2), 2 * rng.normal(), rng.normal(), rng.uniform(0.3, 2))
+        UR = _cons(gam, rng.uniform(0.3, 2), 2 * rng.normal(), rng.normal(), rng.uniform(0.3, 2))
+        for d in (0, 1):
+            m = np.ones(4)
+            m[1 + d] = -1.0
+            f = po.hllc_2d(gam, d, UL, UR)
+            g = po.hllc_2d(gam, d, m * UR, m * UL)
+            np.testing.assert_allclose(g, -m * f, rtol=1e-12, atol=1e-13)
