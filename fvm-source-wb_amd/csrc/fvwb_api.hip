@@ -25,6 +25,8 @@ template <typename T>
 hipError_t launch_dev_to_aos(const T* in, const double* w, double* u, int nx, int ny, int pitch, long long plane,
                              double gamma, bool add_eq, hipStream_t s);
 hipError_t launch_max_speed_aos(const double* u, long long n, double gamma, unsigned long long* out, hipStream_t s);
+hipError_t launch_plain_update(const double* u, double* dudt, int nx, int ny, double gamma, double boxlen_x,
+                               double boxlen_y, hipStream_t s);
 template <typename T>
 hipError_t launch_cfl_init(const T* in, const double* rxc, const double* eyc, Ctrl* ctrl, int nx, int ny, int pitch,
                            long long plane, double gamma, double p_over_rho, double inv_gm1, hipStream_t s);
@@ -257,6 +259,22 @@ extern "C" int fvwb_compute_update_exact(fvwb_handle h, const double* u, const d
 
 extern "C" int fvwb_compute_residual(fvwb_handle h, const double* u, const double* w_eq, double* dudt) {
   return residual_host(h, u, w_eq, dudt, false);
+}
+
+// compute_update (benchmark_2d.f90:370-463), the plain non-well-balanced
+// residual; w_eq is part of the reference's signature and unused there too.
+extern "C" int fvwb_compute_update(fvwb_handle h, const double* u, const double* w_eq, double* dudt) {
+  (void)w_eq;
+  if (!h || !u || !dudt) return fail(h, FVWB_ERR_ARG, "NULL argument");
+  int rc = ensure_residual_buffers(h);
+  if (rc) return rc;
+  const fvwb_params& p = h->p;
+  const size_t bytes = 4 * (size_t)p.nx * p.ny * sizeof(double);
+  FVWB_HIP(h, hipMemcpyAsync(h->stage_u, u, bytes, hipMemcpyHostToDevice, h->stream));
+  FVWB_HIP(h, launch_plain_update(h->stage_u, h->stage_w, p.nx, p.ny, p.gamma, p.boxlen_x, p.boxlen_y, h->stream));
+  FVWB_HIP(h, hipMemcpyAsync(dudt, h->stage_w, bytes, hipMemcpyDeviceToHost, h->stream));
+  FVWB_HIP(h, hipStreamSynchronize(h->stream));
+  return FVWB_OK;
 }
 
 extern "C" int fvwb_compute_max_speed(fvwb_handle h, const double* u, double* cmax) {

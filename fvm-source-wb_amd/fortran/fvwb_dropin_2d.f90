@@ -57,6 +57,19 @@ subroutine compute_update_exact(u, w_eq, dudt)
   call fvwb_check(fvwb_compute_update_exact(h, u, w_eq, dudt), h, 'compute_update_exact')
 end subroutine compute_update_exact
 
+! The plain, non-well-balanced residual (benchmark_2d.f90:370-463).
+subroutine compute_update(u, w_eq, dudt)
+  use parameters_2d, only: nvar, nx, ny
+  use fvwb_iface, only: fvwb_compute_update, fvwb_check
+  use fvwb_dropin_2d, only: fvwb_dropin_handle
+  use iso_c_binding, only: c_ptr
+  implicit none
+  real(kind=8), dimension(1:nvar, 1:nx, 1:ny) :: u, w_eq, dudt
+  type(c_ptr) :: h
+  h = fvwb_dropin_handle()
+  call fvwb_check(fvwb_compute_update(h, u, w_eq, dudt), h, 'compute_update')
+end subroutine compute_update
+
 ! cmax = max over cells of |v| + c (benchmark_2d.f90:264-279).
 subroutine compute_max_speed(u, cmax)
   use parameters_2d, only: nvar, nx, ny

@@ -27,7 +27,7 @@ module fvwb_iface
   end type fvwb_params
 
   public :: fvwb_params_reference_2d, fvwb_create, fvwb_destroy, fvwb_last_error
-  public :: fvwb_compute_update_exact, fvwb_compute_max_speed
+  public :: fvwb_compute_update_exact, fvwb_compute_update, fvwb_compute_max_speed
   public :: fvwb_upload, fvwb_init_gaussian, fvwb_advance, fvwb_download, fvwb_download_deviation
   public :: fvwb_get_time
   public :: fvwb_batch_create, fvwb_batch_destroy, fvwb_batch_last_error, fvwb_batch_init_gaussian
@@ -81,6 +81,14 @@ module fvwb_iface
        real(c_double), intent(in) :: u(*), w_eq(*)
        real(c_double), intent(out) :: dudt(*)
      end function fvwb_compute_update_exact
+
+     ! drop-in for benchmark_2d.f90:370-463 (compute_update(u,w_eq,dudt), plain, non-WB)
+     integer(c_int) function fvwb_compute_update(h, u, w_eq, dudt) bind(C, name='fvwb_compute_update')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*), w_eq(*)
+       real(c_double), intent(out) :: dudt(*)
+     end function fvwb_compute_update
 
      ! drop-in for benchmark_2d.f90:264-279 (compute_max_speed(u,cmax))
      integer(c_int) function fvwb_compute_max_speed(h, u, cmax) bind(C, name='fvwb_compute_max_speed')

@@ -130,6 +130,7 @@ def lib():
         "fvwb_create": [ctypes.POINTER(H), pp],
         "fvwb_destroy": [H],
         "fvwb_compute_update_exact": [H, dp, dp, dp],
+        "fvwb_compute_update": [H, dp, dp, dp],
         "fvwb_compute_residual": [H, dp, dp, dp],
         "fvwb_compute_max_speed": [H, dp, dp],
         "fvwb_upload": [H, dp, dp],
@@ -261,6 +262,14 @@ class Solver2D:
         u, w_eq = _fortran(u, self.shape), _fortran(w_eq, self.shape)
         out = np.zeros(self.shape, order="F")
         _check(lib().fvwb_compute_update_exact(self._h, _dp(u), _dp(w_eq), _dp(out)), self._h)
+        return out
+
+    def compute_update(self, u, w_eq=None):
+        """dudt of compute_update(u, w_eq, dudt) (benchmark_2d.f90:370-463), the plain non-WB residual."""
+        u = _fortran(u, self.shape)
+        w_eq = np.zeros(self.shape, order="F") if w_eq is None else _fortran(w_eq, self.shape)
+        out = np.zeros(self.shape, order="F")
+        _check(lib().fvwb_compute_update(self._h, _dp(u), _dp(w_eq), _dp(out)), self._h)
         return out
 
     def compute_residual(self, u, w_eq):

@@ -105,6 +105,12 @@ int fvwb_compute_update_exact(fvwb_handle h, const double* u, const double* w_eq
  * (identical to compute_update_exact on its square grids). */
 int fvwb_compute_residual(fvwb_handle h, const double* u, const double* w_eq, double* dudt);
 
+/* Replaces `subroutine compute_update(u, w_eq, dudt)` (benchmark_2d.f90:370-463),
+ * the plain non-well-balanced residual: Rusanov between the cell states at every
+ * face, the pointwise source with grad phi = (1,1), a frozen ring; FP64, the
+ * reference's result bit for bit.  w_eq is accepted and unused, as there. */
+int fvwb_compute_update(fvwb_handle h, const double* u, const double* w_eq, double* dudt);
+
 /* Replaces `subroutine compute_max_speed(u, cmax)` (benchmark_2d.f90:264-279):
  * cmax = max over cells of sqrt(vx^2+vy^2) + sqrt(gamma*max(p,1e-10)/max(rho,1e-10)). */
 int fvwb_compute_max_speed(fvwb_handle h, const double* u, double* cmax);

@@ -59,6 +59,8 @@ void   orc_hllc_2d(double gamma, int dir, const double ul[4], const double ur[4]
 void   orc_ref2d_equilibrium(int nequilibrium, int n, const double* x, const double* y, double* w);
 /* compute_update_exact (465-618) on the reference's set-up: nequilibrium
  * selects the closed form of the face equilibrium; gamma and box from p. */
+/* :370-463 compute_update, the plain non-well-balanced residual */
+void   orc_ref2d_update_plain(const fvwb_params* p, const double* u, double* dudt);
 void   orc_ref2d_update_exact(const fvwb_params* p, int nequilibrium,
                               const double* u, const double* w_eq, double* dudt);
 /* compute_max_speed (264-279) */

@@ -137,6 +137,55 @@ void orc_fv2d_equilibrium(const fvwb_params* p, double x, double y, double w[4])
 
 /* ---- compute_update_exact restatement (benchmark_2d.f90:465-618) --------- */
 
+/* benchmark_2d.f90:370-463 compute_update: the plain, non-well-balanced
+ * residual.  Rusanov (compute_llflux) between the two cell states at every face,
+ * the pointwise source get_source (:327-350, grad phi = (1,1)), a frozen ring.
+ * The reference's last x face takes column ny (:418); that flux reaches only the
+ * ring, which is zeroed, so column nx is used here with the same result. */
+void orc_ref2d_update_plain(const fvwb_params* p, const double* u, double* dudt)
+{
+  const int nx = p->nx, ny = p->ny;
+  const double gamma = p->gamma;
+  const double dx = p->boxlen_x / (double)nx, dy = p->boxlen_y / (double)ny;
+  const double odx = 1.0 / dx, ody = 1.0 / dy;
+  const size_t nc = (size_t)nx * ny;
+  double* fx = malloc(sizeof(double) * 4 * nc);
+  double* fy = malloc(sizeof(double) * 4 * nc);
+  double* F = malloc(sizeof(double) * 4 * (size_t)(nx + 1) * ny);
+  double* G = malloc(sizeof(double) * 4 * (size_t)nx * (ny + 1));
+  for (size_t c = 0; c < nc; ++c) orc_ref_flux(gamma, u + 4 * c, fx + 4 * c, fy + 4 * c);
+  for (int j = 0; j < ny; ++j)
+    for (int f = 0; f <= nx; ++f) {
+      const int il = f == 0 ? 0 : f - 1, ir = f == nx ? nx - 1 : f;
+      const size_t a = (size_t)il + (size_t)nx * j, b = (size_t)ir + (size_t)nx * j;
+      orc_ref_llflux(gamma, u + 4 * a, u + 4 * b, fx + 4 * a, fx + 4 * b, F + 4 * ((size_t)f + (size_t)(nx + 1) * j));
+    }
+  for (int i = 0; i < nx; ++i)
+    for (int f = 0; f <= ny; ++f) {
+      const int jl = f == 0 ? 0 : f - 1, jr = f == ny ? ny - 1 : f;
+      const size_t a = (size_t)i + (size_t)nx * jl, b = (size_t)i + (size_t)nx * jr;
+      orc_ref_llflux(gamma, u + 4 * a, u + 4 * b, fy + 4 * a, fy + 4 * b, G + 4 * ((size_t)i + (size_t)nx * f));
+    }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const size_t c = (size_t)i + (size_t)nx * j;
+      double w[4], sv[4];
+      orc_ref_primitive(gamma, u + 4 * c, w);
+      sv[0] = 0.0;
+      sv[1] = -w[0] * 1.0;
+      sv[2] = -w[0] * 1.0;
+      sv[3] = -w[0] * (w[1] * 1.0 + w[2] * 1.0);
+      const double* Fl = F + 4 * ((size_t)i + (size_t)(nx + 1) * j);
+      const double* Fr = F + 4 * ((size_t)i + 1 + (size_t)(nx + 1) * j);
+      const double* Gb = G + 4 * ((size_t)i + (size_t)nx * j);
+      const double* Gt = G + 4 * ((size_t)i + (size_t)nx * (j + 1));
+      const int ring = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+      for (int k = 0; k < 4; ++k)
+        dudt[4 * c + k] = ring ? 0.0 : -(Fr[k] - Fl[k]) * odx - (Gt[k] - Gb[k]) * ody + sv[k];
+    }
+  free(fx); free(fy); free(F); free(G);
+}
+
 void orc_ref2d_update_exact(const fvwb_params* p, int nequilibrium,
                             const double* u, const double* w_eq, double* dudt)
 {

@@ -107,6 +107,7 @@ def lib():
         dp = ctypes.POINTER(ctypes.c_double)
         pp = ctypes.POINTER(Params)
         L.orc_ref2d_update_exact.argtypes = [pp, ctypes.c_int, dp, dp, dp]
+        L.orc_ref2d_update_plain.argtypes = [pp, dp, dp]
         L.orc_ref2d_max_speed.argtypes = [pp, dp]
         L.orc_ref2d_max_speed.restype = ctypes.c_double
         L.orc_ref2d_evolve.argtypes = [pp, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_int,
@@ -176,6 +177,14 @@ def ref2d_update_exact(p: Params, nequilibrium, u, w_eq):
     u, w_eq = fortran(u), fortran(w_eq)
     out = np.zeros_like(u, order="F")
     lib().orc_ref2d_update_exact(ctypes.byref(p), nequilibrium, _p(u), _p(w_eq), _p(out))
+    return out
+
+
+def ref2d_update_plain(p: Params, u):
+    """compute_update (benchmark_2d.f90:370-463), the plain non-WB residual."""
+    u = fortran(u)
+    out = np.zeros_like(u, order="F")
+    lib().orc_ref2d_update_plain(ctypes.byref(p), _p(u), _p(out))
     return out
 
 

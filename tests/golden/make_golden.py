@@ -91,12 +91,23 @@ def hllc_fixture():
     return {"ul": ul, "ur": ur, "f": f, "gamma": np.array(gamma)}
 
 
+def plain_fixture():
+    """compute_update (benchmark_2d.f90:370-463), the plain non-WB residual, on the
+    perturbed 300x300 input of the compute_update_exact fixture."""
+    p, u, w = bench2d_inputs(salt=3, amp_bump=1e-2, amp_noise=1e-3)
+    d = rr.run("bench2d_update_plain", u=u, w_eq=w)["dudt"]
+    return dict(summarize(d), salt=3, amp_bump=1e-2, amp_noise=1e-3)
+
+
 def main(only=None):
     assert rr.available("bench2d") and rr.available("dg2d"), "build oracle/_ref first"
     out = {}
-    if only == "dg1d_riemann_hllc":
-        assert rr.available("dg1d"), "build oracle/_ref first"
-        out["dg1d_riemann_hllc"] = hllc_fixture()
+    if only in ("dg1d_riemann_hllc", "bench2d_update_plain_300"):
+        if only == "dg1d_riemann_hllc":
+            assert rr.available("dg1d"), "build oracle/_ref first"
+            out["dg1d_riemann_hllc"] = hllc_fixture()
+        else:
+            out["bench2d_update_plain_300"] = plain_fixture()
         for name, arrs in out.items():
             path = os.path.join(HERE, name + ".npz")
             np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrs.items()})
@@ -172,6 +183,9 @@ def main(only=None):
     fr1 = np.array([po.ref1d_flux(p1.gamma, x) for x in ur])
     f1 = rr.run("bench1d_llflux", ul=ul, ur=ur, fl=fl1, fr=fr1)["f"]
     out["bench1d_llflux"] = {"ul": ul, "ur": ur, "fl": fl1, "fr": fr1, "f": f1, "gamma": np.array(p1.gamma)}
+
+    # A9 compute_update, the plain non-WB residual
+    out["bench2d_update_plain_300"] = plain_fixture()
 
     # dg_with_source.f90 riemann_hllc (the HLLC of the DG codes)
     out["dg1d_riemann_hllc"] = hllc_fixture()

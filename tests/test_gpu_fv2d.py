@@ -66,6 +66,36 @@ def test_compute_update_exact_matches_reference_golden_and_oracle():
         assert np.count_nonzero(a) == 0
 
 
+def test_compute_update_plain_matches_reference_golden_bitwise():
+    """compute_update (benchmark_2d.f90:370-463), the plain non-WB residual: the
+    device result equals the reference's (golden summary) and the oracle's bit for
+    bit on the reference's 300x300 set-up."""
+    say("case: compute_update (plain) 300x300 reference set-up")
+    g = load("bench2d_update_plain_300")
+    p, u, w = bench2d_inputs(salt=int(g["salt"]), amp_bump=float(g["amp_bump"]), amp_noise=float(g["amp_noise"]))
+    with fvwb.Solver2D(fvwb.reference_params(300, 300, 2)) as s:
+        d = s.compute_update(u, w)
+    np.testing.assert_array_equal(d, po.ref2d_update_plain(p, u))
+    sm = summarize(d)
+    for k in ("sample", "sum", "abssum", "maxabs"):
+        np.testing.assert_array_equal(sm[k], g[k])
+
+
+@pytest.mark.parametrize("shape", [(64, 48), (17, 90), (4, 5)])
+def test_compute_update_plain_other_shapes_bitwise(shape):
+    nx, ny = shape
+    say(f"case: compute_update (plain) {nx}x{ny}")
+    gp = fvwb.reference_params(nx, ny, 2)
+    p = orc_params(gp)
+    rng = np.random.default_rng(nx * 100 + ny)
+    w = po.equilibrium_primitive(p)
+    u = po.conservative(p.gamma, w)
+    u = u * (1 + 1e-2 * rng.normal(size=u.shape))
+    with fvwb.Solver2D(gp) as s:
+        d = s.compute_update(u)
+    np.testing.assert_array_equal(d, po.ref2d_update_plain(p, u))
+
+
 def test_compute_update_exact_equilibrium_is_bitwise_zero():
     say("case: compute_update_exact at equilibrium 300x300")
     p, u, w = bench2d_inputs()

@@ -38,6 +38,30 @@ def test_update_exact_matches_reference_golden():
     np.testing.assert_allclose(s["abssum"], g["abssum"], rtol=1e-12)
 
 
+def test_update_plain_matches_reference_golden_bitwise():
+    """compute_update (benchmark_2d.f90:370-463), the plain non-well-balanced
+    residual: no exp on this path, so the restatement is bit-identical."""
+    g = load("bench2d_update_plain_300")
+    p, u, w = bench2d_inputs(salt=int(g["salt"]), amp_bump=float(g["amp_bump"]), amp_noise=float(g["amp_noise"]))
+    s = summarize(po.ref2d_update_plain(p, u))
+    for k in ("idx", "sample", "sum", "abssum", "maxabs"):
+        np.testing.assert_array_equal(s[k], g[k])
+
+
+def test_update_plain_is_not_well_balanced():
+    """The plain residual of a resting atmosphere is a truncation error: the
+    Rusanov diffusion of the stratified density is first order, the imbalance of
+    pressure differences against the pointwise source second order."""
+    errs = []
+    for n in (40, 80):
+        p = po.make_params(n, n, gamma=po.GAMMA_REF)
+        w = po.equilibrium_primitive(p)
+        d = po.ref2d_update_plain(p, po.conservative(p.gamma, w))
+        errs.append((np.max(np.abs(d[0])), np.max(np.abs(d[2]))))
+    assert errs[0][0] > 1e-2 and 1.6 < errs[0][0] / errs[1][0] < 2.4
+    assert errs[0][1] > 1e-5 and 3.2 < errs[0][1] / errs[1][1] < 4.8
+
+
 def test_update_exact_equilibrium_is_zero_like_reference():
     g = load("bench2d_update_exact_eq_300")
     assert int(g["nonzero"]) == 0 and np.all(g["maxabs"] == 0.0)
