@@ -1,10 +1,12 @@
 // fvwb_fv1d.hip — 1D well-balanced Euler with gravity (benchmark_1d.f90) on MI355X.
 //
-// Two device paths:
+// Device paths:
 //  * fv1d_ref_update_kernel: the drop-in for compute_update (benchmark_1d.f90:
 //    263-377), one thread per cell, global memory, FP64 rounding statement by
 //    statement (no contraction, IEEE division and square root), so it returns
 //    the reference's dudt bit for bit given the same face equilibrium table.
+//  * fv1d_ref_update_sr_kernel: the drop-in for compute_update_sr (:553-747),
+//    the hydrostatic-reconstruction solver 'WB1', in the same style.
 //  * fv1d_run_kernel: the configured scheme for a batch of independent 1D
 //    problems.  A 1D problem of up to 2048 cells fits in one CU's LDS, so one
 //    workgroup owns one instance for the whole run: the state lives in
@@ -168,6 +170,66 @@ __global__ __launch_bounds__(256) void fv1d_ref_update_kernel(const double* __re
   se[2] = -(uec[0] * uec[1] * 1 * (xp - xm) / (2 * delta_s));
   for (int k = 0; k < 3; ++k)
     dudt[3 * i + k] = -((FR[1][k] - FR[0][k]) * oneoverdx) + s[k] + (FE[1][k] - FE[0][k]) * oneoverdx - se[k];
+}
+
+// ---- drop-in: compute_update_sr, benchmark_1d.f90:553-747 ('WB1') ---------------
+// Hydrostatic reconstruction of one cell's face states (:596-629): entropy
+// K = max(p,1e-5)/max(rho,1e-5)^gamma and enthalpy h = gamma/(gamma-1) p/rho are
+// carried from the centre to each face along phi = x, the velocity is the
+// cell's.  Cell i (zero based) has centre (i+0.5)*dx and faces i*dx, (i+1)*dx.
+// The 1e-5 floors are FP32 literals in the reference.  pow is the device
+// library's, so this path agrees with the reference to a few ulps, not bit for
+// bit.
+__device__ inline void sr_states(double gamma, double dx, int i, const double* __restrict__ u, double w[3],
+                                 double wl[3], double wr[3]) {
+  primitive(gamma, &u[3 * i], w);
+  const double e5 = (double)1e-5f;
+  const double h = fmax(w[2], e5) / fmax(e5, w[0]) * (1 + 1.0 / (gamma - 1));
+  const double xc = ((double)(i + 1) - 0.5) * dx;
+  const double h0[2] = {h + xc - (double)i * dx, h + xc - (double)(i + 1) * dx};
+  const double kap = fmax(e5, w[2]) / pow(fmax(e5, w[0]), gamma);
+  const double ik = pow(1.0 / kap, 1 / (gamma - 1));
+  double* o[2] = {wl, wr};
+  for (int s = 0; s < 2; ++s) {
+    o[s][0] = pow((1.0 / kap) * (gamma - 1) / gamma * h0[s], 1 / (gamma - 1));
+    o[s][1] = w[1];
+    o[s][2] = ik * pow((gamma - 1) / gamma * h0[s], gamma / (gamma - 1));
+  }
+}
+
+// One thread per cell.  Interior faces only carry Rusanov fluxes (:637-665: the
+// face loop runs over 2..nx, so its boundary branches never execute), the
+// source is get_source_rg (:409-434) and the boundary cells copy their
+// neighbours (:741-742).
+__global__ __launch_bounds__(256) void fv1d_ref_update_sr_kernel(const double* __restrict__ u,
+                                                                 double* __restrict__ dudt, int nx, double gamma,
+                                                                 double dx, double oneoverdx, double delta) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  const int c = i == 0 ? 1 : (i == nx - 1 ? nx - 2 : i);
+  double w[3], wl[3], wr[3], wa[3], wb[3], t[3];
+  sr_states(gamma, dx, c, u, w, wl, wr);
+  double ul[3], ur[3], fl[3], fr[3];
+  conservative(gamma, wl, ul);
+  conservative(gamma, wr, ur);
+  flux(gamma, ul, fl);
+  flux(gamma, ur, fr);
+  double F0[3], F1[3];
+  {
+    double ua[3], fa[3];
+    sr_states(gamma, dx, c - 1, u, t, wa, wb);   // right face state of cell c-1
+    conservative(gamma, wb, ua);
+    flux(gamma, ua, fa);
+    llflux(gamma, ua, ul, fa, fl, F0);
+    sr_states(gamma, dx, c + 1, u, t, wa, wb);   // left face state of cell c+1
+    conservative(gamma, wa, ua);
+    flux(gamma, ua, fa);
+    llflux(gamma, ur, ua, fr, fa, F1);
+  }
+  const double xm = ((double)c - 0.5) * dx;
+  const double xp = ((double)(c + 2) - 0.5) * dx;
+  const double s[3] = {0.0, (wr[2] - wl[2]) / delta, -(w[0] * w[1] * 1 * (xp - xm) / (2 * delta))};
+  for (int k = 0; k < 3; ++k) dudt[3 * i + k] = -((F1[k] - F0[k]) * oneoverdx) + s[k];
 }
 
 // compute_max_speed (:157-169) over AoS(3,n)
@@ -572,6 +634,22 @@ extern "C" int fvwb1d_compute_update(fvwb1d_handle h, const double* u, const dou
   const double dx = p.boxlen / (double)nx;
   hipLaunchKernelGGL(fv1d_ref_update_kernel, dim3((nx + 255) / 256), dim3(256), 0, h->stream, du, dw, h->d_reff, dd,
                      nx, p.gamma, dx, 1 / dx, 1 / (double)nx);
+  HIPC(h, hipGetLastError());
+  HIPC(h, hipMemcpyAsync(dudt, dd, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPC(h, hipStreamSynchronize(h->stream));
+  return FVWB_OK;
+}
+
+extern "C" int fvwb1d_compute_update_sr(fvwb1d_handle h, const double* u, double* dudt) {
+  if (!h || !u || !dudt) return fail1d(h, FVWB_ERR_ARG, "NULL argument");
+  const fvwb1d_params& p = h->p[0];
+  const int nx = p.nx;
+  const size_t bytes = sizeof(double) * 3 * (size_t)nx;
+  double *du = h->d_stage, *dd = du + 3 * (size_t)nx;
+  HIPC(h, hipMemcpyAsync(du, u, bytes, hipMemcpyHostToDevice, h->stream));
+  const double dx = p.boxlen / (double)nx;
+  hipLaunchKernelGGL(fv1d_ref_update_sr_kernel, dim3((nx + 255) / 256), dim3(256), 0, h->stream, du, dd, nx,
+                     p.gamma, dx, p.boxlen / dx, 1 / (double)nx);
   HIPC(h, hipGetLastError());
   HIPC(h, hipMemcpyAsync(dudt, dd, bytes, hipMemcpyDeviceToHost, h->stream));
   HIPC(h, hipStreamSynchronize(h->stream));

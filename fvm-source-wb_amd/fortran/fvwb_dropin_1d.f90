@@ -53,6 +53,20 @@ subroutine compute_update(u, w_eq, dudt)
   call fvwb1d_check(fvwb1d_compute_update(h, u, w_eq, dudt), h, 'compute_update')
 end subroutine compute_update
 
+! dudt of the hydrostatic-reconstruction residual, solver 'WB1'
+! (benchmark_1d.f90:553-747); w_eq is unused, as in the reference
+subroutine compute_update_sr(u, w_eq, dudt)
+  use parameters, only: nvar, nx
+  use fvwb_iface, only: fvwb1d_compute_update_sr, fvwb1d_check
+  use fvwb_dropin_1d, only: fvwb1d_dropin_handle
+  use iso_c_binding, only: c_ptr
+  implicit none
+  real(kind=8), dimension(1:nvar, 1:nx) :: u, w_eq, dudt
+  type(c_ptr) :: h
+  h = fvwb1d_dropin_handle()
+  call fvwb1d_check(fvwb1d_compute_update_sr(h, u, dudt), h, 'compute_update_sr')
+end subroutine compute_update_sr
+
 ! cmax = max over cells of |v| + c (benchmark_1d.f90:157-169)
 subroutine compute_max_speed(u, cmax)
   use parameters, only: nvar, nx

@@ -179,6 +179,7 @@ def lib():
         "fvwb1d_create": [ctypes.POINTER(H), ctypes.c_int32, p1],
         "fvwb1d_destroy": [H],
         "fvwb1d_compute_update": [H, dp, dp, dp],
+        "fvwb1d_compute_update_sr": [H, dp, dp],
         "fvwb1d_compute_max_speed": [H, dp, dp],
         "fvwb1d_compute_residual": [H, ctypes.c_int32, dp, dp, dp],
         "fvwb1d_upload": [H, ctypes.c_int32, dp, dp],
@@ -510,6 +511,14 @@ class Solver1D:
         u, w_eq = self._arr(u), self._arr(w_eq)
         out = np.zeros((3, self._nx(0)), order="F")
         _check1d(lib().fvwb1d_compute_update(self._h, _dp(u), _dp(w_eq), _dp(out)), self._h)
+        return out
+
+    def compute_update_sr(self, u, w_eq=None):
+        """dudt of compute_update_sr(u, w_eq, dudt) (benchmark_1d.f90:553-747), the
+        hydrostatic-reconstruction solver 'WB1', instance 0; w_eq is unused there."""
+        u = self._arr(u)
+        out = np.zeros((3, self._nx(0)), order="F")
+        _check1d(lib().fvwb1d_compute_update_sr(self._h, _dp(u), _dp(out)), self._h)
         return out
 
     def compute_max_speed(self, u):

@@ -247,6 +247,19 @@ const char* fvwb1d_last_error(fvwb1d_handle h);
  * the two boundary faces, whose fluxes reach no other cell, so it does not
  * change dudt.  Always FP64, rounding statement by statement as the reference. */
 int fvwb1d_compute_update(fvwb1d_handle h, const double* u, const double* w_eq, double* dudt);
+/* Replaces `subroutine compute_update_sr(u, w_eq, dudt)` (benchmark_1d.f90:553-747),
+ * the solver 'WB1' that the reference selects by default (parameters.f90:8), for
+ * instance 0 (w_eq is unused there, so it is not an argument here).  Hydrostatic
+ * reconstruction: each cell's face states keep its entropy max(p,1e-5)/max(rho,
+ * 1e-5)^gamma and its enthalpy plus phi = x (g = 1, as written in the reference's
+ * phi), with the cell's velocity; Rusanov at the interior faces; source
+ * ((p_R - p_L)*nx, -rho*v*(x(i+1)-x(i-1))*nx/2) on momentum and energy
+ * (get_source_rg :409-434); dudt(1) = dudt(2), dudt(nx) = dudt(nx-1).  Uses the
+ * handle's nx, gamma and boxlen only.  FP64 in the reference's statement order;
+ * the device pow makes it agree to a few ulps rather than bit for bit.  An
+ * isentropic rest state rho = (1-(gamma-1)/gamma*x)^(1/(gamma-1)), p = rho^gamma
+ * gives dudt at rounding level. */
+int fvwb1d_compute_update_sr(fvwb1d_handle h, const double* u, double* dudt);
 /* Replaces `subroutine compute_max_speed(u, cmax)` (benchmark_1d.f90:157-184)
  * for instance 0: cmax = max |v| + sqrt(gamma*max(p,1e-10)/max(rho,1e-10)). */
 int fvwb1d_compute_max_speed(fvwb1d_handle h, const double* u, double* cmax);

@@ -100,6 +100,8 @@ void   orc_ref1d_equilibrium(double gamma, int nequilibrium, int n, const double
  * (1 periodic, 2 zero gradient, 3 reflexive) and nequilibrium */
 void   orc_ref1d_update(const fvwb1d_params* p, const double* u, const double* w_eq, double* dudt);
 /* compute_max_speed (:157-169) */
+/* :553-747 compute_update_sr, the 'WB1' hydrostatic reconstruction */
+void   orc_ref1d_update_sr(const fvwb1d_params* p, const double* u, double* dudt);
 double orc_ref1d_max_speed(const fvwb1d_params* p, const double* u);
 
 /* ---- 1D configured scheme (deviation form, plain FP64) -------------------- */

@@ -186,6 +186,70 @@ void orc_ref1d_update(const fvwb1d_params* p, const double* u, const double* w_e
 }
 
 /* benchmark_1d.f90:157-169 compute_max_speed */
+/* compute_update_sr (benchmark_1d.f90:553-747), the 'WB1' solver the reference
+ * ships (parameters.f90:8): hydrostatic reconstruction of each cell's face states
+ * from its entropy K = p/rho^gamma and enthalpy h = gamma/(gamma-1) p/rho, moved
+ * along phi = x (:596-629); Rusanov between the reconstructed states at the
+ * interior faces (:637-665; the loop runs over faces 2..nx only, so the boundary
+ * branches inside it never execute); the source of get_source_rg (:409-434); the
+ * boundary cells copy their neighbours (:741-742).  Constants as written: the
+ * 1e-5 floors are default-real literals, oneoverdx = boxlen/dx (:573). */
+void orc_ref1d_update_sr(const fvwb1d_params* p, const double* u, double* dudt)
+{
+  const int nx = p->nx;
+  const double gamma = p->gamma;
+  const double dx = p->boxlen / (double)nx;
+  const double oneoverdx = p->boxlen / dx;
+  const double e5 = (double)1e-5f;
+  double* wl = malloc(sizeof(double) * 3 * nx);
+  double* wr = malloc(sizeof(double) * 3 * nx);
+  double* ul = malloc(sizeof(double) * 3 * nx);
+  double* ur = malloc(sizeof(double) * 3 * nx);
+  double* fl = malloc(sizeof(double) * 3 * nx);
+  double* fr = malloc(sizeof(double) * 3 * nx);
+  double* w = malloc(sizeof(double) * 3 * nx);
+  double* x = malloc(sizeof(double) * nx);
+  double* F = malloc(sizeof(double) * 3 * (nx + 1));
+  for (int i = 0; i < nx; ++i) {
+    x[i] = ((double)(i + 1) - 0.5) * dx;                      /* (i-0.5)*dx, exact in FP32 */
+    orc_ref1d_primitive(gamma, &u[3 * i], &w[3 * i]);
+    const double rho = w[3 * i], pr = w[3 * i + 2];
+    const double h = fmax(pr, e5) / fmax(e5, rho) * (1 + 1.0 / (gamma - 1));
+    const double h0l = h + x[i] - (double)i * dx;             /* phi = x at the centre and faces */
+    const double h0r = h + x[i] - (double)(i + 1) * dx;
+    const double kap = fmax(e5, pr) / pow(fmax(e5, rho), gamma);
+    A3(wl, 0, i) = pow((1.0 / kap) * (gamma - 1) / gamma * h0l, 1 / (gamma - 1));
+    A3(wl, 1, i) = w[3 * i + 1];
+    A3(wl, 2, i) = pow(1.0 / kap, 1 / (gamma - 1)) * pow((gamma - 1) / gamma * h0l, gamma / (gamma - 1));
+    A3(wr, 0, i) = pow((1.0 / kap) * (gamma - 1) / gamma * h0r, 1 / (gamma - 1));
+    A3(wr, 1, i) = w[3 * i + 1];
+    A3(wr, 2, i) = pow(1.0 / kap, 1 / (gamma - 1)) * pow((gamma - 1) / gamma * h0r, gamma / (gamma - 1));
+    orc_ref1d_conservative(gamma, &wl[3 * i], &ul[3 * i]);
+    orc_ref1d_conservative(gamma, &wr[3 * i], &ur[3 * i]);
+    orc_ref1d_flux(gamma, &ul[3 * i], &fl[3 * i]);
+    orc_ref1d_flux(gamma, &ur[3 * i], &fr[3 * i]);
+  }
+  /* interior faces: zero-based face f between cells f-1 and f */
+  for (int f = 1; f < nx; ++f)
+    orc_ref1d_llflux(gamma, &ur[3 * (f - 1)], &ul[3 * f], &fr[3 * (f - 1)], &fl[3 * f], &F[3 * f]);
+  /* get_source_rg with delta = 1/nx and x extended by delta at both ends */
+  const double delta = 1.0 / (double)nx;
+  for (int i = 1; i < nx - 1; ++i) {
+    const double xm = i == 0 ? x[0] - delta : x[i - 1];
+    const double xp = i == nx - 1 ? x[nx - 1] + delta : x[i + 1];
+    const double s1 = (A3(wr, 2, i) - A3(wl, 2, i)) / delta;
+    const double s2 = -(w[3 * i] * w[3 * i + 1] * 1 * (xp - xm) / (2 * delta));
+    dudt[3 * i + 0] = -(F[3 * (i + 1) + 0] - F[3 * i + 0]) * oneoverdx + 0.0;
+    dudt[3 * i + 1] = -(F[3 * (i + 1) + 1] - F[3 * i + 1]) * oneoverdx + s1;
+    dudt[3 * i + 2] = -(F[3 * (i + 1) + 2] - F[3 * i + 2]) * oneoverdx + s2;
+  }
+  for (int k = 0; k < 3; ++k) {
+    dudt[k] = dudt[3 + k];
+    dudt[3 * (nx - 1) + k] = dudt[3 * (nx - 2) + k];
+  }
+  free(wl); free(wr); free(ul); free(ur); free(fl); free(fr); free(w); free(x); free(F);
+}
+
 double orc_ref1d_max_speed(const fvwb1d_params* p, const double* u)
 {
   double cmax = 0.0;

@@ -132,6 +132,7 @@ def lib():
         L.orc_ref_speed.restype = ctypes.c_double
         p1 = ctypes.POINTER(Params1D)
         L.orc_ref1d_update.argtypes = [p1, dp, dp, dp]
+        L.orc_ref1d_update_sr.argtypes = [p1, dp, dp]
         L.orc_ref1d_max_speed.argtypes = [p1, dp]
         L.orc_ref1d_max_speed.restype = ctypes.c_double
         L.orc_ref1d_equilibrium.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp]
@@ -345,6 +346,14 @@ def ref1d_update(p: Params1D, u, w_eq):
     u, w_eq = fortran(u), fortran(w_eq)
     out = np.zeros_like(u, order="F")
     lib().orc_ref1d_update(ctypes.byref(p), _p(u), _p(w_eq), _p(out))
+    return out
+
+
+def ref1d_update_sr(p: Params1D, u):
+    """compute_update_sr (benchmark_1d.f90:553-747), the 'WB1' residual."""
+    u = fortran(u)
+    out = np.zeros_like(u, order="F")
+    lib().orc_ref1d_update_sr(ctypes.byref(p), _p(u), _p(out))
     return out
 
 

@@ -68,6 +68,32 @@ def summarize(a):
     }
 
 
+def sr_fixture():
+    """compute_update_sr (benchmark_1d.f90:553-747, the 'WB1' solver) at the
+    reference's nx = 128 on the perturbed state, on its isentropic equilibrium
+    (nequilibrium = 2, isothermal, which it does not preserve), on the isentropic
+    rest state of ninit = 3 (which the hydrostatic reconstruction preserves) and on
+    a state far from both (a jump with flow), so every term is exercised."""
+    assert rr.available("bench1d"), "build oracle/_ref first"
+    p1, u1, w1 = bench1d_inputs()
+    ue = np.asfortranarray(po.conservative_1d(p1.gamma, w1))
+    x = (np.arange(p1.nx) + 0.5) / p1.nx
+    wj = np.zeros((3, p1.nx), order="F")
+    wj[0] = np.where(x < 0.5, 1.0, 0.125) * (1 + 0.1 * np.sin(7 * x))
+    wj[1] = 0.3 * np.cos(3 * x)
+    wj[2] = np.where(x < 0.5, 1.0, 0.1)
+    uj = np.asfortranarray(po.conservative_1d(p1.gamma, wj))
+    g = p1.gamma
+    wi = np.zeros((3, p1.nx), order="F")
+    wi[0] = (1 - ((g - 1) / g) * x) ** (1 / (g - 1))
+    wi[2] = (1 - ((g - 1) / g) * x) ** (g / (g - 1))
+    ui = np.asfortranarray(po.conservative_1d(g, wi))
+    return {"u_isen": ui, "dudt_isen": rr.run("bench1d_update_sr", u=ui, w_eq=w1)["dudt"],
+            "u": u1, "w_eq": w1, "dudt": rr.run("bench1d_update_sr", u=u1, w_eq=w1)["dudt"],
+            "u_eq": ue, "dudt_eq": rr.run("bench1d_update_sr", u=ue, w_eq=w1)["dudt"],
+            "u_jump": uj, "dudt_jump": rr.run("bench1d_update_sr", u=uj, w_eq=w1)["dudt"]}
+
+
 def hllc_fixture():
     """riemann_hllc (dg_with_source.f90:1318-1374) on 1D state pairs chosen to
     reach all four sampled regions (supersonic either way, star states either
@@ -102,12 +128,10 @@ def plain_fixture():
 def main(only=None):
     assert rr.available("bench2d") and rr.available("dg2d"), "build oracle/_ref first"
     out = {}
-    if only in ("dg1d_riemann_hllc", "bench2d_update_plain_300"):
-        if only == "dg1d_riemann_hllc":
-            assert rr.available("dg1d"), "build oracle/_ref first"
-            out["dg1d_riemann_hllc"] = hllc_fixture()
-        else:
-            out["bench2d_update_plain_300"] = plain_fixture()
+    single = {"dg1d_riemann_hllc": hllc_fixture, "bench2d_update_plain_300": plain_fixture,
+              "bench1d_update_sr_128": sr_fixture}
+    if only in single:
+        out[only] = single[only]()
         for name, arrs in out.items():
             path = os.path.join(HERE, name + ".npz")
             np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrs.items()})
@@ -183,6 +207,9 @@ def main(only=None):
     fr1 = np.array([po.ref1d_flux(p1.gamma, x) for x in ur])
     f1 = rr.run("bench1d_llflux", ul=ul, ur=ur, fl=fl1, fr=fr1)["f"]
     out["bench1d_llflux"] = {"ul": ul, "ur": ur, "fl": fl1, "fr": fr1, "f": f1, "gamma": np.array(p1.gamma)}
+
+    # compute_update_sr, the 'WB1' hydrostatic reconstruction
+    out["bench1d_update_sr_128"] = sr_fixture()
 
     # A9 compute_update, the plain non-WB residual
     out["bench2d_update_plain_300"] = plain_fixture()

@@ -96,6 +96,11 @@ def _child(op, fin, fout):
             d = np.zeros_like(u, order="F")
             L.compute_update_(_dp(u), _dp(w), _dp(d))
             out["dudt"] = d
+        elif op == "bench1d_update_sr":
+            u = np.asfortranarray(z["u"]); w = np.asfortranarray(z["w_eq"])
+            d = np.zeros_like(u, order="F")
+            L.compute_update_sr_(_dp(u), _dp(w), _dp(d))
+            out["dudt"] = d
         elif op == "bench1d_max_speed":
             u = np.asfortranarray(z["u"])
             c = np.zeros(1)

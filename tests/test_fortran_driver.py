@@ -87,8 +87,9 @@ def test_1d_driver_binds_dropins_to_library():
     nm = subprocess.run(["nm", exe("benchmark_1d_fvwb")], capture_output=True, text=True, check=True).stdout
     defined = set(re.findall(r"^[0-9a-f]+ T (\S+)$", nm, re.M))
     undefined = set(re.findall(r"^\s+U (\S+)$", nm, re.M))
-    assert {"compute_update_", "compute_max_speed_"} <= defined
-    for sym in ("fvwb1d_compute_update", "fvwb1d_compute_max_speed", "fvwb1d_create", "fvwb1d_advance"):
+    assert {"compute_update_", "compute_update_sr_", "compute_max_speed_"} <= defined
+    for sym in ("fvwb1d_compute_update", "fvwb1d_compute_update_sr", "fvwb1d_compute_max_speed",
+                "fvwb1d_create", "fvwb1d_advance"):
         assert sym in undefined and sym in fvwb.declared_symbols()
 
 
@@ -206,6 +207,33 @@ def test_1d_driver_stage_loop_matches_oracle_bitwise(tmp_path):
     print(f"steps {steps} vs {n}, t {t} vs {to}, max|du| {np.max(np.abs(u - uo)):.3e}", flush=True)
     assert steps == n and t == to
     assert np.array_equal(u, uo)
+
+
+@pytest.mark.gpu
+def test_1d_driver_wb1_loop_matches_oracle(tmp_path):
+    """nx=64, ninit=2, the 'WB1' branch of evolve (benchmark_1d.f90:240-246):
+    the Heun loop over the drop-in compute_update_sr follows the same loop over
+    the oracle's restatement with the same step count; the device pow rounds
+    differently now and then, so the states agree to 1e-12, not bit for bit."""
+    tend = 0.05
+    prefix = str(tmp_path / "w1")
+    r = run([exe("benchmark_1d_fvwb_64"), "wb1", repr(tend), prefix])
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps, t, u0, w_eq, u = read_state_1d(prefix + ".bin")
+    p = po.make_params_1d(64)
+    dx = 1.0 / 64
+    uo, to, n = np.asfortranarray(u0.copy()), 0.0, 0
+    while to < tend:
+        cmax = po.ref1d_max_speed(p, uo)
+        dt = float(np.float32(0.8)) * dx / cmax / (2.0 * 1.0 + 1.0)
+        d = po.ref1d_update_sr(p, uo)
+        w1 = uo + dt * d
+        d = po.ref1d_update_sr(p, w1)
+        uo = 0.5 * uo + 0.5 * w1 + 0.5 * dt * d
+        to, n = to + dt, n + 1
+    print(f"steps {steps} vs {n}, t {t} vs {to}, max|du| {np.max(np.abs(u - uo)):.3e}", flush=True)
+    assert steps == n and abs(t - to) <= 1e-14
+    assert np.max(np.abs(u - uo)) <= 1e-12
 
 
 @pytest.mark.gpu

@@ -1,7 +1,9 @@
 """1D HIP path (through the C ABI) against the oracle and the reference
-fixtures.  Both device paths round statement by statement like the oracle (no
-contraction, IEEE division and square root), so every comparison here is bit
-for bit: the drop-in compute_update / compute_max_speed of benchmark_1d.f90,
+fixtures.  The device paths round statement by statement like the oracle (no
+contraction, IEEE division and square root), so the comparisons are bit for bit
+except for compute_update_sr, whose pow comes from the device library (bounded
+by a few ulps of the face fluxes times nx): the drop-in compute_update /
+compute_max_speed of benchmark_1d.f90,
 the configured residual, and whole runs of the LDS-resident batch kernel,
 including the 1D gravity workload (256 cells, MUSCL-minmod, Rusanov, SSP-RK2,
 walls, CFL 0.5, pulse 1e-3 at x=0.5 of width 0.05, t_end = 0.25)."""
@@ -62,6 +64,59 @@ def test_compute_update_matches_oracle_bitwise(nx, neq):
     with fvwb.Solver1D(fvwb.reference_params_1d(nx, neq)) as s:
         same(s.compute_update(u, w), po.ref1d_update(op, u, w), "dudt")
         assert s.compute_max_speed(u) == po.ref1d_max_speed(op, u)
+
+
+# ---- compute_update_sr, the 'WB1' hydrostatic reconstruction ---------------------------
+def sr_bound(u, gamma):
+    """The device pow and the host's differ in the last place now and then;
+    one ulp in a face state moves a flux by about ulp(|u| (|v|+c) + p), and the
+    residual divides four of those by dx: the bound is 16 such ulps times nx."""
+    w = np.stack([u[0], u[1] / u[0], (gamma - 1) * (u[2] - 0.5 * u[1] ** 2 / u[0])])
+    c = np.sqrt(gamma * np.abs(w[2] / w[0]))
+    f = np.max(np.sum(np.abs(u), axis=0) * (np.abs(w[1]) + c) + np.abs(w[2]))
+    return 16 * np.finfo(float).eps * u.shape[1] * f
+
+
+def close_sr(a, b, u, gamma, what):
+    tol = sr_bound(u, gamma)
+    err = np.max(np.abs(np.asarray(a) - np.asarray(b)))
+    say(f"{what}: max|diff| {err:.3e} (bound {tol:.3e}), bitwise {np.mean(np.asarray(a) == b):.3f}")
+    assert err <= tol, what
+
+
+def test_compute_update_sr_matches_reference_golden():
+    say("case: 1D compute_update_sr 128 cells, reference fixture")
+    g = load("bench1d_update_sr_128")
+    p = po.make_params_1d(128)
+    with fvwb.Solver1D(fvwb.reference_params_1d(128, 2)) as s:
+        for k in ("", "_eq", "_isen", "_jump"):
+            close_sr(s.compute_update_sr(g["u" + k]), g["dudt" + k], g["u" + k], p.gamma, "dudt" + k)
+        assert np.max(np.abs(s.compute_update_sr(g["u_isen"]))) <= 64 * np.finfo(float).eps * 128
+
+
+@pytest.mark.parametrize("nx", [4, 37, 1000, 5000])
+def test_compute_update_sr_matches_oracle(nx):
+    say(f"case: 1D compute_update_sr nx={nx}")
+    op = po.make_params_1d(nx)
+    u, _ = po.perturbed_state_1d(op, amp_bump=1e-2, amp_noise=1e-3, salt=nx + 1)
+    u[1] += 0.05 * np.sin(np.arange(nx))
+    with fvwb.Solver1D(fvwb.reference_params_1d(nx, 2)) as s:
+        close_sr(s.compute_update_sr(u), po.ref1d_update_sr(op, u), u, op.gamma, "dudt")
+
+
+def test_compute_update_sr_keeps_isentropic_rest():
+    """p = rho^gamma at rest under phi = x: dudt at rounding level on the device too."""
+    nx = 2048
+    op = po.make_params_1d(nx)
+    x = (np.arange(nx) + 0.5) / nx
+    w = np.zeros((3, nx), order="F")
+    w[0] = (1 - ((op.gamma - 1) / op.gamma) * x) ** (1 / (op.gamma - 1))
+    w[2] = w[0] ** op.gamma
+    u = po.conservative_1d(op.gamma, w)
+    with fvwb.Solver1D(fvwb.reference_params_1d(nx, 2)) as s:
+        d = s.compute_update_sr(u)
+    say(f"rest: max|dudt| {np.max(np.abs(d)):.3e}")
+    assert np.max(np.abs(d)) <= 64 * np.finfo(float).eps * nx
 
 
 SCHEMES = [(r, bc, i) for r in (0, 1) for bc in (po.BC1D_ZERO_GRADIENT, po.BC1D_WALL) for i in (0, 1)]

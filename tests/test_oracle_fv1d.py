@@ -1,4 +1,5 @@
-"""The 1D CPU oracle: the benchmark_1d.f90 restatement against the fixtures
+"""The 1D CPU oracle: the benchmark_1d.f90 restatements ('EQL' compute_update
+and 'WB1' compute_update_sr) against the fixtures
 written from the flang-built reference (bit for bit) and live against
 oracle/_ref when present; the configured 1D scheme against what the
 mathematics fixes (exact rest, flux consistency, wall symmetry and mass
@@ -33,6 +34,60 @@ def test_ref1d_update_matches_reference_golden_bitwise():
     ue = po.conservative_1d(p.gamma, w)
     np.testing.assert_array_equal(po.ref1d_update(p, ue, w), g["dudt_eq"])
     assert np.count_nonzero(g["dudt_eq"]) == 0
+
+
+def test_ref1d_update_sr_matches_reference_golden_bitwise():
+    """compute_update_sr (benchmark_1d.f90:553-747, the 'WB1' solver) on the
+    perturbed state, the isothermal rest state, the isentropic rest state and a
+    jump with flow: bit for bit (the oracle calls the same libm pow)."""
+    g = load("bench1d_update_sr_128")
+    p, u, _ = bench1d_inputs()
+    np.testing.assert_array_equal(u, g["u"])
+    for k in ("", "_eq", "_isen", "_jump"):
+        np.testing.assert_array_equal(po.ref1d_update_sr(p, g["u" + k]), g["dudt" + k], err_msg=k)
+
+
+def isentropic_rest(p):
+    x = (np.arange(p.nx) + 0.5) * (p.boxlen / p.nx)
+    w = np.zeros((3, p.nx), order="F")
+    w[0] = (1 - ((p.gamma - 1) / p.gamma) * x) ** (1 / (p.gamma - 1))
+    w[2] = w[0] ** p.gamma
+    return po.conservative_1d(p.gamma, w)
+
+
+@pytest.mark.parametrize("nx", [16, 128, 1000])
+def test_ref1d_update_sr_keeps_isentropic_rest(nx):
+    """The hydrostatic reconstruction is exact for p = rho^gamma at rest under
+    phi = x: every face state equals the equilibrium there, the Rusanov fluxes
+    reduce to the face pressure and the pressure source cancels their
+    difference, so dudt is rounding only (a few ulps of p, times nx)."""
+    p = po.make_params_1d(nx)
+    d = po.ref1d_update_sr(p, isentropic_rest(p))
+    assert np.max(np.abs(d)) <= 64 * np.finfo(float).eps * nx
+
+
+def test_ref1d_update_sr_converges_to_the_euler_gravity_operator():
+    """On a smooth moving state the residual tends to -(F(U))_x + S with
+    S = (0, -rho, -rho v) (phi = x), at first order (the Rusanov diffusion);
+    a dropped source term or a wrong sign leaves an O(1) error instead."""
+    errs = []
+    for nx in (100, 200, 400):
+        p = po.make_params_1d(nx)
+        gm = p.gamma
+        x = (np.arange(nx) + 0.5) / nx
+        k = 2 * np.pi
+        r, rx = 1 + 0.2 * np.sin(k * x), 0.2 * k * np.cos(k * x)
+        v, vx = 0.3 * np.cos(k * x), -0.3 * k * np.sin(k * x)
+        q, qx = 1 + 0.1 * np.cos(k * x), -0.1 * k * np.sin(k * x)
+        w = np.asfortranarray(np.stack([r, v, q]))
+        d = po.ref1d_update_sr(p, po.conservative_1d(gm, w))
+        E, Ex = q / (gm - 1) + 0.5 * r * v * v, qx / (gm - 1) + 0.5 * (rx * v * v + 2 * r * v * vx)
+        exact = np.stack([-(rx * v + r * vx),
+                          -(rx * v * v + 2 * r * v * vx + qx) - r,
+                          -((Ex + qx) * v + (E + q) * vx) - r * v])
+        errs.append(np.max(np.abs(d - exact)[:, 2:-2]))
+    assert errs[0] < 0.2
+    assert errs[0] / errs[1] > 1.8 and errs[1] / errs[2] > 1.8, errs
 
 
 def test_ref1d_llflux_matches_reference_golden_bitwise():
