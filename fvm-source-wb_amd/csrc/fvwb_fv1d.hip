@@ -6,7 +6,9 @@
 //    statement (no contraction, IEEE division and square root), so it returns
 //    the reference's dudt bit for bit given the same face equilibrium table.
 //  * fv1d_ref_update_sr_kernel: the drop-in for compute_update_sr (:553-747),
-//    the hydrostatic-reconstruction solver 'WB1', in the same style.
+//    the hydrostatic-reconstruction solver 'WB1', in the same style;
+//  * fv1d_plain_update_kernel: the plain residuals, compute_update_fvm
+//    (:454-549, solver 'FVM') and compute_update of the stand-alone fvm.f90.
 //  * fv1d_run_kernel: the configured scheme for a batch of independent 1D
 //    problems.  A 1D problem of up to 2048 cells fits in one CU's LDS, so one
 //    workgroup owns one instance for the whole run: the state lives in
@@ -230,6 +232,61 @@ __global__ __launch_bounds__(256) void fv1d_ref_update_sr_kernel(const double* _
   const double xp = ((double)(c + 2) - 0.5) * dx;
   const double s[3] = {0.0, (wr[2] - wl[2]) / delta, -(w[0] * w[1] * 1 * (xp - xm) / (2 * delta))};
   for (int k = 0; k < 3; ++k) dudt[3 * i + k] = -((F1[k] - F0[k]) * oneoverdx) + s[k];
+}
+
+// ---- drop-ins: the plain (not well-balanced) 1D residuals -----------------------
+// PROGRAM == false: compute_update_fvm (benchmark_1d.f90:454-549, solver 'FVM'):
+//   Rusanov between the cell states, get_source (:379-405) on the conservative
+//   state with delta = 1/nx, boundary cells copy their neighbours (:546-547).
+// PROGRAM == true: compute_update of the stand-alone fvm.f90 (:188-251): the
+//   end cells' outer neighbours wrap (periodic) or repeat the cell (zero
+//   gradient), the optional source uses the primitive state with x- and x+
+//   clamped to the cell's own centre at the two ends (:232-243) and delta = dx;
+//   no boundary copy.
+// One thread per cell, both faces of the cell computed by that thread.
+template <bool PROGRAM>
+__global__ __launch_bounds__(256) void fv1d_plain_update_kernel(const double* __restrict__ u,
+                                                                double* __restrict__ dudt, int nx, double gamma,
+                                                                double dx, double oneoverdx, double delta,
+                                                                int periodic, int source) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  int c = i, im = i - 1, ip = i + 1;
+  if (!PROGRAM) {
+    c = i == 0 ? 1 : (i == nx - 1 ? nx - 2 : i);
+    im = c - 1;
+    ip = c + 1;
+  } else {
+    if (i == 0) im = periodic ? nx - 1 : 0;
+    if (i == nx - 1) ip = periodic ? 0 : nx - 1;
+  }
+  const double* um = &u[3 * im];
+  const double* uc = &u[3 * c];
+  const double* up = &u[3 * ip];
+  double fm[3], fc[3], fp[3], FL[3], FR[3];
+  flux(gamma, um, fm);
+  flux(gamma, uc, fc);
+  flux(gamma, up, fp);
+  llflux(gamma, um, uc, fm, fc, FL);
+  llflux(gamma, uc, up, fc, fp, FR);
+  double s[3] = {0.0, 0.0, 0.0};
+  if (!PROGRAM) {
+    const double xm = ((double)c - 0.5) * dx;
+    const double xp = ((double)(c + 2) - 0.5) * dx;
+    s[1] = -(uc[0] * 1 * (xp - xm) / (2 * delta));
+    s[2] = -(uc[0] * uc[1] * 1 * (xp - xm) / (2 * delta));
+    for (int k = 0; k < 3; ++k) dudt[3 * i + k] = -((FR[k] - FL[k]) * oneoverdx) + s[k];
+  } else {
+    if (source == 2) {
+      const double xm = (i == 0 ? 0.5 : (double)i - 0.5) * dx;
+      const double xp = (i == nx - 1 ? (double)nx - 0.5 : (double)(i + 2) - 0.5) * dx;
+      double w[3];
+      primitive(gamma, uc, w);
+      s[1] = -(w[0] * 1 * (xp - xm) / (2 * delta));
+      s[2] = -(w[0] * w[1] * 1 * (xp - xm) / (2 * delta));
+    }
+    for (int k = 0; k < 3; ++k) dudt[3 * i + k] = -(oneoverdx * (FR[k] - FL[k])) + s[k];
+  }
 }
 
 // compute_max_speed (:157-169) over AoS(3,n)
@@ -654,6 +711,40 @@ extern "C" int fvwb1d_compute_update_sr(fvwb1d_handle h, const double* u, double
   HIPC(h, hipMemcpyAsync(dudt, dd, bytes, hipMemcpyDeviceToHost, h->stream));
   HIPC(h, hipStreamSynchronize(h->stream));
   return FVWB_OK;
+}
+
+static int plain_update(fvwb1d* h, bool program, int source, const double* u, double* dudt) {
+  const fvwb1d_params& p = h->p[0];
+  const int nx = p.nx;
+  const size_t bytes = sizeof(double) * 3 * (size_t)nx;
+  double *du = h->d_stage, *dd = du + 3 * (size_t)nx;
+  HIPC(h, hipMemcpyAsync(du, u, bytes, hipMemcpyHostToDevice, h->stream));
+  const double dx = p.boxlen / (double)nx;
+  const dim3 grid((nx + 255) / 256), block(256);
+  if (program)
+    hipLaunchKernelGGL(fv1d_plain_update_kernel<true>, grid, block, 0, h->stream, du, dd, nx, p.gamma, dx, 1.0 / dx,
+                       dx, p.bc == FVWB1D_BC_PERIODIC ? 1 : 0, source);
+  else
+    hipLaunchKernelGGL(fv1d_plain_update_kernel<false>, grid, block, 0, h->stream, du, dd, nx, p.gamma, dx, 1 / dx,
+                       1 / (double)nx, 0, 2);
+  HIPC(h, hipGetLastError());
+  HIPC(h, hipMemcpyAsync(dudt, dd, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPC(h, hipStreamSynchronize(h->stream));
+  return FVWB_OK;
+}
+
+extern "C" int fvwb1d_compute_update_fvm(fvwb1d_handle h, const double* u, double* dudt) {
+  if (!h || !u || !dudt) return fail1d(h, FVWB_ERR_ARG, "NULL argument");
+  return plain_update(h, false, 2, u, dudt);
+}
+
+extern "C" int fvwb1d_fvm_compute_update(fvwb1d_handle h, int32_t source, const double* u, double* dudt) {
+  if (!h || !u || !dudt) return fail1d(h, FVWB_ERR_ARG, "NULL argument");
+  if (source != 1 && source != 2) return fail1d(h, FVWB_ERR_ARG, "source must be 1 (none) or 2 (gravity)");
+  const int bc = h->p[0].bc;
+  if (bc != FVWB1D_BC_PERIODIC && bc != FVWB1D_BC_ZERO_GRADIENT)
+    return fail1d(h, FVWB_ERR_ARG, "fvm.f90's compute_update handles bc 1 (periodic) and 2 (zero gradient) only");
+  return plain_update(h, true, source, u, dudt);
 }
 
 extern "C" int fvwb1d_compute_max_speed(fvwb1d_handle h, const double* u, double* cmax) {

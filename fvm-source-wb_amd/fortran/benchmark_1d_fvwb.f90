@@ -9,12 +9,14 @@
 ! which makes that branch forward Euler; this driver passes w1 (Heun, as the
 ! 2D driver's evolve).
 !
-! usage: benchmark_1d_fvwb [stage|wb1|resident] [tend] [prefix]
+! usage: benchmark_1d_fvwb [stage|wb1|fvm|resident] [tend] [prefix]
 !   stage     the loop above on the host; every residual and wave-speed
 !             evaluation is the drop-in compute_update / compute_max_speed
 !   wb1       the same loop with evolve's 'WB1' branch: the residual is the
 !             drop-in compute_update_sr (hydrostatic reconstruction), whose
 !             second call takes w1 in the reference too
+!   fvm       the same loop with the plain residual of the 'FVM' branch, the
+!             drop-in compute_update_fvm (second call on w1, as in stage)
 !   resident  the configured device scheme instead: MUSCL-minmod, Rusanov,
 !             SSP-RK2, solid walls, dt = 0.5*dx/cmax, the last step clipped to
 !             tend, the whole run in one launch (fvwb1d_advance)
@@ -54,13 +56,15 @@ program benchmark_1d_fvwb
 
   select case (trim(mode))
   case ('stage')
-     call evolve_stages(u, w_eq, t, iter, .false.)
+     call evolve_stages(u, w_eq, t, iter, 'EQL')
   case ('wb1')
-     call evolve_stages(u, w_eq, t, iter, .true.)
+     call evolve_stages(u, w_eq, t, iter, 'WB1')
+  case ('fvm')
+     call evolve_stages(u, w_eq, t, iter, 'FVM')
   case ('resident')
      call evolve_resident(u, w_eq, t, iter)
   case default
-     error stop 'benchmark_1d_fvwb: mode must be stage, wb1 or resident'
+     error stop 'benchmark_1d_fvwb: mode must be stage, wb1, fvm or resident'
   end select
   write(*, '(a,a,a,i0,a,i0,a,1pe24.16)') 'benchmark_1d_fvwb mode=', trim(mode), ' nx=', nx, ' steps=', iter, &
        ' t=', t
@@ -124,12 +128,12 @@ contains
     u(3, :) = w(3, :) / (gamma - 1.0) + 0.5 * w(1, :) * w(2, :)**2
   end subroutine initial_state
 
-  subroutine evolve_stages(u, w_eq, t, iter, wb1)
+  subroutine evolve_stages(u, w_eq, t, iter, which)
     real(kind=8), intent(inout) :: u(nvar, nx)
     real(kind=8), intent(in) :: w_eq(nvar, nx)
     real(kind=8), intent(out) :: t
     integer, intent(out) :: iter
-    logical, intent(in) :: wb1
+    character(len=3), intent(in) :: which
     real(kind=8), allocatable :: dudt(:,:), w1(:,:)
     real(kind=8) :: dt, cmax
     allocate(dudt(nvar, nx), w1(nvar, nx))
@@ -138,10 +142,14 @@ contains
     do while (t < tend)
        call compute_max_speed(u, cmax)
        dt = 0.8 * dx / cmax / (2.0 * dble(1) + 1.0)
-       if (wb1) then
+       if (which == 'WB1') then
           call compute_update_sr(u, w_eq, dudt)
           w1 = u + dt * dudt
           call compute_update_sr(w1, w_eq, dudt)
+       else if (which == 'FVM') then
+          call compute_update_fvm(u, w_eq, dudt)
+          w1 = u + dt * dudt
+          call compute_update_fvm(w1, w_eq, dudt)
        else
           call compute_update(u, w_eq, dudt)
           w1 = u + dt * dudt

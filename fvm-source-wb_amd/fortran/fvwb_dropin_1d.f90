@@ -1,7 +1,8 @@
 ! Drop-in replacements for the reference's 1D per-stage routines.
 !
-! compute_update(u, w_eq, dudt) and compute_max_speed(u, cmax) keep the external
-! signatures of benchmark_1d.f90:263-266 and :157-162 (implicit interface,
+! compute_update(u, w_eq, dudt), compute_update_sr(u, w_eq, dudt),
+! compute_update_fvm(u, w_eq, dudt) and compute_max_speed(u, cmax) keep the external
+! signatures of benchmark_1d.f90:263-266, :553-556, :454-457 and :157-162 (implicit interface,
 ! explicit-shape real(kind=8) arrays (1:nvar,1:nx), sizes and constants from
 ! module parameters).  The first call creates one library handle from
 ! parameters (nx, nequilibrium, gamma, boxlen, bc); fvwb1d_dropin_close()
@@ -66,6 +67,20 @@ subroutine compute_update_sr(u, w_eq, dudt)
   h = fvwb1d_dropin_handle()
   call fvwb1d_check(fvwb1d_compute_update_sr(h, u, dudt), h, 'compute_update_sr')
 end subroutine compute_update_sr
+
+! dudt of the plain residual, solver 'FVM' (benchmark_1d.f90:454-549); w_eq is
+! unused, as in the reference
+subroutine compute_update_fvm(u, w_eq, dudt)
+  use parameters, only: nvar, nx
+  use fvwb_iface, only: fvwb1d_compute_update_fvm, fvwb1d_check
+  use fvwb_dropin_1d, only: fvwb1d_dropin_handle
+  use iso_c_binding, only: c_ptr
+  implicit none
+  real(kind=8), dimension(1:nvar, 1:nx) :: u, w_eq, dudt
+  type(c_ptr) :: h
+  h = fvwb1d_dropin_handle()
+  call fvwb1d_check(fvwb1d_compute_update_fvm(h, u, dudt), h, 'compute_update_fvm')
+end subroutine compute_update_fvm
 
 ! cmax = max over cells of |v| + c (benchmark_1d.f90:157-169)
 subroutine compute_max_speed(u, cmax)

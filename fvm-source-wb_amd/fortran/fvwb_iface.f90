@@ -46,7 +46,8 @@ module fvwb_iface
   end type fvwb1d_params
 
   public :: fvwb1d_params_reference, fvwb1d_create, fvwb1d_destroy, fvwb1d_last_error
-  public :: fvwb1d_compute_update, fvwb1d_compute_update_sr, fvwb1d_compute_max_speed
+  public :: fvwb1d_compute_update, fvwb1d_compute_update_sr, fvwb1d_compute_update_fvm
+  public :: fvwb1d_fvm_compute_update, fvwb1d_compute_max_speed
   public :: fvwb1d_upload, fvwb1d_init_gaussian
   public :: fvwb1d_advance, fvwb1d_download, fvwb1d_get_time, fvwb1d_check
 
@@ -245,6 +246,24 @@ module fvwb_iface
        real(c_double), intent(in) :: u(*)
        real(c_double), intent(out) :: dudt(*)
      end function fvwb1d_compute_update_sr
+
+     ! drop-in for benchmark_1d.f90:454-549 (compute_update_fvm(u,w_eq,dudt), solver 'FVM')
+     integer(c_int) function fvwb1d_compute_update_fvm(h, u, dudt) bind(C, name='fvwb1d_compute_update_fvm')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*)
+       real(c_double), intent(out) :: dudt(*)
+     end function fvwb1d_compute_update_fvm
+
+     ! drop-in for fvm.f90:188-251 (compute_update(u,dudt) of the stand-alone program)
+     integer(c_int) function fvwb1d_fvm_compute_update(h, source, u, dudt) &
+          bind(C, name='fvwb1d_fvm_compute_update')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: source
+       real(c_double), intent(in) :: u(*)
+       real(c_double), intent(out) :: dudt(*)
+     end function fvwb1d_fvm_compute_update
 
      ! drop-in for benchmark_1d.f90:157-169 (compute_max_speed(u,cmax))
      integer(c_int) function fvwb1d_compute_max_speed(h, u, cmax) bind(C, name='fvwb1d_compute_max_speed')

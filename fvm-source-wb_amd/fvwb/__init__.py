@@ -180,6 +180,8 @@ def lib():
         "fvwb1d_destroy": [H],
         "fvwb1d_compute_update": [H, dp, dp, dp],
         "fvwb1d_compute_update_sr": [H, dp, dp],
+        "fvwb1d_compute_update_fvm": [H, dp, dp],
+        "fvwb1d_fvm_compute_update": [H, ctypes.c_int32, dp, dp],
         "fvwb1d_compute_max_speed": [H, dp, dp],
         "fvwb1d_compute_residual": [H, ctypes.c_int32, dp, dp, dp],
         "fvwb1d_upload": [H, ctypes.c_int32, dp, dp],
@@ -519,6 +521,22 @@ class Solver1D:
         u = self._arr(u)
         out = np.zeros((3, self._nx(0)), order="F")
         _check1d(lib().fvwb1d_compute_update_sr(self._h, _dp(u), _dp(out)), self._h)
+        return out
+
+    def compute_update_fvm(self, u, w_eq=None):
+        """dudt of compute_update_fvm(u, w_eq, dudt) (benchmark_1d.f90:454-549), the
+        plain solver 'FVM', instance 0; w_eq is unused there."""
+        u = self._arr(u)
+        out = np.zeros((3, self._nx(0)), order="F")
+        _check1d(lib().fvwb1d_compute_update_fvm(self._h, _dp(u), _dp(out)), self._h)
+        return out
+
+    def fvm_compute_update(self, u, source=2):
+        """dudt of compute_update(u, dudt) of fvm.f90 (:188-251), instance 0, with
+        the handle's bc (1 or 2) and fvm_commons' source switch (1 none, 2 gravity)."""
+        u = self._arr(u)
+        out = np.zeros((3, self._nx(0)), order="F")
+        _check1d(lib().fvwb1d_fvm_compute_update(self._h, int(source), _dp(u), _dp(out)), self._h)
         return out
 
     def compute_max_speed(self, u):

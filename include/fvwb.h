@@ -260,8 +260,22 @@ int fvwb1d_compute_update(fvwb1d_handle h, const double* u, const double* w_eq, 
  * isentropic rest state rho = (1-(gamma-1)/gamma*x)^(1/(gamma-1)), p = rho^gamma
  * gives dudt at rounding level. */
 int fvwb1d_compute_update_sr(fvwb1d_handle h, const double* u, double* dudt);
-/* Replaces `subroutine compute_max_speed(u, cmax)` (benchmark_1d.f90:157-184)
- * for instance 0: cmax = max |v| + sqrt(gamma*max(p,1e-10)/max(rho,1e-10)). */
+/* Replaces `subroutine compute_update_fvm(u, w_eq, dudt)` (benchmark_1d.f90:454-549),
+ * the plain solver 'FVM', for instance 0 (w_eq is unused there): Rusanov between
+ * the cell states, source -rho*(x(i+1)-x(i-1))*nx/2 on momentum and the same
+ * times the momentum on energy (get_source :379-405 called on the conservative
+ * state), dudt(1) = dudt(2), dudt(nx) = dudt(nx-1).  Bit for bit. */
+int fvwb1d_compute_update_fvm(fvwb1d_handle h, const double* u, double* dudt);
+/* Replaces `subroutine compute_update(u, dudt)` of the stand-alone program fvm.f90
+ * (:188-264), for instance 0: Rusanov between the cell states with the handle's
+ * bc, FVWB1D_BC_PERIODIC or FVWB1D_BC_ZERO_GRADIENT (:200-230; other codes give
+ * FVWB_ERR_ARG, the reference indexes out of bounds there); `source` is
+ * fvm_commons' switch, 1 none, 2 gravity: -rho*(x+ - x-)/(2 dx) on momentum and
+ * -rho*v*(x+ - x-)/(2 dx) on energy, x- and x+ clamped to the cell's centre at the
+ * two ends (:229-236, compute_source :253-264).  Bit for bit. */
+int fvwb1d_fvm_compute_update(fvwb1d_handle h, int32_t source, const double* u, double* dudt);
+/* Replaces `subroutine compute_max_speed(u, cmax)` (benchmark_1d.f90:157-184,
+ * and the identical fvm.f90:320-336) for instance 0: cmax = max |v| + sqrt(gamma*max(p,1e-10)/max(rho,1e-10)). */
 int fvwb1d_compute_max_speed(fvwb1d_handle h, const double* u, double* cmax);
 /* Residual of the configured scheme for instance `inst` (what fvwb1d_advance
  * integrates): deviation form with face equilibrium, p.recon, Rusanov, source

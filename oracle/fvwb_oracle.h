@@ -13,8 +13,9 @@
  *   orc_fv2d_*   — the configured scheme (MUSCL-minmod / HLL / reflecting walls /
  *                  SSP-RK3) in the same plain FP64 form, pinned by invariants
  *                  and closed forms in tests/;
- *   orc_ref1d_*  — restatements of benchmark_1d.f90 (the 'EQL' path), pinned
- *                  like orc_ref2d_*;
+ *   orc_ref1d_*  — restatements of benchmark_1d.f90 (its 'EQL', 'WB1' and 'FVM'
+ *                  residuals), pinned like orc_ref2d_*;
+ *   orc_fvm_*    — the stand-alone 1D program fvm.f90, pinned likewise;
  *   orc_fv1d_*   — the configured 1D scheme (deviation form, MUSCL-minmod,
  *                  Rusanov, SSP-RK2/RK3, walls), pinned by invariants;
  *   orc_dg_*, orc_ref_dg2d_* — the reference's Legendre basis, Gauss-Legendre
@@ -99,10 +100,15 @@ void   orc_ref1d_equilibrium(double gamma, int nequilibrium, int n, const double
 /* compute_update, solver 'EQL' (:263-377), with p->nx, gamma, boxlen, bc code
  * (1 periodic, 2 zero gradient, 3 reflexive) and nequilibrium */
 void   orc_ref1d_update(const fvwb1d_params* p, const double* u, const double* w_eq, double* dudt);
-/* compute_max_speed (:157-169) */
 /* :553-747 compute_update_sr, the 'WB1' hydrostatic reconstruction */
 void   orc_ref1d_update_sr(const fvwb1d_params* p, const double* u, double* dudt);
+/* :454-549 compute_update_fvm, the plain 'FVM' solver */
+void   orc_ref1d_update_fvm(const fvwb1d_params* p, const double* u, double* dudt);
+/* compute_max_speed (:157-169) */
 double orc_ref1d_max_speed(const fvwb1d_params* p, const double* u);
+/* fvm.f90:188-251 compute_update of the stand-alone 1D program (bc 1 or 2 from
+ * p, source 1 none or 2 gravity) */
+void   orc_fvm_update(const fvwb1d_params* p, int source, const double* u, double* dudt);
 
 /* ---- 1D configured scheme (deviation form, plain FP64) -------------------- */
 void   orc_fv1d_equilibrium(const fvwb1d_params* p, double x, double w[3]);

@@ -185,7 +185,6 @@ void orc_ref1d_update(const fvwb1d_params* p, const double* u, const double* w_e
   free(xf); free(xc); free(weqf); free(ueqf); free(feq); free(fr);
 }
 
-/* benchmark_1d.f90:157-169 compute_max_speed */
 /* compute_update_sr (benchmark_1d.f90:553-747), the 'WB1' solver the reference
  * ships (parameters.f90:8): hydrostatic reconstruction of each cell's face states
  * from its entropy K = p/rho^gamma and enthalpy h = gamma/(gamma-1) p/rho, moved
@@ -250,6 +249,79 @@ void orc_ref1d_update_sr(const fvwb1d_params* p, const double* u, double* dudt)
   free(wl); free(wr); free(ul); free(ur); free(fl); free(fr); free(w); free(x); free(F);
 }
 
+/* compute_update_fvm (benchmark_1d.f90:454-549), the plain 'FVM' solver: Rusanov
+ * between the cell states themselves, get_source (:379-405) on the conservative
+ * state, and the boundary cells copy their neighbours (:546-547), so only the
+ * interior faces reach dudt and the bc code does not.  oneoverdx = 1/dx (:470). */
+void orc_ref1d_update_fvm(const fvwb1d_params* p, const double* u, double* dudt)
+{
+  const int nx = p->nx;
+  const double gamma = p->gamma;
+  const double dx = p->boxlen / (double)nx;
+  const double oneoverdx = 1 / dx;
+  const double delta = 1 / (double)nx;
+  double* f = malloc(sizeof(double) * 3 * nx);
+  double* F = malloc(sizeof(double) * 3 * (nx + 1));
+  for (int i = 0; i < nx; ++i) orc_ref1d_flux(gamma, &u[3 * i], &f[3 * i]);
+  for (int k = 1; k < nx; ++k) orc_ref1d_llflux(gamma, &u[3 * (k - 1)], &u[3 * k], &f[3 * (k - 1)], &f[3 * k], &F[3 * k]);
+  for (int i = 1; i < nx - 1; ++i) {
+    const double xm = ((double)i - 0.5) * dx;        /* x(i-1) of the one-based cell i+1 */
+    const double xp = ((double)(i + 2) - 0.5) * dx;
+    const double s[3] = {0.0, -(A3(u, 0, i) * 1 * (xp - xm) / (2 * delta)),
+                         -(A3(u, 0, i) * A3(u, 1, i) * 1 * (xp - xm) / (2 * delta))};
+    for (int k = 0; k < 3; ++k) A3(dudt, k, i) = -((A3(F, k, i + 1) - A3(F, k, i)) * oneoverdx) + s[k];
+  }
+  for (int k = 0; k < 3; ++k) {
+    A3(dudt, k, 0) = A3(dudt, k, 1);
+    A3(dudt, k, nx - 1) = A3(dudt, k, nx - 2);
+  }
+  free(f); free(F);
+}
+
+/* compute_update of the stand-alone program fvm.f90 (:188-251), with its
+ * compute_source (:253-264) and compute_llflux (:271-288): Rusanov between cell
+ * states, each cell computing both of its faces; bc 1 wraps the neighbours of
+ * the end cells, bc 2 repeats the end cell (:200-230); source 2 is
+ * -rho*(x+ - x-)/(2 dx) on momentum and -rho*v*(x+ - x-)/(2 dx) on energy from
+ * the primitive state, with x- and x+ clamped to the cell's own centre at the two
+ * ends (:232-243), source 1 is zero.  No boundary copy: every cell is its own.
+ * oneoverdx = 1.0/dx (:198). */
+void orc_fvm_update(const fvwb1d_params* p, int source, const double* u, double* dudt)
+{
+  const int nx = p->nx;
+  const double gamma = p->gamma;
+  const double dx = p->boxlen / (double)nx;
+  const double oneoverdx = 1.0 / dx;
+  for (int i = 0; i < nx; ++i) {
+    int im = i - 1, ip = i + 1;
+    if (p->bc == FVWB1D_BC_PERIODIC) {
+      if (i == 0) im = nx - 1;
+      if (i == nx - 1) ip = 0;
+    } else {
+      if (i == 0) im = 0;
+      if (i == nx - 1) ip = nx - 1;
+    }
+    double fm[3], fc[3], fp[3], FL[3], FR[3];
+    orc_ref1d_flux(gamma, &u[3 * im], fm);
+    orc_ref1d_flux(gamma, &u[3 * i], fc);
+    orc_ref1d_flux(gamma, &u[3 * ip], fp);
+    orc_ref1d_llflux(gamma, &u[3 * im], &u[3 * i], fm, fc, FL);
+    orc_ref1d_llflux(gamma, &u[3 * i], &u[3 * ip], fc, fp, FR);
+    double s[3] = {0.0, 0.0, 0.0};
+    if (source == 2) {
+      double xm = ((double)(i + 1 - 1) - 0.5) * dx, xp = ((double)(i + 1 + 1) - 0.5) * dx;
+      if (i == 0) xm = ((double)1 - 0.5) * dx;
+      if (i == nx - 1) xp = ((double)nx - 0.5) * dx;
+      double w[3];
+      orc_ref1d_primitive(gamma, &u[3 * i], w);
+      s[1] = -(w[0] * 1 * (xp - xm) / (2 * dx));
+      s[2] = -(w[0] * w[1] * 1 * (xp - xm) / (2 * dx));
+    }
+    for (int k = 0; k < 3; ++k) A3(dudt, k, i) = -(oneoverdx * (FR[k] - FL[k])) + s[k];
+  }
+}
+
+/* benchmark_1d.f90:157-169 compute_max_speed */
 double orc_ref1d_max_speed(const fvwb1d_params* p, const double* u)
 {
   double cmax = 0.0;

@@ -133,6 +133,8 @@ def lib():
         p1 = ctypes.POINTER(Params1D)
         L.orc_ref1d_update.argtypes = [p1, dp, dp, dp]
         L.orc_ref1d_update_sr.argtypes = [p1, dp, dp]
+        L.orc_ref1d_update_fvm.argtypes = [p1, dp, dp]
+        L.orc_fvm_update.argtypes = [p1, ctypes.c_int, dp, dp]
         L.orc_ref1d_max_speed.argtypes = [p1, dp]
         L.orc_ref1d_max_speed.restype = ctypes.c_double
         L.orc_ref1d_equilibrium.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp]
@@ -354,6 +356,22 @@ def ref1d_update_sr(p: Params1D, u):
     u = fortran(u)
     out = np.zeros_like(u, order="F")
     lib().orc_ref1d_update_sr(ctypes.byref(p), _p(u), _p(out))
+    return out
+
+
+def ref1d_update_fvm(p: Params1D, u):
+    """compute_update_fvm (benchmark_1d.f90:454-549), the plain 'FVM' residual."""
+    u = fortran(u)
+    out = np.zeros_like(u, order="F")
+    lib().orc_ref1d_update_fvm(ctypes.byref(p), _p(u), _p(out))
+    return out
+
+
+def fvm_update(p: Params1D, u, source=2):
+    """compute_update of fvm.f90 (:188-251); bc from p (1 or 2), source 1 or 2."""
+    u = fortran(u)
+    out = np.zeros_like(u, order="F")
+    lib().orc_fvm_update(ctypes.byref(p), int(source), _p(u), _p(out))
     return out
 
 

@@ -94,6 +94,41 @@ def sr_fixture():
             "u_jump": uj, "dudt_jump": rr.run("bench1d_update_sr", u=uj, w_eq=w1)["dudt"]}
 
 
+def fvm1d_fixtures():
+    """The two plain 1D residuals: compute_update_fvm (benchmark_1d.f90:454-549,
+    solver 'FVM') at nx = 128 on the perturbed and jump states, and compute_update
+    of fvm.f90 (:188-251) at its nx = 200 (bc 2, source 2) on its own Sod set-up
+    (condinit, ninit = 4), on a smooth moving state and on the polytropic rest
+    state of condinit case 7."""
+    assert rr.available("bench1d") and rr.available("fvm"), "build oracle/_ref first"
+    p1, u1, w1 = bench1d_inputs()
+    x = (np.arange(p1.nx) + 0.5) / p1.nx
+    wj = np.zeros((3, p1.nx), order="F")
+    wj[0] = np.where(x < 0.5, 1.0, 0.125) * (1 + 0.1 * np.sin(7 * x))
+    wj[1] = 0.3 * np.cos(3 * x)
+    wj[2] = np.where(x < 0.5, 1.0, 0.1)
+    uj = np.asfortranarray(po.conservative_1d(p1.gamma, wj))
+    a = {"u": u1, "w_eq": w1, "dudt": rr.run("bench1d_update_fvm", u=u1, w_eq=w1)["dudt"],
+         "u_jump": uj, "dudt_jump": rr.run("bench1d_update_fvm", u=uj, w_eq=w1)["dudt"]}
+    n = 200
+    xs = (np.arange(n) + 0.5) / n
+    sod = rr.run("fvm_condinit", x=xs)["u"]
+    ws = np.zeros((3, n), order="F")
+    ws[0] = 1 + 0.2 * np.sin(2 * np.pi * xs)
+    ws[1] = 0.5 * np.cos(2 * np.pi * xs)
+    ws[2] = 1 + 0.1 * np.cos(4 * np.pi * xs)
+    g = p1.gamma
+    us = np.asfortranarray(po.conservative_1d(g, ws))
+    wp = np.zeros((3, n), order="F")
+    wp[0] = 1 - (g - 1) / g * xs
+    wp[2] = wp[0] ** g
+    up = np.asfortranarray(po.conservative_1d(g, wp))
+    b = {"x": xs, "u_sod": sod, "dudt_sod": rr.run("fvm_update", u=sod)["dudt"],
+         "u_smooth": us, "dudt_smooth": rr.run("fvm_update", u=us)["dudt"],
+         "u_poly": up, "dudt_poly": rr.run("fvm_update", u=up)["dudt"]}
+    return {"bench1d_update_fvm_128": a, "fvm_update_200": b}
+
+
 def hllc_fixture():
     """riemann_hllc (dg_with_source.f90:1318-1374) on 1D state pairs chosen to
     reach all four sampled regions (supersonic either way, star states either
@@ -132,6 +167,9 @@ def main(only=None):
               "bench1d_update_sr_128": sr_fixture}
     if only in single:
         out[only] = single[only]()
+    elif only == "fvm1d":
+        out.update(fvm1d_fixtures())
+    if out:
         for name, arrs in out.items():
             path = os.path.join(HERE, name + ".npz")
             np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrs.items()})
@@ -210,6 +248,9 @@ def main(only=None):
 
     # compute_update_sr, the 'WB1' hydrostatic reconstruction
     out["bench1d_update_sr_128"] = sr_fixture()
+
+    # the plain 1D residuals: benchmark_1d 'FVM' and fvm.f90
+    out.update(fvm1d_fixtures())
 
     # A9 compute_update, the plain non-WB residual
     out["bench2d_update_plain_300"] = plain_fixture()

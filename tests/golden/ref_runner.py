@@ -9,7 +9,8 @@ call on a thread with a 2 GiB stack.
 
 The reference's grid sizes are compile-time parameters of its own modules:
 benchmark_2d is 300x300 (parameters_2d.f90:3-4), benchmark_1d has nx=128
-(parameters.f90:3), 2d DG is 8x8 cells with mx=my=2 (2d/parameters_dg_2d.f90:3-6).
+(parameters.f90:3), fvm.f90 has nx=200, bc=2, source=2 and the Sod set-up
+ninit=4 (fvm_commons.f90:5-14), 2d DG is 8x8 cells with mx=my=2 (2d/parameters_dg_2d.f90:3-6).
 """
 from __future__ import annotations
 
@@ -30,6 +31,7 @@ LIBS = {
     "bench1d": "libref_bench1d.so",
     "dg2d": "libref_dg2d.so",
     "dg1d": "libref_dg1d.so",
+    "fvm": "libref_fvm.so",
 }
 
 
@@ -101,6 +103,11 @@ def _child(op, fin, fout):
             d = np.zeros_like(u, order="F")
             L.compute_update_sr_(_dp(u), _dp(w), _dp(d))
             out["dudt"] = d
+        elif op == "bench1d_update_fvm":
+            u = np.asfortranarray(z["u"]); w = np.asfortranarray(z["w_eq"])
+            d = np.zeros_like(u, order="F")
+            L.compute_update_fvm_(_dp(u), _dp(w), _dp(d))
+            out["dudt"] = d
         elif op == "bench1d_max_speed":
             u = np.asfortranarray(z["u"])
             c = np.zeros(1)
@@ -115,6 +122,23 @@ def _child(op, fin, fout):
                 L.compute_llflux_(_dp(a), _dp(b), _dp(c), _dp(e), _dp(o))
                 res[q] = o
             out["f"] = res
+        else:
+            raise ValueError(op)
+    elif op.startswith("fvm_"):
+        L = ctypes.CDLL(os.path.join(REF_DIR, LIBS["fvm"]))
+        if op == "fvm_update":
+            u = np.asfortranarray(z["u"])
+            d = np.zeros_like(u, order="F")
+            L.compute_update_(_dp(u), _dp(d))
+            out["dudt"] = d
+        elif op == "fvm_condinit":
+            x = np.ascontiguousarray(z["x"])
+            res = np.zeros((3, len(x)), order="F")
+            for i, xv in enumerate(x):
+                xx, uu = ctypes.c_double(float(xv)), np.zeros(3)
+                L.condinit_(ctypes.byref(xx), _dp(uu))
+                res[:, i] = uu
+            out["u"] = res
         else:
             raise ValueError(op)
     elif op.startswith("dg1d_"):

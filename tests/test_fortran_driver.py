@@ -93,6 +93,15 @@ def test_1d_driver_binds_dropins_to_library():
         assert sym in undefined and sym in fvwb.declared_symbols()
 
 
+def test_fvm_driver_binds_dropins_to_library():
+    nm = subprocess.run(["nm", exe("fvm_fvwb")], capture_output=True, text=True, check=True).stdout
+    defined = set(re.findall(r"^[0-9a-f]+ T (\S+)$", nm, re.M))
+    undefined = set(re.findall(r"^\s+U (\S+)$", nm, re.M))
+    assert {"compute_update_", "compute_max_speed_"} <= defined
+    for sym in ("fvwb1d_fvm_compute_update", "fvwb1d_compute_max_speed", "fvwb1d_create"):
+        assert sym in undefined and sym in fvwb.declared_symbols()
+
+
 def test_driver_without_device_fails_loudly():
     """Without a usable GPU the driver stops with the library's error instead
     of computing anything on the host."""
@@ -234,6 +243,65 @@ def test_1d_driver_wb1_loop_matches_oracle(tmp_path):
     print(f"steps {steps} vs {n}, t {t} vs {to}, max|du| {np.max(np.abs(u - uo)):.3e}", flush=True)
     assert steps == n and abs(t - to) <= 1e-14
     assert np.max(np.abs(u - uo)) <= 1e-12
+
+
+def oracle_heun_1d(residual, u0, tend, dx, stages_per_dt):
+    """The drivers' loop: dt = 0.8*dx/cmax/(2n+1) with the FP32 literal 0.8, Heun."""
+    p = po.make_params_1d(u0.shape[1])
+    uo, to, n = np.asfortranarray(u0.copy()), 0.0, 0
+    while to < tend:
+        cmax = po.ref1d_max_speed(p, uo)
+        dt = float(np.float32(0.8)) * dx / cmax / stages_per_dt
+        d = residual(uo)
+        w1 = uo + dt * d
+        d = residual(w1)
+        uo = 0.5 * uo + 0.5 * w1 + 0.5 * dt * d
+        to, n = to + dt, n + 1
+    return uo, to, n
+
+
+@pytest.mark.gpu
+def test_1d_driver_fvm_loop_matches_oracle_bitwise(tmp_path):
+    """nx=64, the plain 'FVM' residual of benchmark_1d.f90 through the drop-in
+    compute_update_fvm: the same loop over the oracle, bit for bit."""
+    prefix = str(tmp_path / "f1")
+    r = run([exe("benchmark_1d_fvwb_64"), "fvm", "0.05", prefix])
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps, t, u0, w_eq, u = read_state_1d(prefix + ".bin")
+    p = po.make_params_1d(64)
+    uo, to, n = oracle_heun_1d(lambda v: po.ref1d_update_fvm(p, v), u0, 0.05, 1.0 / 64, 3.0)
+    print(f"steps {steps} vs {n}, t {t} vs {to}, max|du| {np.max(np.abs(u - uo)):.3e}", flush=True)
+    assert steps == n and t == to
+    assert np.array_equal(u, uo)
+
+
+@pytest.mark.gpu
+def test_fvm_program_driver_matches_oracle_bitwise(tmp_path):
+    """The stand-alone fvm.f90 program (nx=200, Sod, zero gradient, gravity
+    source) over the drop-in compute_update / compute_max_speed: its initial
+    state equals the reference's condinit bit for bit, and the run equals the
+    oracle's loop over fvm.f90's compute_update bit for bit."""
+    tend = 0.05
+    prefix = str(tmp_path / "p") + "/"
+    os.makedirs(prefix)
+    r = run([exe("fvm_fvwb"), repr(tend), prefix])
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(prefix + ".bin", "rb") as f:
+        nx, nvar, steps = np.frombuffer(f.read(12), dtype=np.int32)
+        t = float(np.frombuffer(f.read(8), dtype=np.float64)[0])
+        arrs = np.frombuffer(f.read(2 * 3 * 200 * 8), dtype=np.float64)
+    assert (nx, nvar) == (200, 3)
+    u0, u = (arrs[k * 600:(k + 1) * 600].reshape((3, 200), order="F") for k in range(2))
+    g = np.load(os.path.join(ROOT, "tests", "golden", "fvm_update_200.npz"))
+    assert np.array_equal(u0, g["u_sod"])
+    p = po.make_params_1d(200, bc=po.BC1D_ZERO_GRADIENT)
+    uo, to, n = oracle_heun_1d(lambda v: po.fvm_update(p, v, 2), u0, tend, 1.0 / 200, 7.0)
+    print(f"steps {steps} vs {n}, t {t} vs {to}, max|du| {np.max(np.abs(u - uo)):.3e}", flush=True)
+    assert steps == n and t == to
+    assert np.array_equal(u, uo)
+    for name in ("value00000.dat", "value99999.dat"):
+        lines = open(prefix + name).read().splitlines()
+        assert len(lines) == 200 and all(re.fullmatch(r"(?:[ -]\d\.\d{5}E[+-]\d\d ?){4}", ln) for ln in lines)
 
 
 @pytest.mark.gpu

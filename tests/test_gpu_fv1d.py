@@ -119,6 +119,51 @@ def test_compute_update_sr_keeps_isentropic_rest():
     assert np.max(np.abs(d)) <= 64 * np.finfo(float).eps * nx
 
 
+# ---- the plain residuals: compute_update_fvm and fvm.f90's compute_update -------------
+def test_plain_updates_match_reference_golden_bitwise():
+    say("case: 1D compute_update_fvm 128 cells and fvm.f90 compute_update 200 cells, reference fixtures")
+    g = load("bench1d_update_fvm_128")
+    with fvwb.Solver1D(fvwb.reference_params_1d(128, 2)) as s:
+        for k in ("", "_jump"):
+            same(s.compute_update_fvm(g["u" + k]), g["dudt" + k], "fvm dudt" + k)
+    g = load("fvm_update_200")
+    q = fvwb.reference_params_1d(200, 2)
+    q.bc = fvwb.BC1D_ZERO_GRADIENT
+    with fvwb.Solver1D(q) as s:
+        for k in ("_sod", "_smooth", "_poly"):
+            same(s.fvm_compute_update(g["u" + k], 2), g["dudt" + k], "fvm.f90 dudt" + k)
+
+
+@pytest.mark.parametrize("nx", [4, 37, 1000, 5000])
+def test_plain_updates_match_oracle_bitwise(nx):
+    say(f"case: plain 1D residuals nx={nx}")
+    op = po.make_params_1d(nx)
+    u, _ = po.perturbed_state_1d(op, amp_bump=1e-2, amp_noise=1e-3, salt=nx + 2)
+    u[1] += 0.05 * np.sin(np.arange(nx))
+    with fvwb.Solver1D(fvwb.reference_params_1d(nx, 2)) as s:
+        same(s.compute_update_fvm(u), po.ref1d_update_fvm(op, u), "compute_update_fvm")
+    for bc in (po.BC1D_PERIODIC, po.BC1D_ZERO_GRADIENT):
+        q = fvwb.reference_params_1d(nx, 2)
+        q.bc = bc
+        with fvwb.Solver1D(q) as s:
+            for src in (1, 2):
+                same(s.fvm_compute_update(u, src), po.fvm_update(po.make_params_1d(nx, bc=bc), u, src),
+                     f"fvm.f90 bc={bc} source={src}")
+
+
+def test_plain_update_error_paths():
+    say("case: fvm.f90 compute_update argument checks")
+    u = po.conservative_1d(1.4, np.tile([[1.0], [0.0], [1.0]], (1, 16)))
+    with fvwb.Solver1D(fvwb.reference_params_1d(16, 2)) as s:
+        with pytest.raises(fvwb.FvwbError, match="source"):
+            s.fvm_compute_update(u, 3)
+    q = fvwb.reference_params_1d(16, 2)
+    q.bc = po.BC1D_WALL
+    with fvwb.Solver1D(q) as s:
+        with pytest.raises(fvwb.FvwbError, match="bc 1"):
+            s.fvm_compute_update(u, 2)
+
+
 SCHEMES = [(r, bc, i) for r in (0, 1) for bc in (po.BC1D_ZERO_GRADIENT, po.BC1D_WALL) for i in (0, 1)]
 
 

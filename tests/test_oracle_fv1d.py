@@ -1,5 +1,6 @@
-"""The 1D CPU oracle: the benchmark_1d.f90 restatements ('EQL' compute_update
-and 'WB1' compute_update_sr) against the fixtures
+"""The 1D CPU oracle: the benchmark_1d.f90 restatements ('EQL' compute_update,
+'WB1' compute_update_sr, 'FVM' compute_update_fvm) and fvm.f90's compute_update
+against the fixtures
 written from the flang-built reference (bit for bit) and live against
 oracle/_ref when present; the configured 1D scheme against what the
 mathematics fixes (exact rest, flux consistency, wall symmetry and mass
@@ -88,6 +89,61 @@ def test_ref1d_update_sr_converges_to_the_euler_gravity_operator():
         errs.append(np.max(np.abs(d - exact)[:, 2:-2]))
     assert errs[0] < 0.2
     assert errs[0] / errs[1] > 1.8 and errs[1] / errs[2] > 1.8, errs
+
+
+def test_ref1d_update_fvm_matches_reference_golden_bitwise():
+    """compute_update_fvm (benchmark_1d.f90:454-549, the plain 'FVM' solver)."""
+    g = load("bench1d_update_fvm_128")
+    p, u, _ = bench1d_inputs()
+    np.testing.assert_array_equal(u, g["u"])
+    for k in ("", "_jump"):
+        np.testing.assert_array_equal(po.ref1d_update_fvm(p, g["u" + k]), g["dudt" + k], err_msg=k)
+
+
+def test_fvm_update_matches_reference_golden_bitwise():
+    """compute_update of fvm.f90 (:188-251) as the reference compiles it (nx=200,
+    bc 2, source 2) on its own Sod set-up, a smooth moving state and condinit's
+    case-7 state."""
+    g = load("fvm_update_200")
+    p = po.make_params_1d(200, bc=po.BC1D_ZERO_GRADIENT)
+    for k in ("_sod", "_smooth", "_poly"):
+        np.testing.assert_array_equal(po.fvm_update(p, g["u" + k], 2), g["dudt" + k], err_msg=k)
+
+
+def test_fvm_update_periodic_is_translation_invariant():
+    """bc 1 (which the reference build cannot reach): without the source every
+    cell sees its wrapped neighbours the same way, so a cyclic shift of the state
+    shifts dudt bit for bit; with zero gradient the end cells differ."""
+    nx = 64
+    u = np.asfortranarray(load("fvm_update_200")["u_smooth"][:, :nx])
+    pp = po.make_params_1d(nx, bc=po.BC1D_PERIODIC)
+    d = po.fvm_update(pp, u, 1)
+    for k in (1, 17, 63):
+        np.testing.assert_array_equal(po.fvm_update(pp, np.roll(u, k, axis=1), 1), np.roll(d, k, axis=1))
+    pz = po.make_params_1d(nx, bc=po.BC1D_ZERO_GRADIENT)
+    dz = po.fvm_update(pz, u, 1)
+    np.testing.assert_array_equal(dz[:, 1:-1], d[:, 1:-1])
+    assert not np.array_equal(dz[:, 0], d[:, 0])
+
+
+def test_fvm_update_source_switch_adds_the_gravity_source():
+    """source 2 minus source 1 is the closed-form source: -rho and -rho*v inside,
+    half of that in the two end cells (x- and x+ clamped to the cell centre),
+    to the rounding of the flux part; a uniform state at rest gives exactly zero
+    with source 1 (flux consistency)."""
+    nx = 50
+    g = load("fvm_update_200")
+    u = np.asfortranarray(g["u_smooth"][:, :nx])
+    p = po.make_params_1d(nx, bc=po.BC1D_ZERO_GRADIENT)
+    d1, d2 = po.fvm_update(p, u, 1), po.fvm_update(p, u, 2)
+    rho, v = u[0], u[1] / u[0]
+    half = np.ones(nx); half[[0, -1]] = 0.5
+    tol = 8 * np.finfo(float).eps * (np.abs(d1) + np.abs(d2) + 1)
+    np.testing.assert_array_equal(d2[0], d1[0])
+    assert np.all(np.abs((d2[1] - d1[1]) - (-rho * half)) <= tol[1])
+    assert np.all(np.abs((d2[2] - d1[2]) - (-rho * v * half)) <= tol[2])
+    uni = np.asfortranarray(np.tile([[1.3], [0.0], [2.0]], (1, nx)))
+    assert np.count_nonzero(po.fvm_update(p, uni, 1)) == 0
 
 
 def test_ref1d_llflux_matches_reference_golden_bitwise():
