@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/fvwb.h"
+#include "fvwb_euler1d.h"
 
 // Every expression in this file rounds as written (the oracle and the
 // reference are built without FMA contraction).
@@ -50,41 +51,6 @@ struct Clock {
   int32_t steps;
   uint32_t err;     // bit 0: rho<=0, p<=0 or non-finite seen; bit 1: dt not positive
 };
-
-// ---- pointwise physics (benchmark_1d.f90:115-198, 438-451) ----------------------
-__host__ __device__ inline void primitive(double gamma, const double u[3], double w[3]) {
-  w[0] = u[0];
-  w[1] = u[1] / w[0];
-  w[2] = (gamma - 1.0) * (u[2] - 0.5 * w[0] * (w[1] * w[1]));
-}
-
-__host__ __device__ inline void conservative(double gamma, const double w[3], double u[3]) {
-  u[0] = w[0];
-  u[1] = w[0] * w[1];
-  u[2] = w[2] / (gamma - 1.0) + 0.5 * w[0] * (w[1] * w[1]);
-}
-
-__host__ __device__ inline double speed(double gamma, const double u[3]) {
-  double w[3];
-  primitive(gamma, u, w);
-  double cs = sqrt(gamma * fmax(w[2], 1e-10) / fmax(w[0], 1e-10));
-  return fabs(w[1]) + cs;
-}
-
-__host__ __device__ inline void flux(double gamma, const double u[3], double f[3]) {
-  double w[3];
-  primitive(gamma, u, w);
-  f[0] = w[1] * u[0];
-  f[1] = w[1] * u[1] + w[2];
-  f[2] = w[1] * u[2] + w[2] * w[1];
-}
-
-// compute_llflux(uleft, uright, f_left, f_right)
-__host__ __device__ inline void llflux(double gamma, const double ul[3], const double ur[3], const double fl[3],
-                                       const double fr[3], double out[3]) {
-  double cmax = fmax(speed(gamma, ul), speed(gamma, ur));
-  for (int k = 0; k < 3; ++k) out[k] = 0.5 * (fr[k] + fl[k]) - 0.5 * cmax * (ur[k] - ul[k]);
-}
 
 // limiters.f90:1-21 minmod(x, y, y): the common-sign minimum magnitude
 __device__ inline double minmod2(double x, double y) {

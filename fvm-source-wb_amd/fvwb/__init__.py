@@ -96,6 +96,19 @@ class DGParams(ctypes.Structure):
     ]
 
 
+DG1D_RIEMANN_LLF, DG1D_RIEMANN_HLLC = 1, 2
+
+
+class DG1DParams(ctypes.Structure):
+    """``fvwb_dg1d_params`` of include/fvwb.h."""
+
+    _fields_ = [
+        ("nvar", ctypes.c_int32), ("nx", ctypes.c_int32), ("n", ctypes.c_int32),
+        ("riemann", ctypes.c_int32), ("source", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("boxlen", ctypes.c_double),
+    ]
+
+
 def make_dg_params(nx, ny, *, degree=2, flux=FLUX_HLL, limiter=DG_LIMITER_TVB, gamma=1.4, boxlen_x=1.0,
                    boxlen_y=1.0, cfl=0.8, rho0=1.0, p0=1.0, g=1.0, tvb_m=50.0, integrator=0) -> DGParams:
     """Defaults: the DG gravity workload (P2, HLL, TVB with M = 50, walls, CFL 0.8)."""
@@ -173,6 +186,13 @@ def lib():
         "fvwb_dg_get_time": [H, dp, dp, ip],
         "fvwb_dg_get_stream": [H, ctypes.POINTER(ctypes.c_void_p)],
     })
+    pd1 = ctypes.POINTER(DG1DParams)
+    sig.update({
+        "fvwb_dg1d_params_reference": [pd1, ctypes.c_int32],
+        "fvwb_dg1d_create": [ctypes.POINTER(H), pd1],
+        "fvwb_dg1d_destroy": [H],
+        "fvwb_dg1d_compute_update_exact_delta": [H, dp, dp, dp],
+    })
     p1 = ctypes.POINTER(Params1D)
     sig.update({
         "fvwb1d_params_reference": [p1, ctypes.c_int32, ctypes.c_int32],
@@ -203,6 +223,8 @@ def lib():
     L.fvwb1d_last_error.restype = ctypes.c_char_p
     L.fvwb_dg_last_error.argtypes = [H]
     L.fvwb_dg_last_error.restype = ctypes.c_char_p
+    L.fvwb_dg1d_last_error.argtypes = [H]
+    L.fvwb_dg1d_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -676,3 +698,60 @@ class SolverDG:
         s = ctypes.c_void_p()
         _checkdg(lib().fvwb_dg_get_stream(self._h, ctypes.byref(s)), self._h)
         return s.value or 0
+
+
+# ---- 1D modal DG (dg_with_source.f90) ------------------------------------------------
+def reference_params_dg1d(nx=128) -> DG1DParams:
+    """dg_commons.f90's set-up: n = 3 modes, HLLC, gravity source, gamma 1.4 (FP32)."""
+    p = DG1DParams()
+    rc = lib().fvwb_dg1d_params_reference(ctypes.byref(p), nx)
+    if rc != OK:
+        raise FvwbError(rc, "bad reference parameters")
+    return p
+
+
+def _checkdg1d(rc, h=None):
+    if rc != OK:
+        msg = lib().fvwb_dg1d_last_error(h)
+        raise FvwbError(rc, msg.decode() if msg else "")
+
+
+class SolverDG1D:
+    """The reference's 1D modal DG on the GPU (FP64).  Modes and the equilibrium
+    at the Gauss points are float64 AoS(3,n,nx): numpy shape (3, n, nx), order="F"."""
+
+    def __init__(self, params: DG1DParams):
+        self.params = params
+        self.shape = (3, params.n, params.nx)
+        self._h = ctypes.c_void_p()
+        _checkdg1d(lib().fvwb_dg1d_create(ctypes.byref(self._h), ctypes.byref(params)))
+
+    def close(self):
+        if self._h:
+            lib().fvwb_dg1d_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _arr(self, a):
+        a = np.asfortranarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise FvwbError(ERR_ARG, f"expected shape {self.shape}, got {a.shape}")
+        return a
+
+    def compute_update_exact_delta(self, delta_u, u_eq):
+        """dudt of compute_update_exact_delta(delta_u, u_eq, dudt) (dg_with_source.f90:1749-2031)."""
+        d, e = self._arr(delta_u), self._arr(u_eq)
+        out = np.zeros(self.shape, order="F")
+        _checkdg1d(lib().fvwb_dg1d_compute_update_exact_delta(self._h, _dp(d), _dp(e), _dp(out)), self._h)
+        return out

@@ -360,6 +360,49 @@ int fvwb_dg_download(fvwb_dg_handle h, double* modes);
 int fvwb_dg_get_time(fvwb_dg_handle h, double* t, double* dt_last, int32_t* steps_done);
 int fvwb_dg_get_stream(fvwb_dg_handle h, void** stream);
 
+/* ---- 1D modal DG with source (dg_with_source.f90, dg_commons.f90) ---------
+ * Modes delta_u(1:nvar,1:n,1:nx) (variable fastest, then mode, then cell) of
+ * the deviation from the equilibrium, over the basis of legendre.f90:1-24
+ * (phi_0 = sqrt(0.5), phi_1 = sqrt(6)/2 x, phi_2 = sqrt(10)/4 (3x^2-1) with the
+ * square roots taken in FP32 as the reference writes them, phi_3.. the plain
+ * Legendre polynomials), with n Gauss points from gl_quadrature
+ * (legendre.f90:77-127).  The equilibrium is rho = p = exp(-x), v = 0
+ * (get_eq_solution :1031-1048); u_eq(1:nvar,1:n,1:nx) holds it, conservative,
+ * at each cell's Gauss points, as the reference's main program builds it. */
+#define FVWB_DG1D_RIEMANN_LLF   1   /* riemann_llf  (dg_with_source.f90:1299-1316) */
+#define FVWB_DG1D_RIEMANN_HLLC  2   /* riemann_hllc (dg_with_source.f90:1318-1374) */
+
+typedef struct fvwb_dg1d_params {
+  int32_t nvar;      /* must be 3                                              */
+  int32_t nx;        /* cells, >= 3                                            */
+  int32_t n;         /* modes = Gauss points, 1..6 (dg_commons.f90:4-7)        */
+  int32_t riemann;   /* FVWB_DG1D_RIEMANN_*                                    */
+  int32_t source;    /* 1 none, 2 gravity phi = x (compute_source :1196-1211)  */
+  int32_t pad;
+  double  gamma, boxlen;
+} fvwb_dg1d_params;
+
+typedef struct fvwb_dg1d* fvwb_dg1d_handle;
+/* dg_commons.f90 set-up: n = 3, HLLC, source 2, gamma = 1.4 and boxlen = 1
+ * rounded as the reference's default-real literals. */
+int fvwb_dg1d_params_reference(fvwb_dg1d_params* p, int32_t nx);
+int fvwb_dg1d_create(fvwb_dg1d_handle* h, const fvwb_dg1d_params* p);
+int fvwb_dg1d_destroy(fvwb_dg1d_handle h);
+const char* fvwb_dg1d_last_error(fvwb_dg1d_handle h);
+/* Replaces `subroutine compute_update_exact_delta(delta_u, u_eq, dudt)`
+ * (dg_with_source.f90:1749-2031), the well-balanced deviation-form DG residual:
+ * volume terms F(u_eq + delta) - F(u_eq) at the Gauss points, face states
+ * u_eq(face) + delta trace, numerical flux minus the equilibrium's physical
+ * flux at the faces, source s(u_eq + delta) - s(u_eq); dudt of the two end
+ * cells is zero (:2027-2028), so neither the boundary faces nor the bc code
+ * reach it.  The reference's scaling is kept: its basis is orthonormal on
+ * [-1,1] and its rate is (1/dx)(volume - faces) + (1/2) source, half the usual
+ * DG rate.  FP64, the reference's statement order: equal to it bit for bit
+ * (the face equilibria and the tables are computed on the host with the
+ * reference's expressions). */
+int fvwb_dg1d_compute_update_exact_delta(fvwb_dg1d_handle h, const double* delta_u, const double* u_eq,
+                                         double* dudt);
+
 #ifdef __cplusplus
 }
 #endif

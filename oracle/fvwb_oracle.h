@@ -21,7 +21,9 @@
  *   orc_dg_*, orc_ref_dg2d_* — the reference's Legendre basis, Gauss-Legendre
  *                  quadrature and tensor projections (2d/), pinned like above;
  *   orc_dg2d_*   — the configured modal DG (deviation form, P_k, k <= 2),
- *                  pinned by invariants.
+ *                  pinned by invariants;
+ *   orc_dg1d_*   — restatements of the 1D modal DG of dg_with_source.f90 and
+ *                  its legendre.f90, pinned like orc_ref2d_*.
  */
 #ifndef FVWB_ORACLE_H
 #define FVWB_ORACLE_H
@@ -139,6 +141,17 @@ void   orc_dg2d_advance(const fvwb_dg_params* p, double* d, double t_end, int ma
 /* deviation at the (k+1)^2 volume points, AoS(4,nx,ny,(k+1)^2), point a + (k+1)*b */
 void   orc_dg2d_nodal(const fvwb_dg_params* p, const double* d, double* nodal);
 void   orc_dg2d_project(const fvwb_dg_params* p, const double* nodal, double* d);
+
+/* ---- 1D modal DG (dg_with_source.f90 with legendre.f90) ------------------- */
+double orc_dg1d_legendre(double x, int n);                  /* legendre.f90:1-24   */
+double orc_dg1d_legendre_prime(double x, int n);            /* legendre.f90:26-48  */
+void   orc_dg1d_gl_quadrature(int n, double* x, double* w); /* legendre.f90:77-127 */
+/* the tables of one set-up: Gauss points and weights, phi_i(xi_j) and
+ * phi_i'(xi_j) at [i*n+j], phi_i(-1), phi_i(+1) at [2i], [2i+1] */
+void   orc_dg1d_tables(int n, double* chsi, double* wq, double* phi_q, double* dphi_q, double* phi_lr);
+/* compute_update_exact_delta (:1749-2031); modes and u_eq AoS(3,n,nx) */
+void   orc_dg1d_update_exact_delta(const fvwb_dg1d_params* p, const double* delta_u, const double* u_eq,
+                                   double* dudt);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,25 @@ def make_params_1d(nx, *, bc=BC1D_ZERO_GRADIENT, recon=RECON_FIRST_ORDER, integr
     return Params1D(3, nx, bc, recon, integrator, nequilibrium, gamma, boxlen, cfl, g)
 
 
+DG1D_RIEMANN_LLF, DG1D_RIEMANN_HLLC = 1, 2
+
+
+class DG1DParams(ctypes.Structure):
+    """Mirror of ``fvwb_dg1d_params`` (include/fvwb.h)."""
+
+    _fields_ = [
+        ("nvar", ctypes.c_int32), ("nx", ctypes.c_int32), ("n", ctypes.c_int32),
+        ("riemann", ctypes.c_int32), ("source", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("boxlen", ctypes.c_double),
+    ]
+
+
+def make_dg1d_params(nx=128, *, n=3, riemann=DG1D_RIEMANN_HLLC, source=2, gamma=GAMMA_REF,
+                     boxlen=1.0) -> DG1DParams:
+    """dg_commons.f90 defaults: n = 3, HLLC (riemann = 2), source 2, gamma 1.4 as FP32."""
+    return DG1DParams(3, nx, n, riemann, source, 0, gamma, boxlen)
+
+
 DG_LIMITER_NONE, DG_LIMITER_TVB = 0, 1
 DG_SSPRK3, DG_SSPRK54 = 0, 1
 
@@ -151,6 +170,13 @@ def lib():
         L.orc_dg_legendre_prime.argtypes = [ctypes.c_double, ctypes.c_int]
         L.orc_dg_legendre_prime.restype = ctypes.c_double
         L.orc_dg_gl_quadrature.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_dg1d_legendre.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_dg1d_legendre.restype = ctypes.c_double
+        L.orc_dg1d_legendre_prime.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_dg1d_legendre_prime.restype = ctypes.c_double
+        L.orc_dg1d_gl_quadrature.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_dg1d_tables.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
+        L.orc_dg1d_update_exact_delta.argtypes = [ctypes.POINTER(DG1DParams), dp, dp, dp]
         L.orc_ref_dg2d_modes_from_nodes.argtypes = [ctypes.c_int] * 3 + [dp, dp]
         L.orc_ref_dg2d_nodes_from_modes.argtypes = [ctypes.c_int] * 3 + [dp, dp]
         L.orc_dg2d_nmodes.argtypes = [ctypes.c_int]
@@ -465,6 +491,49 @@ def dg_gl_quadrature(n):
     x, w = np.zeros(n), np.zeros(n)
     lib().orc_dg_gl_quadrature(n, _p(x), _p(w))
     return x, w
+
+
+# ---- 1D modal DG (dg_with_source.f90, legendre.f90) -----------------------------------
+def dg1d_legendre(x, n):
+    return lib().orc_dg1d_legendre(float(x), int(n))
+
+
+def dg1d_legendre_prime(x, n):
+    return lib().orc_dg1d_legendre_prime(float(x), int(n))
+
+
+def dg1d_gl_quadrature(n):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orc_dg1d_gl_quadrature(n, _p(x), _p(w))
+    return x, w
+
+
+def dg1d_tables(n):
+    """(chsi, w, phi[i,j] = phi_i(chsi_j), dphi[i,j], phi_lr[i] = (phi_i(-1), phi_i(+1)))."""
+    c, w, ph, dph, lr = np.zeros(n), np.zeros(n), np.zeros(n * n), np.zeros(n * n), np.zeros(2 * n)
+    lib().orc_dg1d_tables(n, _p(c), _p(w), _p(ph), _p(dph), _p(lr))
+    return c, w, ph.reshape(n, n), dph.reshape(n, n), lr.reshape(n, 2)
+
+
+def dg1d_equilibrium_nodes(p: DG1DParams):
+    """u_eq(1:3,1:n,1:nx) as the reference's main program builds it
+    (dg_with_source.f90:71-89): exp(-x) at x = xcell + dx/2*chsi_j, conservative."""
+    chsi, _ = dg1d_gl_quadrature(p.n)
+    dx = p.boxlen / p.nx
+    xc = (np.arange(1, p.nx + 1) - 0.5) * dx
+    x = xc[None, :] + dx / 2.0 * chsi[:, None]
+    r = np.exp(-x)
+    u = np.zeros((3, p.n, p.nx), order="F")
+    u[0] = r
+    u[2] = r / (p.gamma - 1.0)
+    return u
+
+
+def dg1d_update_exact_delta(p: DG1DParams, delta_u, u_eq):
+    delta_u, u_eq = fortran(delta_u), fortran(u_eq)
+    out = np.zeros_like(delta_u, order="F")
+    lib().orc_dg1d_update_exact_delta(ctypes.byref(p), _p(delta_u), _p(u_eq), _p(out))
+    return out
 
 
 def ref_dg2d_modes_from_nodes(nodes):

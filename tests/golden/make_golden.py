@@ -129,6 +129,52 @@ def fvm1d_fixtures():
     return {"bench1d_update_fvm_128": a, "fvm_update_200": b}
 
 
+def dg1d_initial_delta(p, pert=float(np.float32(1e-8))):
+    """The reference main program's initial modes for ninit = 8 (dg_with_source.f90:
+    168-189): the Gauss-point values of condinit minus the equilibrium,
+    projected with its factor 0.5."""
+    chsi, wq, ph, _, _ = po.dg1d_tables(p.n)
+    dx = p.boxlen / p.nx
+    x = ((np.arange(1, p.nx + 1) - 0.5) * dx)[None, :] + dx / 2.0 * chsi[:, None]
+    dE = pert * np.exp(-100 * (x - p.boxlen / 2.0) ** 2) / (p.gamma - 1.0)
+    d = np.zeros((3, p.n, p.nx), order="F")
+    for i in range(p.n):
+        d[2, i] = np.sum(0.5 * dE * ph[i][:, None] * wq[:, None], axis=0)
+    return d
+
+
+def dg1d_fixtures():
+    """The reference's 1D DG: legendre / legendre_prime (legendre.f90:1-48) for
+    n = 0..6 on a grid with the clamp, gl_quadrature for n = 1..6, and
+    compute_update_exact_delta (dg_with_source.f90:1749-2031) as compiled (n = 3,
+    nx = 128, HLLC, source 2) on its own initial modes, on random modes of size
+    1e-2, on strong modes with flow, and on zero modes."""
+    assert rr.available("dg1d"), "build oracle/_ref first"
+    x = np.concatenate([np.linspace(-1, 1, 41), [-1.5, 1.5]])
+    r = rr.run("dg1d_legendre", x=x)
+    out = {"dg1d_legendre": {"x": x, "P": r["P"], "dP": r["dP"]},
+           "dg1d_gl_quadrature": rr.run("dg1d_gl_quadrature")}
+    p = po.make_dg1d_params(128)
+    ue = po.dg1d_equilibrium_nodes(p)
+    rng = np.random.default_rng(1749)
+    states = {"init": dg1d_initial_delta(p),
+              "rand": np.asfortranarray(rng.normal(size=(3, 3, 128)) * 1e-2),
+              "zero": np.zeros((3, 3, 128), order="F")}
+    xs = (np.arange(128) + 0.5) / 128
+    strong = np.zeros((3, 3, 128), order="F")
+    strong[0, 0] = 0.2 * np.sin(6 * xs) * np.exp(-xs)
+    strong[1, 0] = 0.3 * np.cos(5 * xs) * np.exp(-xs)
+    strong[2, 0] = 0.4 * np.sin(3 * xs) * np.exp(-xs)
+    strong[:, 1:] = rng.normal(size=(3, 2, 128)) * 0.02
+    states["strong"] = strong
+    upd = {"u_eq": ue}
+    for k, d in states.items():
+        upd["du_" + k] = d
+        upd["dudt_" + k] = rr.run("dg1d_update_exact_delta", delta_u=d, u_eq=ue)["dudt"]
+    out["dg1d_update_exact_delta_128"] = upd
+    return out
+
+
 def hllc_fixture():
     """riemann_hllc (dg_with_source.f90:1318-1374) on 1D state pairs chosen to
     reach all four sampled regions (supersonic either way, star states either
@@ -169,6 +215,8 @@ def main(only=None):
         out[only] = single[only]()
     elif only == "fvm1d":
         out.update(fvm1d_fixtures())
+    elif only == "dg1d":
+        out.update(dg1d_fixtures())
     if out:
         for name, arrs in out.items():
             path = os.path.join(HERE, name + ".npz")
@@ -251,6 +299,9 @@ def main(only=None):
 
     # the plain 1D residuals: benchmark_1d 'FVM' and fvm.f90
     out.update(fvm1d_fixtures())
+
+    # the reference's 1D DG: basis, quadrature, compute_update_exact_delta
+    out.update(dg1d_fixtures())
 
     # A9 compute_update, the plain non-WB residual
     out["bench2d_update_plain_300"] = plain_fixture()

@@ -10,7 +10,8 @@ call on a thread with a 2 GiB stack.
 The reference's grid sizes are compile-time parameters of its own modules:
 benchmark_2d is 300x300 (parameters_2d.f90:3-4), benchmark_1d has nx=128
 (parameters.f90:3), fvm.f90 has nx=200, bc=2, source=2 and the Sod set-up
-ninit=4 (fvm_commons.f90:5-14), 2d DG is 8x8 cells with mx=my=2 (2d/parameters_dg_2d.f90:3-6).
+ninit=4 (fvm_commons.f90:5-14), the 1D DG has n=3 modes on nx=128 cells with
+HLLC and the gravity source (dg_commons.f90:4-15), 2d DG is 8x8 cells with mx=my=2 (2d/parameters_dg_2d.f90:3-6).
 """
 from __future__ import annotations
 
@@ -143,7 +144,28 @@ def _child(op, fin, fout):
             raise ValueError(op)
     elif op.startswith("dg1d_"):
         L = ctypes.CDLL(os.path.join(REF_DIR, LIBS["dg1d"]))
-        if op == "dg1d_riemann_hllc":
+        if op == "dg1d_legendre":
+            x = np.ascontiguousarray(z["x"])
+            L.legendre_.restype = ctypes.c_double
+            L.legendre_prime_.restype = ctypes.c_double
+            P = np.zeros((7, len(x))); dP = np.zeros((7, len(x)))
+            for k in range(7):
+                for i, xv in enumerate(x):
+                    a, b = ctypes.c_double(float(xv)), ctypes.c_double(float(xv))
+                    P[k, i] = L.legendre_(ctypes.byref(a), ctypes.byref(ctypes.c_int(k)))
+                    dP[k, i] = L.legendre_prime_(ctypes.byref(b), ctypes.byref(ctypes.c_int(k)))
+            out["P"], out["dP"] = P, dP
+        elif op == "dg1d_gl_quadrature":
+            for n in range(1, 7):
+                xq, wq = np.zeros(n), np.zeros(n)
+                L.gl_quadrature_(_dp(xq), _dp(wq), ctypes.byref(ctypes.c_int(n)))
+                out[f"x{n}"], out[f"w{n}"] = xq, wq
+        elif op == "dg1d_update_exact_delta":
+            d = np.asfortranarray(z["delta_u"]); e = np.asfortranarray(z["u_eq"])
+            r = np.zeros_like(d, order="F")
+            L.compute_update_exact_delta_(_dp(d), _dp(e), _dp(r))
+            out["dudt"] = r
+        elif op == "dg1d_riemann_hllc":
             ul, ur = (np.ascontiguousarray(z[k]) for k in ("ul", "ur"))
             gam = ctypes.c_double(float(z["gamma"]))
             nv = ctypes.c_int(3)

@@ -46,3 +46,22 @@ def test_null_handle_calls_are_argument_errors():
     assert L.fvwb_download(None, None) == fvwb.ERR_ARG
     assert L.fvwb_compute_update_exact(None, None, None, None) == fvwb.ERR_ARG
     assert L.fvwb_destroy(None) == fvwb.OK
+
+
+@pytest.mark.parametrize("field,value", [("nvar", 4), ("nx", 2), ("n", 0), ("n", 7), ("riemann", 0),
+                                         ("riemann", 3), ("source", 3), ("gamma", 1.0), ("boxlen", 0.0)])
+def test_dg1d_create_rejects_bad_params_before_any_device_call(field, value):
+    p = fvwb.reference_params_dg1d(32)
+    assert (p.nvar, p.nx, p.n, p.riemann, p.source) == (3, 32, 3, fvwb.DG1D_RIEMANN_HLLC, 2)
+    assert p.gamma == 1.399999976158142 and p.boxlen == 1.0
+    setattr(p, field, value)
+    h = ctypes.c_void_p()
+    rc = fvwb.lib().fvwb_dg1d_create(ctypes.byref(h), ctypes.byref(p))
+    assert rc == fvwb.ERR_ARG and not h.value
+    assert fvwb.lib().fvwb_dg1d_last_error(None)
+
+
+def test_dg1d_null_handle_calls_are_argument_errors():
+    L = fvwb.lib()
+    assert L.fvwb_dg1d_compute_update_exact_delta(None, None, None, None) == fvwb.ERR_ARG
+    assert L.fvwb_dg1d_destroy(None) == fvwb.OK
