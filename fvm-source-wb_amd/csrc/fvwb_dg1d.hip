@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -169,18 +170,25 @@ __device__ inline void source_term(double gamma, const double u[3], double s[3])
   s[2] = -w[0] * w[1];
 }
 
-// One thread per cell c.  du, ueq, dudt: AoS(3,N,nx); ufe, ffe: the
-// equilibrium's conservative state and physical flux at the nx+1 faces.
+struct Geometry {
+  const double* ueq;   // equilibrium at the Gauss points, AoS(3,N,nx)
+  const double* ufe;   // equilibrium conservative state at the nx+1 faces
+  const double* ffe;   // its physical flux there
+  int nx;
+  int riemann, source;
+  double gamma, oneoverdx;
+};
+
+// dudt of cell c (AoS(3,N) into out) for the modes du, AoS(3,N,nx).
 template <int N>
-__global__ __launch_bounds__(256) void dg1d_update_kernel(const double* __restrict__ du,
-                                                          const double* __restrict__ ueq,
-                                                          const double* __restrict__ ufe,
-                                                          const double* __restrict__ ffe,
-                                                          double* __restrict__ dudt, int nx, double gamma,
-                                                          double oneoverdx, int riemann, int source, Tables T) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nx) return;
-  double* out = dudt + (size_t)3 * N * c;
+__device__ inline void cell_residual(const double* du, const Geometry& G, const Tables& T, int c,
+                                     double out[3 * N]) {
+  const int nx = G.nx;
+  const double gamma = G.gamma, oneoverdx = G.oneoverdx;
+  const int riemann = G.riemann, source = G.source;
+  const double* ueq = G.ueq;
+  const double* ufe = G.ufe;
+  const double* ffe = G.ffe;
   if (c == 0 || c == nx - 1) {   // dudt(:,:,1) = dudt(:,:,nx) = 0 (:2027-2028)
     for (int k = 0; k < 3 * N; ++k) out[k] = 0;
     return;
@@ -247,6 +255,120 @@ __global__ __launch_bounds__(256) void dg1d_update_kernel(const double* __restri
   }
 }
 
+// The drop-in: one thread per cell.
+template <int N>
+__global__ __launch_bounds__(256) void dg1d_update_kernel(const double* __restrict__ du, Geometry G, Tables T,
+                                                          double* __restrict__ dudt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G.nx) return;
+  double out[3 * N];
+  cell_residual<N>(du, G, T, c, out);
+  for (int k = 0; k < 3 * N; ++k) dudt[(size_t)3 * N * c + k] = out[k];
+}
+
+// ---- the main program's time loop, integrator 'RKi' ------------------------------
+struct Clock {
+  double t;
+  double dt_last;
+  int32_t steps;
+  uint32_t err;   // bit 0: the wave-speed maximum or dt was not finite and positive
+};
+
+struct RunArgs {
+  double* d;                 // modes, AoS(3,N,nx)
+  double *w1, *w2, *w3, *w4, *acc;
+  double* uinit;             // the nodal reconstruction uinit, AoS(3,N,nx)
+  const double* phx;         // phi_k(x_quad(c,i)) at [(c*N + i)*N + k]
+  Clock* clk;
+  int32_t step_limit;        // stop after this many steps in total
+  double t_end;
+  double dx;
+  double coef[15];           // 'RKi' literals rounded to FP32, in the order of the stages
+};
+
+// One workgroup runs the whole loop; its threads stride over the cells, and a
+// workgroup barrier separates the stages (each stage reads its input's
+// neighbours).  The stage arrays live in device memory, where one workgroup's
+// writes are visible to its own threads after the barrier.
+template <int N>
+__global__ __launch_bounds__(256) void dg1d_run_kernel(Geometry G, Tables T, RunArgs R) {
+  __shared__ double red[4];
+  const int tid = threadIdx.x;
+  const int nx = G.nx;
+  const size_t M = (size_t)3 * N;
+  double t = R.clk->t, dtl = R.clk->dt_last;
+  int32_t steps = R.clk->steps;
+  uint32_t err = R.clk->err;
+  const double* k = R.coef;
+  while (err == 0 && steps < R.step_limit && t < R.t_end) {
+    // cmax over the cells of |v|+c at the first Gauss point (compute_max_speed :1136-1152)
+    double cm = 0.0;
+    for (int c = tid; c < nx; c += blockDim.x) cm = fmax(cm, fv1d::speed(G.gamma, R.uinit + M * c));
+    for (int off = 32; off > 0; off >>= 1) cm = fmax(cm, __shfl_xor(cm, off));
+    if ((tid & 63) == 0) red[tid >> 6] = cm;
+    __syncthreads();
+    double cmax = 0.0;
+    for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) cmax = fmax(cmax, red[wv]);
+    __syncthreads();
+    const double dt = (double)0.9f * R.dx / cmax / (2.0 * (double)N + 1.0);   // :197
+    if (!(dt > 0.0) || !isfinite(dt)) {
+      err = 1;
+      break;
+    }
+    double L[3 * N];
+    // :264-288, each line as written: (c*dt)*dudt added last
+    for (int c = tid; c < nx; c += blockDim.x) {
+      cell_residual<N>(R.d, G, T, c, L);
+      for (int q = 0; q < 3 * N; ++q) R.w1[M * c + q] = R.d[M * c + q] + k[0] * dt * L[q];
+    }
+    __syncthreads();
+    for (int c = tid; c < nx; c += blockDim.x) {
+      cell_residual<N>(R.w1, G, T, c, L);
+      for (int q = 0; q < 3 * N; ++q)
+        R.w2[M * c + q] = k[1] * R.d[M * c + q] + k[2] * R.w1[M * c + q] + k[3] * dt * L[q];
+    }
+    __syncthreads();
+    for (int c = tid; c < nx; c += blockDim.x) {
+      cell_residual<N>(R.w2, G, T, c, L);
+      for (int q = 0; q < 3 * N; ++q)
+        R.w3[M * c + q] = k[4] * R.d[M * c + q] + k[5] * R.w2[M * c + q] + k[6] * dt * L[q];
+    }
+    __syncthreads();
+    for (int c = tid; c < nx; c += blockDim.x) {
+      cell_residual<N>(R.w3, G, T, c, L);
+      for (int q = 0; q < 3 * N; ++q) {
+        R.w4[M * c + q] = k[7] * R.d[M * c + q] + k[8] * R.w3[M * c + q] + k[9] * dt * L[q];
+        R.acc[M * c + q] = k[10] * R.w2[M * c + q] + k[11] * R.w3[M * c + q] + k[12] * dt * L[q];
+      }
+    }
+    __syncthreads();
+    for (int c = tid; c < nx; c += blockDim.x) {
+      cell_residual<N>(R.w4, G, T, c, L);
+      for (int q = 0; q < 3 * N; ++q)
+        R.d[M * c + q] = R.acc[M * c + q] + k[13] * R.w4[M * c + q] + k[14] * dt * L[q];
+    }
+    __syncthreads();
+    // uinit = u_eq + the modes at legendre(x_quad) (:303-330), own cell only
+    for (int c = tid; c < nx; c += blockDim.x)
+      for (int i = 0; i < N; ++i)
+        for (int v = 0; v < 3; ++v) {
+          double a = 0.0;
+          for (int q = 0; q < N; ++q) a = a + R.d[M * c + 3 * q + v] * R.phx[((size_t)c * N + i) * N + q];
+          R.uinit[M * c + 3 * i + v] = G.ueq[M * c + 3 * i + v] + a;
+        }
+    __syncthreads();
+    t = t + dt;
+    dtl = dt;
+    ++steps;
+  }
+  if (tid == 0) {
+    R.clk->t = t;
+    R.clk->dt_last = dtl;
+    R.clk->steps = steps;
+    R.clk->err = err;
+  }
+}
+
 }  // namespace dg1d
 
 using namespace dg1d;
@@ -258,7 +380,12 @@ struct fvwb_dg1d {
   double* d_ueq = nullptr;    // equilibrium at the Gauss points, AoS(3,n,nx)
   double* d_dudt = nullptr;
   double* d_face = nullptr;   // ufe then ffe, 3*(nx+1) each
+  double* d_run = nullptr;    // the time loop's modes, w1..w4, acc and uinit
+  double* d_phx = nullptr;    // phi_k at the physical x of each Gauss point, (nx, n, n)
+  Clock* d_clk = nullptr;
+  bool loaded = false;
   Tables tab{};
+  double coef[15];
   std::string err;
 };
 
@@ -294,7 +421,7 @@ extern "C" const char* fvwb_dg1d_last_error(fvwb_dg1d_handle h) { return h ? h->
 extern "C" int fvwb_dg1d_destroy(fvwb_dg1d_handle h) {
   if (!h) return FVWB_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* ps[] = {h->d_du, h->d_ueq, h->d_dudt, h->d_face};
+  void* ps[] = {h->d_du, h->d_ueq, h->d_dudt, h->d_face, h->d_run, h->d_phx, h->d_clk};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -346,12 +473,35 @@ extern "C" int fvwb_dg1d_create(fvwb_dg1d_handle* out, const fvwb_dg1d_params* p
     fv1d::conservative(p->gamma, wf, &face[3 * i]);
     fv1d::flux(p->gamma, &face[3 * i], &face[3 * nf + 3 * i]);
   }
+  // phi_k at x_quad = xcell + dx/2*chsi_i, the physical coordinate the main
+  // program's reconstruction passes to legendre (:314-323)
+  const size_t nphx = (size_t)nx * n * n;
+  double* phx = new (std::nothrow) double[nphx];
+  if (!phx) {
+    delete[] face;
+    delete h;
+    return faildg1d(nullptr, FVWB_ERR_OOM, "host allocation failed");
+  }
+  for (int c = 0; c < nx; ++c) {
+    const double xcell = ((double)(c + 1) - 0.5) * dx;
+    for (int i = 0; i < n; ++i)
+      for (int q = 0; q < n; ++q) phx[((size_t)c * n + i) * n + q] = legendre(xcell + dx / 2.0 * chsi[i], q);
+  }
+  static const float rki[15] = {0.391752226571890f, 0.444370493651235f, 0.555629506348765f, 0.368410593050371f,
+                                0.620101851488403f, 0.379898148511597f, 0.251891774271694f, 0.178079954393132f,
+                                0.821920045606868f, 0.544974750228521f, 0.517231671970585f, 0.096059710526147f,
+                                0.063692468666290f, 0.386708617503269f, 0.226007483236906f};
+  for (int i = 0; i < 15; ++i) h->coef[i] = (double)rki[i];
   const size_t bytes = sizeof(double) * 3 * (size_t)n * (size_t)nx;
   bool ok = hipStreamCreate(&h->stream) == hipSuccess && hipMalloc(&h->d_du, bytes) == hipSuccess &&
             hipMalloc(&h->d_ueq, bytes) == hipSuccess && hipMalloc(&h->d_dudt, bytes) == hipSuccess &&
             hipMalloc(&h->d_face, sizeof(double) * 6 * nf) == hipSuccess &&
-            hipMemcpy(h->d_face, face, sizeof(double) * 6 * nf, hipMemcpyHostToDevice) == hipSuccess;
+            hipMalloc(&h->d_run, 7 * bytes) == hipSuccess && hipMalloc(&h->d_phx, sizeof(double) * nphx) == hipSuccess &&
+            hipMalloc(&h->d_clk, sizeof(Clock)) == hipSuccess &&
+            hipMemcpy(h->d_face, face, sizeof(double) * 6 * nf, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(h->d_phx, phx, sizeof(double) * nphx, hipMemcpyHostToDevice) == hipSuccess;
   delete[] face;
+  delete[] phx;
   if (!ok) {
     fvwb_dg1d_destroy(h);
     return faildg1d(nullptr, FVWB_ERR_OOM, "device allocation failed");
@@ -360,13 +510,25 @@ extern "C" int fvwb_dg1d_create(fvwb_dg1d_handle* out, const fvwb_dg1d_params* p
   return FVWB_OK;
 }
 
+static Geometry geometry(const fvwb_dg1d* h, const double* ueq) {
+  const fvwb_dg1d_params& p = h->p;
+  const size_t nf = (size_t)p.nx + 1;
+  Geometry G;
+  G.ueq = ueq;
+  G.ufe = h->d_face;
+  G.ffe = h->d_face + 3 * nf;
+  G.nx = p.nx;
+  G.riemann = p.riemann;
+  G.source = p.source;
+  G.gamma = p.gamma;
+  G.oneoverdx = 1. / (p.boxlen / (double)p.nx);
+  return G;
+}
+
 template <int N>
 static void launch_update(fvwb_dg1d* h) {
-  const fvwb_dg1d_params& p = h->p;
-  const double dx = p.boxlen / (double)p.nx;
-  const size_t nf = (size_t)p.nx + 1;
-  hipLaunchKernelGGL(dg1d_update_kernel<N>, dim3((p.nx + 255) / 256), dim3(256), 0, h->stream, h->d_du, h->d_ueq,
-                     h->d_face, h->d_face + 3 * nf, h->d_dudt, p.nx, p.gamma, 1. / dx, p.riemann, p.source, h->tab);
+  hipLaunchKernelGGL(dg1d_update_kernel<N>, dim3((h->p.nx + 255) / 256), dim3(256), 0, h->stream, h->d_du,
+                     geometry(h, h->d_ueq), h->tab, h->d_dudt);
 }
 
 extern "C" int fvwb_dg1d_compute_update_exact_delta(fvwb_dg1d_handle h, const double* delta_u, const double* u_eq,
@@ -386,5 +548,90 @@ extern "C" int fvwb_dg1d_compute_update_exact_delta(fvwb_dg1d_handle h, const do
   HIPD(h, hipGetLastError());
   HIPD(h, hipMemcpyAsync(dudt, h->d_dudt, bytes, hipMemcpyDeviceToHost, h->stream));
   HIPD(h, hipStreamSynchronize(h->stream));
+  return FVWB_OK;
+}
+
+// ---- the time loop ----------------------------------------------------------------
+static double* run_array(fvwb_dg1d* h, int k) {
+  return h->d_run + (size_t)k * 3 * (size_t)h->p.n * (size_t)h->p.nx;
+}
+
+extern "C" int fvwb_dg1d_upload(fvwb_dg1d_handle h, const double* delta_u, const double* u_eq, const double* u_init) {
+  if (!h || !delta_u || !u_eq || !u_init) return faildg1d(h, FVWB_ERR_ARG, "NULL argument");
+  const size_t bytes = sizeof(double) * 3 * (size_t)h->p.n * (size_t)h->p.nx;
+  HIPD(h, hipMemcpyAsync(run_array(h, 0), delta_u, bytes, hipMemcpyHostToDevice, h->stream));
+  HIPD(h, hipMemcpyAsync(h->d_ueq, u_eq, bytes, hipMemcpyHostToDevice, h->stream));
+  HIPD(h, hipMemcpyAsync(run_array(h, 6), u_init, bytes, hipMemcpyHostToDevice, h->stream));
+  const Clock zero{0.0, 0.0, 0, 0};
+  HIPD(h, hipMemcpyAsync(h->d_clk, &zero, sizeof zero, hipMemcpyHostToDevice, h->stream));
+  HIPD(h, hipStreamSynchronize(h->stream));
+  h->loaded = true;
+  return FVWB_OK;
+}
+
+template <int N>
+static void launch_run(fvwb_dg1d* h, int32_t limit, double t_end) {
+  RunArgs R;
+  R.d = run_array(h, 0);
+  R.w1 = run_array(h, 1);
+  R.w2 = run_array(h, 2);
+  R.w3 = run_array(h, 3);
+  R.w4 = run_array(h, 4);
+  R.acc = run_array(h, 5);
+  R.uinit = run_array(h, 6);
+  R.phx = h->d_phx;
+  R.clk = h->d_clk;
+  R.step_limit = limit;
+  R.t_end = t_end;
+  R.dx = h->p.boxlen / (double)h->p.nx;
+  for (int i = 0; i < 15; ++i) R.coef[i] = h->coef[i];
+  hipLaunchKernelGGL(dg1d_run_kernel<N>, dim3(1), dim3(256), 0, h->stream, geometry(h, h->d_ueq), h->tab, R);
+}
+
+// Steps are launched in chunks of at most this many, so one launch stays short.
+static constexpr int32_t kChunk = 2048;
+
+extern "C" int fvwb_dg1d_advance(fvwb_dg1d_handle h, int32_t nsteps, double t_end) {
+  if (!h) return faildg1d(h, FVWB_ERR_ARG, "NULL handle");
+  if (!h->loaded) return faildg1d(h, FVWB_ERR_STATE, "advance before upload");
+  if (nsteps < 0) return faildg1d(h, FVWB_ERR_ARG, "nsteps must be >= 0");
+  Clock clk;
+  HIPD(h, hipMemcpy(&clk, h->d_clk, sizeof clk, hipMemcpyDeviceToHost));
+  const int64_t target = (int64_t)clk.steps + nsteps;
+  for (;;) {
+    if (clk.err) return faildg1d(h, FVWB_ERR_NONPHYSICAL, "wave-speed maximum or dt not finite and positive");
+    if (!(clk.t < t_end) || clk.steps >= target) return FVWB_OK;
+    const int32_t limit = (int32_t)std::min<int64_t>(target, (int64_t)clk.steps + kChunk);
+    switch (h->p.n) {
+      case 1: launch_run<1>(h, limit, t_end); break;
+      case 2: launch_run<2>(h, limit, t_end); break;
+      case 3: launch_run<3>(h, limit, t_end); break;
+      case 4: launch_run<4>(h, limit, t_end); break;
+      case 5: launch_run<5>(h, limit, t_end); break;
+      default: launch_run<6>(h, limit, t_end); break;
+    }
+    HIPD(h, hipGetLastError());
+    HIPD(h, hipMemcpyAsync(&clk, h->d_clk, sizeof clk, hipMemcpyDeviceToHost, h->stream));
+    HIPD(h, hipStreamSynchronize(h->stream));
+  }
+}
+
+extern "C" int fvwb_dg1d_download(fvwb_dg1d_handle h, double* delta_u, double* u_init) {
+  if (!h || !delta_u) return faildg1d(h, FVWB_ERR_ARG, "NULL argument");
+  if (!h->loaded) return faildg1d(h, FVWB_ERR_STATE, "download before upload");
+  const size_t bytes = sizeof(double) * 3 * (size_t)h->p.n * (size_t)h->p.nx;
+  HIPD(h, hipMemcpyAsync(delta_u, run_array(h, 0), bytes, hipMemcpyDeviceToHost, h->stream));
+  if (u_init) HIPD(h, hipMemcpyAsync(u_init, run_array(h, 6), bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPD(h, hipStreamSynchronize(h->stream));
+  return FVWB_OK;
+}
+
+extern "C" int fvwb_dg1d_get_time(fvwb_dg1d_handle h, double* t, double* dt_last, int32_t* steps) {
+  if (!h) return faildg1d(h, FVWB_ERR_ARG, "NULL handle");
+  Clock clk;
+  HIPD(h, hipMemcpy(&clk, h->d_clk, sizeof clk, hipMemcpyDeviceToHost));
+  if (t) *t = clk.t;
+  if (dt_last) *dt_last = clk.dt_last;
+  if (steps) *steps = clk.steps;
   return FVWB_OK;
 }

@@ -192,6 +192,10 @@ def lib():
         "fvwb_dg1d_create": [ctypes.POINTER(H), pd1],
         "fvwb_dg1d_destroy": [H],
         "fvwb_dg1d_compute_update_exact_delta": [H, dp, dp, dp],
+        "fvwb_dg1d_upload": [H, dp, dp, dp],
+        "fvwb_dg1d_advance": [H, ctypes.c_int32, ctypes.c_double],
+        "fvwb_dg1d_download": [H, dp, dp],
+        "fvwb_dg1d_get_time": [H, dp, dp, ip],
     })
     p1 = ctypes.POINTER(Params1D)
     sig.update({
@@ -755,3 +759,24 @@ class SolverDG1D:
         out = np.zeros(self.shape, order="F")
         _checkdg1d(lib().fvwb_dg1d_compute_update_exact_delta(self._h, _dp(d), _dp(e), _dp(out)), self._h)
         return out
+
+    def upload(self, delta_u, u_eq, u_init):
+        """Modes, equilibrium and the nodal state of the first dt, all AoS(3,n,nx)."""
+        d, e, u = self._arr(delta_u), self._arr(u_eq), self._arr(u_init)
+        _checkdg1d(lib().fvwb_dg1d_upload(self._h, _dp(d), _dp(e), _dp(u)), self._h)
+
+    def advance(self, nsteps=(1 << 31) - 1, t_end=math.inf):
+        """The main program's 'RKi' loop while t < t_end, at most nsteps steps."""
+        _checkdg1d(lib().fvwb_dg1d_advance(self._h, int(nsteps), float(t_end)), self._h)
+
+    def download(self):
+        """(modes, u_init) after the last step."""
+        d, u = np.zeros(self.shape, order="F"), np.zeros(self.shape, order="F")
+        _checkdg1d(lib().fvwb_dg1d_download(self._h, _dp(d), _dp(u)), self._h)
+        return d, u
+
+    def time(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int32()
+        _checkdg1d(lib().fvwb_dg1d_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), self._h)
+        return t.value, dt.value, n.value

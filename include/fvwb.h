@@ -402,6 +402,23 @@ const char* fvwb_dg1d_last_error(fvwb_dg1d_handle h);
  * reference's expressions). */
 int fvwb_dg1d_compute_update_exact_delta(fvwb_dg1d_handle h, const double* delta_u, const double* u_eq,
                                          double* dudt);
+/* The main program's time loop with its shipped integrator 'RKi'
+ * (dg_with_source.f90:191-199, 264-288, 303-334), resident on the device:
+ * upload the modes, u_eq and u_init (the nodal state the first dt is taken
+ * from: the main program's initial Gauss-point values); each step takes
+ * dt = 0.9*dx/cmax/(2n+1) with cmax the largest |v|+c over the cells at the
+ * first Gauss point of u_init, runs the five SSP(5,4) stages of
+ * compute_update_exact_delta with the reference's FP32-rounded coefficients,
+ * then sets u_init = u_eq + the modes evaluated, as the reference does, at
+ * legendre(x_quad) with x_quad the point's physical coordinate.  advance runs
+ * while t < t_end, at most nsteps more steps; it returns FVWB_ERR_NONPHYSICAL
+ * and stops the clock if cmax or dt stops being finite and positive.  Equal to
+ * the reference's run bit for bit (every t and dt). */
+int fvwb_dg1d_upload(fvwb_dg1d_handle h, const double* delta_u, const double* u_eq, const double* u_init);
+int fvwb_dg1d_advance(fvwb_dg1d_handle h, int32_t nsteps, double t_end);
+/* the modes and (when u_init is not NULL) the last u_init */
+int fvwb_dg1d_download(fvwb_dg1d_handle h, double* delta_u, double* u_init);
+int fvwb_dg1d_get_time(fvwb_dg1d_handle h, double* t, double* dt_last, int32_t* steps_done);
 
 #ifdef __cplusplus
 }

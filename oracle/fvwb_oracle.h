@@ -152,6 +152,15 @@ void   orc_dg1d_tables(int n, double* chsi, double* wq, double* phi_q, double* d
 /* compute_update_exact_delta (:1749-2031); modes and u_eq AoS(3,n,nx) */
 void   orc_dg1d_update_exact_delta(const fvwb_dg1d_params* p, const double* delta_u, const double* u_eq,
                                    double* dudt);
+/* the 'RKi' SSP(5,4) coefficients as FP32 literals (:264-288) */
+void   orc_dg1d_rki_coeffs(double c[15]);
+/* the main program's set-up for ninit 7 or 8 (:27-31, 105-125, 148-167) */
+void   orc_dg1d_initial(const fvwb_dg1d_params* p, int ninit, double pert, double* delta_u, double* u_eq,
+                        double* u_init);
+/* the main loop with integrator 'RKi' (:191-199, 264-288, 303-334); returns the
+ * number of steps, updates delta_u and u_init (the nodal reconstruction) */
+int    orc_dg1d_run_rki(const fvwb_dg1d_params* p, double* delta_u, const double* u_eq, double* u_init,
+                        double t_end, int max_steps, double* t_out, double* ts, double* dts);
 
 #ifdef __cplusplus
 }

@@ -217,3 +217,107 @@ void orc_dg1d_update_exact_delta(const fvwb_dg1d_params* p, const double* du, co
     }
   free(ufe); free(ffe); free(ff); free(ul); free(ur);
 }
+
+/* the SSP(5,4) coefficients of the 'RKi' branch (dg_with_source.f90:264-288) as
+ * its default-real literals round them, in the order
+ * c10; c20 c21 c22; c30 c32 c33; c40 c43 c44; c52 c53 c54; c55 c56 */
+void orc_dg1d_rki_coeffs(double c[15])
+{
+  static const float k[15] = {0.391752226571890f, 0.444370493651235f, 0.555629506348765f, 0.368410593050371f,
+                              0.620101851488403f, 0.379898148511597f, 0.251891774271694f, 0.178079954393132f,
+                              0.821920045606868f, 0.544974750228521f, 0.517231671970585f, 0.096059710526147f,
+                              0.063692468666290f, 0.386708617503269f, 0.226007483236906f};
+  for (int i = 0; i < 15; ++i) c[i] = (double)k[i];
+}
+
+/* The main program's set-up (dg_with_source.f90:27-31, 105-125, 148-167) for
+ * condinit cases 7 (rest) and 8 (rest plus a pressure pulse of size pert at
+ * boxlen/2): u_eq and u_init at each cell's Gauss points xcell + dx/2*chsi_j,
+ * and the initial modes sum_j 0.5*(u_init - u_eq)*phi_i(chsi_j)*w_j. */
+void orc_dg1d_initial(const fvwb_dg1d_params* p, int ninit, double pert, double* du, double* ueq, double* uinit)
+{
+  const int nx = p->nx, n = p->n;
+  const double gamma = p->gamma;
+  const double dx = p->boxlen / (double)nx;
+  double chsi[8], wq[8], phq[64], dphq[64], plr[16];
+  orc_dg1d_tables(n, chsi, wq, phq, dphq, plr);
+  for (int c = 0; c < nx; ++c) {
+    const double xcell = ((double)(c + 1) - 0.5) * dx;
+    for (int j = 0; j < n; ++j) {
+      const double x = xcell + dx / 2.0 * chsi[j];
+      const double we[3] = {exp(-x), 0, exp(-x)};
+      double wi[3] = {exp(-x), 0., exp(-x)};
+      if (ninit == 8) wi[2] = exp(-x) + pert * exp(-(100 * ipow(x - p->boxlen / 2., 2)));
+      orc_ref1d_conservative(gamma, we, &V3(ueq, 0, j, c, n));
+      orc_ref1d_conservative(gamma, wi, &V3(uinit, 0, j, c, n));
+    }
+    for (int i = 0; i < n; ++i)
+      for (int v = 0; v < 3; ++v) {
+        double a = 0.0;
+        for (int j = 0; j < n; ++j)
+          a = a + 0.5 * (V3(uinit, v, j, c, n) - V3(ueq, v, j, c, n)) * phq[i * n + j] * wq[j];
+        V3(du, v, i, c, n) = a;
+      }
+  }
+}
+
+/* The main loop with integrator 'RKi' (dg_with_source.f90:191-199, 264-288,
+ * 303-334): dt = 0.9*dx/cmax/(2n+1) with cmax over cells of |v|+c at the first
+ * Gauss point of uinit; the five stages on the modes; then uinit = u_eq plus
+ * the modes evaluated at legendre(x_quad) with x_quad the point's PHYSICAL
+ * coordinate xcell + dx/2*chsi_i, as written (:314-323); t = t + dt.  Runs
+ * while t < t_end and fewer than max_steps steps were taken; writes each
+ * step's t and dt to ts and dts when given; uinit is updated in place. */
+int orc_dg1d_run_rki(const fvwb_dg1d_params* p, double* du, const double* ueq, double* uinit, double t_end,
+                     int max_steps, double* t_out, double* ts, double* dts)
+{
+  const int nx = p->nx, n = p->n;
+  const size_t m = 3 * (size_t)n * (size_t)nx;
+  const double dx = p->boxlen / (double)nx;
+  double c[15], chsi[8], wq[8], phq[64], dphq[64], plr[16];
+  orc_dg1d_rki_coeffs(c);
+  orc_dg1d_tables(n, chsi, wq, phq, dphq, plr);
+  double* L = malloc(sizeof(double) * m);
+  double* w1 = malloc(sizeof(double) * m);
+  double* w2 = malloc(sizeof(double) * m);
+  double* w3 = malloc(sizeof(double) * m);
+  double* w4 = malloc(sizeof(double) * m);
+  double t = 0;
+  int it = 0;
+  while (t < t_end && it < max_steps) {
+    double cmax = 0.0;
+    for (int k = 0; k < nx; ++k) cmax = fmax(cmax, orc_ref1d_speed(p->gamma, &V3(uinit, 0, 0, k, n)));
+    const double dt = (double)0.9f * dx / cmax / (2.0 * (double)n + 1.0);
+    orc_dg1d_update_exact_delta(p, du, ueq, L);
+    for (size_t k = 0; k < m; ++k) w1[k] = du[k] + c[0] * dt * L[k];
+    orc_dg1d_update_exact_delta(p, w1, ueq, L);
+    for (size_t k = 0; k < m; ++k) w2[k] = c[1] * du[k] + c[2] * w1[k] + c[3] * dt * L[k];
+    orc_dg1d_update_exact_delta(p, w2, ueq, L);
+    for (size_t k = 0; k < m; ++k) w3[k] = c[4] * du[k] + c[5] * w2[k] + c[6] * dt * L[k];
+    orc_dg1d_update_exact_delta(p, w3, ueq, L);
+    for (size_t k = 0; k < m; ++k) {
+      w4[k] = c[7] * du[k] + c[8] * w3[k] + c[9] * dt * L[k];
+      du[k] = c[10] * w2[k] + c[11] * w3[k] + c[12] * dt * L[k];
+    }
+    orc_dg1d_update_exact_delta(p, w4, ueq, L);
+    for (size_t k = 0; k < m; ++k) du[k] = du[k] + c[13] * w4[k] + c[14] * dt * L[k];
+    for (int k = 0; k < nx; ++k) {
+      const double xcell = ((double)(k + 1) - 0.5) * dx;
+      for (int i = 0; i < n; ++i) {
+        const double xq = xcell + dx / 2.0 * chsi[i];
+        for (int v = 0; v < 3; ++v) {
+          double a = 0.0;
+          for (int q = 0; q < n; ++q) a = a + V3(du, v, q, k, n) * orc_dg1d_legendre(xq, q);
+          V3(uinit, v, i, k, n) = V3(ueq, v, i, k, n) + a;
+        }
+      }
+    }
+    t = t + dt;
+    if (ts) ts[it] = t;
+    if (dts) dts[it] = dt;
+    ++it;
+  }
+  free(L); free(w1); free(w2); free(w3); free(w4);
+  if (t_out) *t_out = t;
+  return it;
+}

@@ -177,6 +177,11 @@ def lib():
         L.orc_dg1d_gl_quadrature.argtypes = [ctypes.c_int, dp, dp]
         L.orc_dg1d_tables.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
         L.orc_dg1d_update_exact_delta.argtypes = [ctypes.POINTER(DG1DParams), dp, dp, dp]
+        L.orc_dg1d_rki_coeffs.argtypes = [dp]
+        L.orc_dg1d_initial.argtypes = [ctypes.POINTER(DG1DParams), ctypes.c_int, ctypes.c_double, dp, dp, dp]
+        L.orc_dg1d_run_rki.argtypes = [ctypes.POINTER(DG1DParams), dp, dp, dp, ctypes.c_double, ctypes.c_int, dp,
+                                       dp, dp]
+        L.orc_dg1d_run_rki.restype = ctypes.c_int
         L.orc_ref_dg2d_modes_from_nodes.argtypes = [ctypes.c_int] * 3 + [dp, dp]
         L.orc_ref_dg2d_nodes_from_modes.argtypes = [ctypes.c_int] * 3 + [dp, dp]
         L.orc_dg2d_nmodes.argtypes = [ctypes.c_int]
@@ -534,6 +539,31 @@ def dg1d_update_exact_delta(p: DG1DParams, delta_u, u_eq):
     out = np.zeros_like(delta_u, order="F")
     lib().orc_dg1d_update_exact_delta(ctypes.byref(p), _p(delta_u), _p(u_eq), _p(out))
     return out
+
+
+def dg1d_rki_coeffs():
+    c = np.zeros(15)
+    lib().orc_dg1d_rki_coeffs(_p(c))
+    return c
+
+
+def dg1d_initial(p: DG1DParams, ninit=8, pert=float(np.float32(1e-8))):
+    """(delta_u, u_eq, u_init) of the reference main program's set-up, AoS(3,n,nx)."""
+    shape = (3, p.n, p.nx)
+    d, e, u = (np.zeros(shape, order="F") for _ in range(3))
+    lib().orc_dg1d_initial(ctypes.byref(p), int(ninit), float(pert), _p(d), _p(e), _p(u))
+    return d, e, u
+
+
+def dg1d_run_rki(p: DG1DParams, delta_u, u_eq, u_init, t_end, max_steps=1 << 30):
+    """The main loop with 'RKi'; returns (delta_u, u_init, t, steps, ts, dts)."""
+    d, u = fortran(delta_u).copy(order="F"), fortran(u_init).copy(order="F")
+    e = fortran(u_eq)
+    cap = int(min(max_steps, 1 << 20))
+    ts, dts, t = np.zeros(cap), np.zeros(cap), ctypes.c_double()
+    k = lib().orc_dg1d_run_rki(ctypes.byref(p), _p(d), _p(e), _p(u), float(t_end), cap, ctypes.byref(t),
+                               _p(ts), _p(dts))
+    return d, u, t.value, k, ts[:k], dts[:k]
 
 
 def ref_dg2d_modes_from_nodes(nodes):

@@ -148,7 +148,9 @@ def dg1d_fixtures():
     n = 0..6 on a grid with the clamp, gl_quadrature for n = 1..6, and
     compute_update_exact_delta (dg_with_source.f90:1749-2031) as compiled (n = 3,
     nx = 128, HLLC, source 2) on its own initial modes, on random modes of size
-    1e-2, on strong modes with flow, and on zero modes."""
+    1e-2, on strong modes with flow, and on zero modes; and the reference
+    program's own run to tend = 0.2 (integrator 'RKi'): every step's t and dt,
+    its final error and its recon99999.dat."""
     assert rr.available("dg1d"), "build oracle/_ref first"
     x = np.concatenate([np.linspace(-1, 1, 41), [-1.5, 1.5]])
     r = rr.run("dg1d_legendre", x=x)
@@ -172,6 +174,8 @@ def dg1d_fixtures():
         upd["du_" + k] = d
         upd["dudt_" + k] = rr.run("dg1d_update_exact_delta", delta_u=d, u_eq=ue)["dudt"]
     out["dg1d_update_exact_delta_128"] = upd
+    assert rr.available("dg1d_program"), "build oracle/_ref first"
+    out["dg1d_run_rki_128"] = rr.run_dg1d_program()
     return out
 
 

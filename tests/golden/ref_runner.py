@@ -37,6 +37,8 @@ LIBS = {
 
 
 def available(name="bench2d") -> bool:
+    if name == "dg1d_program":
+        return os.path.exists(os.path.join(REF_DIR, "ref_dg1d"))
     return os.path.exists(os.path.join(REF_DIR, LIBS[name]))
 
 
@@ -50,6 +52,24 @@ def run(op: str, **arrays):
                            stdout=dn, check=True)
         with np.load(fout) as z:
             return {k: z[k] for k in z.files}
+
+
+def run_dg1d_program():
+    """Run the reference's 1D DG program (oracle/_ref/ref_dg1d, dg_commons.f90's
+    set-up: n = 3, nx = 128, HLLC, ninit = 8, integrator 'RKi', tend = 0.2) in an
+    empty directory and return what it reports: the time and dt of every step
+    (its stdout, :196-199 and :333), the final 'error' (:406), and the two columns
+    of recon99999.dat (:341-349: the first Gauss point's x and p - p_eq there)."""
+    import re
+    with tempfile.TemporaryDirectory() as td:
+        os.makedirs(os.path.join(td, "simul"))
+        r = subprocess.run([os.path.join(REF_DIR, "ref_dg1d")], cwd=td, capture_output=True, text=True,
+                           check=True, timeout=600)
+        rows = re.findall(r"time=\s+(\d+)\s+(\S+)\s+(\S+)", r.stdout)
+        err = re.findall(r"error\s+(\S+)", r.stdout)
+        recon = np.loadtxt(os.path.join(td, "recon99999.dat"))
+    return {"steps": np.array([int(a) for a, _, _ in rows]), "t": np.array([float(b) for _, b, _ in rows]),
+            "dt": np.array([float(c) for _, _, c in rows]), "error": np.array(float(err[-1])), "recon": recon}
 
 
 # ---- child side ------------------------------------------------------------------

@@ -157,3 +157,62 @@ def test_update_exact_delta_live_against_reference(seed):
     du = np.asfortranarray(np.random.default_rng(seed).normal(size=(3, 3, 128)) * 0.05)
     ref = rr.run("dg1d_update_exact_delta", delta_u=du, u_eq=g["u_eq"])["dudt"]
     np.testing.assert_array_equal(po.dg1d_update_exact_delta(po.make_dg1d_params(128), du, g["u_eq"]), ref)
+
+
+def final_error(p, u_init):
+    """The reference program's closing 'error' (:391-406): max over cells of
+    |p - p_eq| at the first Gauss point of the last reconstruction."""
+    chsi, _ = po.dg1d_gl_quadrature(p.n)
+    dx = p.boxlen / p.nx
+    x = (np.arange(1, p.nx + 1) - 0.5) * dx + dx / 2.0 * chsi[0]
+    U = u_init[:, 0, :]
+    v = U[1] / U[0]
+    pr = (p.gamma - 1.0) * (U[2] - 0.5 * U[0] * (v * v))
+    return np.max(np.abs(pr - np.exp(-x))), x, pr - np.exp(-x)
+
+
+def test_run_rki_matches_reference_program_bitwise():
+    """The reference's own 1D DG run (dg_commons.f90: n = 3, nx = 128, HLLC,
+    ninit = 8 with pert = 1e-8, integrator 'RKi', tend = 0.2): the oracle's
+    restatement of the main program takes the same 236 steps with every t and
+    dt equal bit for bit, ends at the same 'error' bit for bit, and writes the
+    same recon99999.dat to its six printed digits."""
+    g = load("dg1d_run_rki_128")
+    p = po.make_dg1d_params(128)
+    d, e, u = po.dg1d_initial(p)
+    d, u, t, k, ts, dts = po.dg1d_run_rki(p, d, e, u, 0.2)
+    assert k == len(g["t"]) == 236
+    np.testing.assert_array_equal(ts, g["t"])
+    np.testing.assert_array_equal(dts, g["dt"])
+    err, x, dp = final_error(p, u)
+    assert err == float(g["error"])
+    np.testing.assert_allclose(x, g["recon"][:, 0], rtol=6e-6)
+    # p - p_eq at six digits; numpy's exp in final_error may differ from libm's by
+    # an ulp of the O(1) pressure, hence the absolute part
+    np.testing.assert_allclose(dp, g["recon"][:, 1], rtol=6e-6, atol=4 * EPS)
+
+
+def test_rki_coefficients_are_the_ssp54_scheme():
+    """The fifteen literals (rounded to FP32) satisfy the order conditions of a
+    fourth-order method to FP32 accuracy when written as a Butcher tableau."""
+    c = po.dg1d_rki_coeffs()
+    A = np.zeros((5, 5))
+    b = np.zeros(5)
+    # Shu-Osher form u^(i) = sum_k alpha_ik u^(k) + beta_ik dt L(u^(k)), rows of
+    # (alpha, beta) in terms of stage vectors u0..u4
+    alpha = np.zeros((6, 5)); beta = np.zeros((6, 5))
+    alpha[1, 0], beta[1, 0] = 1.0, c[0]
+    alpha[2, 0], alpha[2, 1], beta[2, 1] = c[1], c[2], c[3]
+    alpha[3, 0], alpha[3, 2], beta[3, 2] = c[4], c[5], c[6]
+    alpha[4, 0], alpha[4, 3], beta[4, 3] = c[7], c[8], c[9]
+    alpha[5, 2], alpha[5, 3], beta[5, 3], alpha[5, 4], beta[5, 4] = c[10], c[11], c[12], c[13], c[14]
+    # Butcher coefficients: K[i] = sum_k alpha_ik K[k] + beta_ik e_k
+    K = np.zeros((6, 5))
+    for i in range(1, 6):
+        K[i] = sum(alpha[i, k] * K[k] for k in range(i)) + beta[i]
+    A, b = K[:5], K[5]
+    cc = A.sum(axis=1)
+    for lhs, rhs in [(b.sum(), 1), (b @ cc, 1 / 2), (b @ cc ** 2, 1 / 3), (b @ A @ cc, 1 / 6),
+                     (b @ cc ** 3, 1 / 4), (b @ (cc * (A @ cc)), 1 / 8), (b @ A @ cc ** 2, 1 / 12),
+                     (b @ A @ A @ cc, 1 / 24)]:
+        assert abs(lhs - rhs) < 1e-6, (lhs, rhs)
