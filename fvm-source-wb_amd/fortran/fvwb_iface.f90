@@ -39,6 +39,19 @@ module fvwb_iface
   integer(c_int32_t), parameter, public :: FVWB1D_BC_PERIODIC = 1, FVWB1D_BC_ZERO_GRADIENT = 2, &
        FVWB1D_BC_REFLEXIVE = 3, FVWB1D_BC_WALL = 4
 
+  ! ---- 1D modal DG (dg_with_source.f90) ----
+  integer(c_int32_t), parameter, public :: FVWB_DG1D_RIEMANN_LLF = 1, FVWB_DG1D_RIEMANN_HLLC = 2
+
+  ! struct fvwb_dg1d_params: six int32 then two doubles (24 + 16 bytes)
+  type, bind(C), public :: fvwb_dg1d_params
+     integer(c_int32_t) :: nvar, nx, n, riemann, source, pad
+     real(c_double) :: gamma, boxlen
+  end type fvwb_dg1d_params
+
+  public :: fvwb_dg1d_params_reference, fvwb_dg1d_create, fvwb_dg1d_destroy, fvwb_dg1d_last_error
+  public :: fvwb_dg1d_compute_update_exact_delta, fvwb_dg1d_upload, fvwb_dg1d_advance
+  public :: fvwb_dg1d_download, fvwb_dg1d_get_time, fvwb_dg1d_check
+
   ! struct fvwb1d_params: six int32 then four doubles (24 + 32 bytes)
   type, bind(C), public :: fvwb1d_params
      integer(c_int32_t) :: nvar, nx, bc, recon, integrator, nequilibrium
@@ -313,6 +326,64 @@ module fvwb_iface
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s
      end function c_strlen
+     ! ---- 1D modal DG ----
+     integer(c_int) function fvwb_dg1d_params_reference(p, nx) bind(C, name='fvwb_dg1d_params_reference')
+       import :: c_int, c_int32_t, fvwb_dg1d_params
+       type(fvwb_dg1d_params), intent(out) :: p
+       integer(c_int32_t), value :: nx
+     end function fvwb_dg1d_params_reference
+
+     integer(c_int) function fvwb_dg1d_create(h, p) bind(C, name='fvwb_dg1d_create')
+       import :: c_int, c_ptr, fvwb_dg1d_params
+       type(c_ptr), intent(out) :: h
+       type(fvwb_dg1d_params), intent(in) :: p
+     end function fvwb_dg1d_create
+
+     integer(c_int) function fvwb_dg1d_destroy(h) bind(C, name='fvwb_dg1d_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_dg1d_destroy
+
+     type(c_ptr) function fvwb_dg1d_last_error(h) bind(C, name='fvwb_dg1d_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_dg1d_last_error
+
+     ! drop-in for dg_with_source.f90:1749-2031 (compute_update_exact_delta(delta_u,u_eq,dudt))
+     integer(c_int) function fvwb_dg1d_compute_update_exact_delta(h, delta_u, u_eq, dudt) &
+          bind(C, name='fvwb_dg1d_compute_update_exact_delta')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: delta_u(*), u_eq(*)
+       real(c_double), intent(out) :: dudt(*)
+     end function fvwb_dg1d_compute_update_exact_delta
+
+     ! the main program's 'RKi' loop (dg_with_source.f90:191-334), resident on the device
+     integer(c_int) function fvwb_dg1d_upload(h, delta_u, u_eq, u_init) bind(C, name='fvwb_dg1d_upload')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: delta_u(*), u_eq(*), u_init(*)
+     end function fvwb_dg1d_upload
+
+     integer(c_int) function fvwb_dg1d_advance(h, nsteps, t_end) bind(C, name='fvwb_dg1d_advance')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nsteps
+       real(c_double), value :: t_end
+     end function fvwb_dg1d_advance
+
+     integer(c_int) function fvwb_dg1d_download(h, delta_u, u_init) bind(C, name='fvwb_dg1d_download')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: delta_u(*), u_init(*)
+     end function fvwb_dg1d_download
+
+     integer(c_int) function fvwb_dg1d_get_time(h, t, dt_last, steps) bind(C, name='fvwb_dg1d_get_time')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: t, dt_last
+       integer(c_int32_t), intent(out) :: steps
+     end function fvwb_dg1d_get_time
   end interface
 
 contains
@@ -370,5 +441,17 @@ contains
        error stop 1
     end if
   end subroutine fvwb1d_check
+
+  ! Same for a 1D DG handle.
+  subroutine fvwb_dg1d_check(status, h, what)
+    integer(c_int), intent(in) :: status
+    type(c_ptr), intent(in) :: h
+    character(len=*), intent(in) :: what
+    if (status /= FVWB_OK) then
+       write(0, '(a,a,a,i0,a,a)') 'fvwb: ', what, ' failed with status ', status, ': ', &
+            fvwb_message(fvwb_dg1d_last_error(h))
+       error stop 1
+    end if
+  end subroutine fvwb_dg1d_check
 
 end module fvwb_iface

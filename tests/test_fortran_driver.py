@@ -102,6 +102,16 @@ def test_fvm_driver_binds_dropins_to_library():
         assert sym in undefined and sym in fvwb.declared_symbols()
 
 
+def test_dg1d_driver_binds_dropin_to_library():
+    nm = subprocess.run(["nm", exe("dg1d_fvwb")], capture_output=True, text=True, check=True).stdout
+    defined = set(re.findall(r"^[0-9a-f]+ T (\S+)$", nm, re.M))
+    undefined = set(re.findall(r"^\s+U (\S+)$", nm, re.M))
+    assert "compute_update_exact_delta_" in defined
+    for sym in ("fvwb_dg1d_compute_update_exact_delta", "fvwb_dg1d_create", "fvwb_dg1d_upload",
+                "fvwb_dg1d_advance"):
+        assert sym in undefined and sym in fvwb.declared_symbols()
+
+
 def test_driver_without_device_fails_loudly():
     """Without a usable GPU the driver stops with the library's error instead
     of computing anything on the host."""
@@ -319,3 +329,35 @@ def test_1d_driver_resident_run_matches_oracle_bitwise(tmp_path):
     assert np.array_equal(u, uo)
     lines = open(prefix + "_final.dat").read().splitlines()
     assert len(lines) == 128 and all(re.fullmatch(r"(?:[ -]\d\.\d{5}E[+-]\d\d ?){2}", ln) for ln in lines)
+
+
+def dg1d_golden():
+    return np.load(os.path.join(ROOT, "tests", "golden", "dg1d_run_rki_128.npz"))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["stage", "resident"])
+def test_dg1d_driver_reproduces_reference_program(tmp_path, mode):
+    """The 1D DG program (n = 3, nx = 128, HLLC, ninit = 8, 'RKi', tend = 0.2)
+    with the drop-in residual (stage) or the device-resident loop (resident):
+    the same 236 steps as the reference program, every printed t and dt (stage)
+    or the last t (resident) equal to the reference's bit for bit, the same
+    printed 'error', and recon99999.dat equal to the reference's."""
+    g = dg1d_golden()
+    prefix = str(tmp_path) + "/"
+    r = run([exe("dg1d_fvwb"), mode, "0.2", prefix])
+    assert r.returncode == 0, r.stdout + r.stderr
+    err = float(re.findall(r"error\s+(\S+)", r.stdout)[-1])
+    steps, n, nvar = None, 3, 3
+    with open(prefix + ".bin", "rb") as f:
+        nx, n, nvar, steps = np.frombuffer(f.read(16), dtype=np.int32)
+        t, dt = np.frombuffer(f.read(16), dtype=np.float64)
+    print(f"{mode}: steps {steps}, t {t!r}, error {err!r}", flush=True)
+    assert (nx, n, nvar) == (128, 3, 3) and steps == 236
+    assert t == g["t"][-1] and dt == g["dt"][-1]
+    assert err == float(g["error"])
+    if mode == "stage":
+        rows = re.findall(r"time=\s+(\d+)\s+(\S+)\s+(\S+)", r.stdout)
+        np.testing.assert_array_equal([float(b) for _, b, _ in rows], g["t"])
+        np.testing.assert_array_equal([float(c) for _, _, c in rows], g["dt"])
+    np.testing.assert_array_equal(np.loadtxt(prefix + "recon99999.dat"), g["recon"])
