@@ -416,7 +416,9 @@ struct LimArgs {
   long long plane;
   int pitch, nx, ny, parity;
   int nrec;            // bytes from a variable's mode-0 cell (0,0) to the end of its six planes
-  int limit, cfl, bookkeeping, init;
+  int limit, posit, cfl, bookkeeping, init;
+  const float* eqa;    // A_n(x cell) [3*nx] and B_n(y cell) [3*ny]: equilibrium modes
+  const float* eqb;    //   are rho0 A_a B_b (density) and eover A_a B_b (energy)
   double t_end, cfl_over;
   float gamma, odx, ody, mdx2, mdy2;
   float rho0, eover;   // equilibrium rho and E per e^{-k(x+y)}
@@ -429,6 +431,96 @@ __device__ inline float gminmod(float x, float y, float z, float mh2) {
   if (signbit(y) != sx || signbit(z) != sx) return 0.f;
   const float m = fminf(fminf(fabsf(x), fabsf(y)), fabsf(z));
   return sx ? -m : m;
+}
+
+// solve_for_t (2d/limiters.f90:312-362) in FP64: the t in [0,1] at which the
+// pressure of (1-t) avg + t u reaches eps
+__device__ inline float solve_for_t(double gamma, double eps, const float uf[4], const float af[4]) {
+  const double pa = af[0], mxa = af[1], mya = af[2], ea = af[3];
+  const double pj = uf[0], mxj = uf[1], myj = uf[2], ej = uf[3];
+  double qa = 2.0 * (pj - pa) * (ej - ea) - (mxj - mxa) * (mxj - mxa) - (myj - mya) * (myj - mya);
+  double qb = 2.0 * (pj - pa) * (ea - eps / (gamma - 1)) + 2.0 * pa * (ej - ea) -
+              2.0 * (mxa * (mxj - mxa) + mya * (myj - mya));
+  double qc = 2.0 * pa * ea - (mxa * mxa + mya * mya) - 2.0 * eps * pa / (gamma - 1.0);
+  qb = qb / qa;
+  qc = qc / qa;
+  const double D = sqrt(fabs(qb * qb - 4 * qc));
+  const double t1 = 0.5 * (-qb - D), t2 = 0.5 * (-qb + D);
+  double t = 0.0;
+  if (t1 > -eps && t1 < 1.0 + eps) t = t1;
+  else if (t2 > -eps && t2 < 1.0 + eps) t = t2;
+  return (float)fmax(0.0, fmin(1.0, t));
+}
+
+// Positivity limiter (compute_positivity, 2d/limiters.f90:478-654) on the full
+// state F = E + md of one cell: E the equilibrium's modes from the separable
+// tables; the set is {GLL x GL} u {GL x GLL} u {GL x GL}; thresholds 1e-5 of the
+// cell's equilibrium density and pressure.  Marks each variable it rewrites.
+__device__ inline void positivity_cell(float md[4][NM], bool chg[4], const LimArgs& a, int i, int j) {
+  const float ax[3] = {a.eqa[3 * i], a.eqa[3 * i + 1], a.eqa[3 * i + 2]};
+  const float by[3] = {a.eqb[3 * j], a.eqb[3 * j + 1], a.eqb[3 * j + 2]};
+  constexpr int MI_[NM] = {0, 1, 0, 2, 1, 0}, MJ_[NM] = {0, 0, 1, 0, 1, 2};
+  const float cv[4] = {a.rho0, 0.f, 0.f, a.eover};
+  float E[4][NM], F[4][NM];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      E[v][m] = cv[v] * (ax[MI_[m]] * by[MJ_[m]]);
+      F[v][m] = E[v][m] + md[v][m];
+    }
+  const float gm1 = a.gamma - 1.0f;
+  const float eps_r = fmaxf(1e-10f, 1e-5f * E[0][0]);
+  const float eps_p = fmaxf(1e-10f, 1e-5f * gm1 * E[3][0]);
+  // phi_1, phi_2 at the Gauss-Lobatto points -1, 0, 1 and at the Gauss points
+  const float s3 = 1.7320508075688772f, s5 = 2.2360679774997896f;
+  const float g1[3] = {-s3, 0.f, s3}, g2[3] = {s5, -0.5f * s5, s5};
+  auto val = [&](int v, float p1x, float p2x, float p1y, float p2y) {
+    return F[v][0] + F[v][1] * p1x + F[v][2] * p1y + F[v][3] * p2x + F[v][4] * (p1x * p1y) + F[v][5] * p2y;
+  };
+  // set point k in 0..26: q = k/9, c = (k/3)%3, side = k%3
+  auto point = [&](int k, float& p1x, float& p2x, float& p1y, float& p2y) {
+    const int q = k / 9, c = (k / 3) % 3, side = k % 3;
+    if (side == 0) { p1x = g1[c]; p2x = g2[c]; p1y = a.phi1[q]; p2y = a.phi2[q]; }
+    else if (side == 1) { p1x = a.phi1[q]; p2x = a.phi2[q]; p1y = g1[c]; p2y = g2[c]; }
+    else { p1x = a.phi1[q]; p2x = a.phi2[q]; p1y = a.phi1[c]; p2y = a.phi2[c]; }
+  };
+  float rmin = INFINITY;
+  for (int k = 0; k < 27; ++k) {
+    float p1x, p2x, p1y, p2y;
+    point(k, p1x, p2x, p1y, p2y);
+    rmin = fminf(rmin, val(0, p1x, p2x, p1y, p2y));
+  }
+  const float theta = fminf(fabsf((F[0][0] - eps_r) / (F[0][0] - rmin)), 1.f);
+  if (theta < 1.f) {
+#pragma unroll
+    for (int m = 1; m < NM; ++m) {
+      F[0][m] = theta * F[0][m];
+      md[0][m] = F[0][m] - E[0][m];
+    }
+    chg[0] = true;
+  }
+  const float avg[4] = {F[0][0], F[1][0], F[2][0], F[3][0]};
+  float tmin = 1.f;
+  for (int k = 0; k < 27; ++k) {
+    float p1x, p2x, p1y, p2y;
+    point(k, p1x, p2x, p1y, p2y);
+    float u[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) u[v] = val(v, p1x, p2x, p1y, p2y);
+    const float r = fmaxf(u[0], 1e-9f);
+    const float vx = u[1] / r, vy = u[2] / r;
+    const float pr = gm1 * (u[3] - 0.5f * r * (vx * vx + vy * vy));
+    if (!(pr > eps_p)) tmin = fminf(tmin, solve_for_t(a.gamma, eps_p, u, avg));
+  }
+  if (tmin < 1.f) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+#pragma unroll
+      for (int m = 1; m < NM; ++m) md[v][m] = tmin * F[v][m] - E[v][m];
+      chg[v] = true;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
@@ -452,12 +544,13 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
   __amdgpu_buffer_rsrc_t rv[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) rv[v] = mode_rsrc(a.st + (long long)(v * NM) * a.plane, a.nrec);
-  // the limiter needs modes 0-2 only; the CFL rate needs all six
+  // TVB needs modes 0-2 only; the CFL rate and the positivity limiter need all six
   float md[4][NM];
 #pragma unroll
   for (int v = 0; v < 4; ++v)
 #pragma unroll
-    for (int m = 0; m < NM; ++m) md[v][m] = (m < 3 || a.cfl) ? ldm(rv[v], off(in, ob, m)) : 0.f;
+    for (int m = 0; m < NM; ++m) md[v][m] = (m < 3 || a.cfl || a.posit) ? ldm(rv[v], off(in, ob, m)) : 0.f;
+  bool chg[4] = {false, false, false, false};
 
   if (a.limit) {
     const float s3 = 1.7320508075688772f;
@@ -479,10 +572,16 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
         md[v][1] = xt * (1.0f / 1.7320508075688772f);
         md[v][2] = yt * (1.0f / 1.7320508075688772f);
         md[v][3] = md[v][4] = md[v][5] = 0.f;
+        chg[v] = true;
       }
-#pragma unroll
-      for (int m = 1; m < NM; ++m) stm(rv[v], off(lim, ob, m), md[v][m]);
     }
+  }
+  if (a.posit && in) positivity_cell(md, chg, a, i, j);
+  if (a.limit || a.posit) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int m = 1; m < NM; ++m) stm(rv[v], off(chg[v], ob, m), md[v][m]);
   }
 
   float rate = 0.f;
@@ -601,6 +700,7 @@ struct fvwb_dg {
   double* host_stage = nullptr;   // device FP64 AoS staging
   double *exn = nullptr, *eyn = nullptr, *exf = nullptr, *eyf = nullptr;
   float *exn32 = nullptr, *eyn32 = nullptr, *exf32 = nullptr, *eyf32 = nullptr;
+  float *eqa32 = nullptr, *eqb32 = nullptr;   // 1D Gauss projections of e^{-kx}, e^{-ky} per cell
   float* cell(int b) const { return buf[b] + org; }
   Consts* C = nullptr;
   Ctrl* ctrl = nullptr;
@@ -626,7 +726,7 @@ extern "C" const char* fvwb_dg_last_error(fvwb_dg_handle h) { return h ? h->err.
 extern "C" int fvwb_dg_destroy(fvwb_dg_handle h) {
   if (!h) return FVWB_OK;
   void* ps[] = {h->buf[0], h->buf[1], h->buf[2], h->buf[3], h->host_stage, h->exn, h->eyn, h->exf, h->eyf,
-                h->exn32, h->eyn32, h->exf32, h->eyf32, h->C, h->ctrl};
+                h->exn32, h->eyn32, h->exf32, h->eyf32, h->eqa32, h->eqb32, h->C, h->ctrl};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -642,7 +742,7 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   if (p->degree != 2) return dg_fail(nullptr, FVWB_ERR_ARG, "the device path implements degree 2 (P2)");
   if (p->flux != FVWB_FLUX_RUSANOV && p->flux != FVWB_FLUX_HLL && p->flux != FVWB_FLUX_HLLC)
     return dg_fail(nullptr, FVWB_ERR_ARG, "bad flux");
-  if (p->limiter != FVWB_DG_LIMITER_NONE && p->limiter != FVWB_DG_LIMITER_TVB)
+  if (p->limiter < FVWB_DG_LIMITER_NONE || p->limiter > FVWB_DG_LIMITER_TVB_POSITIVITY)
     return dg_fail(nullptr, FVWB_ERR_ARG, "bad limiter");
   if (p->bc != FVWB_BC_REFLECT) return dg_fail(nullptr, FVWB_ERR_ARG, "the DG path implements walls (FVWB_BC_REFLECT)");
   if (p->integrator != FVWB_DG_SSPRK3 && p->integrator != FVWB_DG_SSPRK54)
@@ -678,6 +778,7 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
       hipMalloc(&h->eyf, sizeof(double) * (ny + 1)) != hipSuccess || hipMalloc(&h->C, sizeof(Consts)) != hipSuccess ||
       hipMalloc(&h->exn32, sizeof(float) * 3 * nx) != hipSuccess || hipMalloc(&h->eyn32, sizeof(float) * 3 * ny) != hipSuccess ||
       hipMalloc(&h->exf32, sizeof(float) * (nx + 1)) != hipSuccess || hipMalloc(&h->eyf32, sizeof(float) * (ny + 1)) != hipSuccess ||
+      hipMalloc(&h->eqa32, sizeof(float) * 3 * nx) != hipSuccess || hipMalloc(&h->eqb32, sizeof(float) * 3 * ny) != hipSuccess ||
       hipMalloc(&h->ctrl, sizeof(Ctrl)) != hipSuccess)
     return bail(FVWB_ERR_OOM, "device allocation failed");
   // padding columns and rows stay zero for the whole life of the handle
@@ -698,6 +799,22 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
     for (int b = 0; b < 3; ++b) ey[3 * j + b] = std::exp(-(kk * ((j + 0.5) * dy + 0.5 * dy * h->xq[b])));
   for (int i = 0; i <= nx; ++i) fx[i] = std::exp(-(kk * (i * dx)));
   for (int j = 0; j <= ny; ++j) fy[j] = std::exp(-(kk * (j * dy)));
+  // the equilibrium's modes for the positivity limiter: U_eq is separable, so
+  // its Gauss projection on mode (a, b) is c_v A_a(x cell) B_b(y cell) with
+  // A_a = 1/2 sum_q w_q e^{-k x_q} phi_a(xi_q)
+  std::vector<float> eqa(3 * nx), eqb(3 * ny);
+  for (int i = 0; i < nx; ++i)
+    for (int n = 0; n < 3; ++n) {
+      double s = 0.0;
+      for (int q = 0; q < 3; ++q) s += 0.5 * h->wq[q] * ex[3 * i + q] * leg(h->xq[q], n);
+      eqa[3 * i + n] = (float)s;
+    }
+  for (int j = 0; j < ny; ++j)
+    for (int n = 0; n < 3; ++n) {
+      double s = 0.0;
+      for (int q = 0; q < 3; ++q) s += 0.5 * h->wq[q] * ey[3 * j + q] * leg(h->xq[q], n);
+      eqb[3 * j + n] = (float)s;
+    }
   Consts C;
   std::memset(&C, 0, sizeof C);
   const double* xq = h->xq;
@@ -756,6 +873,8 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
       hipMemcpy(h->eyn32, ey32.data(), sizeof(float) * ey32.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->exf32, fx32.data(), sizeof(float) * fx32.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->eyf32, fy32.data(), sizeof(float) * fy32.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->eqa32, eqa.data(), sizeof(float) * eqa.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->eqb32, eqb.data(), sizeof(float) * eqb.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemsetAsync(h->ctrl, 0, sizeof(Ctrl), h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess)
     return bail(FVWB_ERR_HIP, "copying tables failed");
   *out = h;
@@ -791,7 +910,9 @@ static LimArgs lim_args(fvwb_dg* h, float* st) {
   const double dx = p.boxlen_x / p.nx, dy = p.boxlen_y / p.ny;
   a.mdx2 = (float)(p.tvb_m * (dx * dx)); a.mdy2 = (float)(p.tvb_m * (dy * dy));
   a.rho0 = (float)p.rho0; a.eover = (float)(p.p0 / (p.gamma - 1.0));
-  a.limit = p.limiter == FVWB_DG_LIMITER_TVB;
+  a.limit = p.limiter == FVWB_DG_LIMITER_TVB || p.limiter == FVWB_DG_LIMITER_TVB_POSITIVITY;
+  a.posit = p.limiter == FVWB_DG_LIMITER_POSITIVITY || p.limiter == FVWB_DG_LIMITER_TVB_POSITIVITY;
+  a.eqa = h->eqa32; a.eqb = h->eqb32;
   a.t_end = INFINITY;
   for (int q = 0; q < 3; ++q) {
     a.phi1[q] = (float)leg(h->xq[q], 1);
@@ -926,24 +1047,24 @@ static int enqueue_step(fvwb_dg* h, double t_end) {
     a.in = A; a.un = nullptr; a.out = B; a.alpha = 0.f; a.beta = 1.f; a.dtc = c[0]; a.first = 1;
     launch_stage(h, a);
     la.st = h->cell(1); la.cfl = 0; la.bookkeeping = 0;
-    if (la.limit) launch_limit(h, la);
+    if (la.limit || la.posit) launch_limit(h, la);
     // w2 = a20 u + a21 (w1 + (c21/a21) dt L(w1))
     a.in = B; a.un = A; a.out = Cc; a.alpha = c[1]; a.beta = c[2]; a.dtc = (double)c[3] / c[2]; a.first = 0;
     launch_stage(h, a);
     la.st = h->cell(2);
-    if (la.limit) launch_limit(h, la);
+    if (la.limit || la.posit) launch_limit(h, la);
     // w3 = a30 u + a32 (w2 + (c32/a32) dt L(w2)), over w1
     a.in = Cc; a.un = A; a.out = B; a.alpha = c[4]; a.beta = c[5]; a.dtc = (double)c[6] / c[5];
     launch_stage(h, a);
     la.st = h->cell(1);
-    if (la.limit) launch_limit(h, la);
+    if (la.limit || la.posit) launch_limit(h, la);
     // w4 = a40 u + a43 w3 + c43 dt L(w3) and P = a50 u + a52 w2 + a53 w3 + c53 dt L(w3)
     a.in = B; a.un = A; a.out = D; a.alpha = c[7]; a.beta = c[8]; a.dtc = 1.0;
     a.xin = Cc; a.xout = Cc; a.gam = c[9]; a.pu = c[10]; a.px = c[11]; a.pin = c[12]; a.pl = c[14];
     launch_stage(h, a);
     a.xin = nullptr; a.xout = nullptr;
     la.st = h->cell(3);
-    if (la.limit) launch_limit(h, la);
+    if (la.limit || la.posit) launch_limit(h, la);
     // u = P + a54 (w4 + (c54/a54) dt L(w4))
     a.in = D; a.un = Cc; a.out = A; a.alpha = 1.f; a.beta = c[13]; a.dtc = (double)c[15] / c[13];
     launch_stage(h, a);
@@ -957,12 +1078,12 @@ static int enqueue_step(fvwb_dg* h, double t_end) {
   a.in = A; a.un = nullptr; a.out = B; a.alpha = 0.f; a.beta = 1.f; a.first = 1;
   launch_stage(h, a);
   la.st = h->cell(1); la.cfl = 0; la.bookkeeping = 0;
-  if (la.limit) launch_limit(h, la);
+  if (la.limit || la.posit) launch_limit(h, la);
   // stage 2
   a.in = B; a.un = A; a.out = Cc; a.alpha = 0.75f; a.beta = 0.25f; a.first = 0;
   launch_stage(h, a);
   la.st = h->cell(2);
-  if (la.limit) launch_limit(h, la);
+  if (la.limit || la.posit) launch_limit(h, la);
   // stage 3 (in place over U^n: each owned cell's U^n is read by the lane that writes it, before the write)
   a.in = Cc; a.un = A; a.out = A; a.alpha = (float)(1.0 / 3.0); a.beta = (float)(2.0 / 3.0);
   launch_stage(h, a);

@@ -315,6 +315,22 @@ int fvwb1d_get_stream(fvwb1d_handle h, void** stream);
 #define FVWB_DG_LIMITER_TVB   1   /* 2d/limiters.f90:30-54 generalized_minmod on the
                                      linear modes of each conservative deviation;
                                      a limited cell drops its quadratic modes   */
+#define FVWB_DG_LIMITER_POSITIVITY 2   /* compute_positivity, 2d/limiters.f90:478-654
+                                     (limiter_type 'ONP', the reference's shipped
+                                     limiter) on the full state U_eq + dU: on the
+                                     set {Gauss-Lobatto x Gauss} u {Gauss x
+                                     Gauss-Lobatto} u {the volume Gauss points},
+                                     the non-mean density modes are scaled by
+                                     min(|(rho_avg - eps_r)/(rho_avg - rho_min)|,
+                                     1), then every non-mean mode by the smallest
+                                     root of solve_for_t (:312-362) over the points
+                                     whose pressure is not above eps_p; eps_r and
+                                     eps_p are 1e-5 of the cell's equilibrium
+                                     density and pressure (the reference's 1e-10 is
+                                     below what an FP32 deviation resolves); the
+                                     equilibrium's own modes are its Gauss
+                                     projection; means never change            */
+#define FVWB_DG_LIMITER_TVB_POSITIVITY 3   /* TVB, then positivity              */
 #define FVWB_DG_SSPRK3        0   /* Shu-Osher SSP-RK3 (dg_with_source.f90:194-204)  */
 #define FVWB_DG_SSPRK54       1   /* five-stage fourth-order SSP Runge-Kutta of
                                      2d/benchmark_2d_dg.f90:683-710 (solver 'RK4',
@@ -344,8 +360,8 @@ const char* fvwb_dg_last_error(fvwb_dg_handle h);
  * counterpart of compute_update (2d/benchmark_2d_dg.f90:1137-1479) for the
  * deviation form, P2 basis and the face flux of p.flux. */
 int fvwb_dg_compute_residual(fvwb_dg_handle h, const double* modes, double* dudt);
-/* Device state: upload deviation modes (limited on arrival when p.limiter is
- * TVB), or build them from a pressure perturbation
+/* Device state: upload deviation modes (limited on arrival unless p.limiter is
+ * FVWB_DG_LIMITER_NONE), or build them from a pressure perturbation
  * dp = amp * sum_q a[q] cos(2 pi (mk[q] x + nk[q] y) + theta[q]) / sum_q |a[q]|
  * (dE = dp/(gamma-1)) projected on P2; the clock restarts at 0. */
 int fvwb_dg_upload(fvwb_dg_handle h, const double* modes);

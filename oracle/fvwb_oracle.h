@@ -130,6 +130,11 @@ void   orc_dg_gl_quadrature(int n, double* xq, double* wq);
 /* 2d/benchmark_2d_dg.f90:497-592 tensor-basis projections, AoS(4,nx,ny,order,order) */
 void   orc_ref_dg2d_modes_from_nodes(int nx, int ny, int order, const double* nodes, double* modes);
 void   orc_ref_dg2d_nodes_from_modes(int nx, int ny, int order, const double* modes, double* nodes);
+/* 2d/limiters.f90:312-654: the reference's positivity limiter (limiter_type
+ * 'ONP', the shipped one) as written, on its tensor modes AoS(4,nx,ny,o,o) */
+void   orc_ref_dg2d_gll_points(int n, double* x);
+double orc_ref_dg2d_solve_for_t(double gamma, double eps, const double u[4], const double u_avg[4]);
+void   orc_ref_dg2d_positivity(int nx, int ny, int order, double gamma, double eps, double* u);
 /* configured scheme; modes AoS(4,nx,ny,nmodes) of the deviation */
 int    orc_dg2d_nmodes(int degree);
 void   orc_dg2d_residual(const fvwb_dg_params* p, const double* d, double* L);

@@ -282,9 +282,98 @@ void orc_dg2d_residual(const fvwb_dg_params* p, const double* d, double* L)
 
 /* TVB limiter on the linear modes of each conservative deviation; the means
  * are never changed, so cells can be limited in any order. */
+/* The positivity limiter of the configured scheme: compute_positivity
+ * (2d/limiters.f90:478-654) applied to the full state's modes F = E + d, E the
+ * (k+1)x(k+1) Gauss projection of U_eq on the cell's P_k basis.  The set is the
+ * reference's compute_set for k+1 Gauss points, (x_gll, y_gl) and (x_gl, y_gll)
+ * with the k+1 Gauss-Lobatto points (-1, 0, 1 for k = 2; -1, 1 for k = 1), plus
+ * the volume Gauss points, where the scheme evaluates the state too.  The
+ * thresholds are the cell's: 1e-5 of its equilibrium mean density and pressure
+ * (at least the reference's 1e-10), which an FP32 deviation resolves.  A
+ * variable whose modes the limiter leaves alone keeps its deviation bit for bit. */
+static void positivity_cell(const fvwb_dg_params* p, const dg_basis* B, int i, int j, double* d)
+{
+  const int nx = p->nx, ny = p->ny, nm = B->nm, nq = B->nq;
+  const double dx = p->boxlen_x / nx, dy = p->boxlen_y / ny;
+  double E[4][6], F[4][6];
+  for (int v = 0; v < 4; ++v)
+    for (int m = 0; m < nm; ++m) E[v][m] = 0.0;
+  for (int b = 0; b < nq; ++b)
+    for (int a = 0; a < nq; ++a) {
+      double ue[4];
+      ueq_at(p, (i + 0.5) * dx + 0.5 * dx * B->xq[a], (j + 0.5) * dy + 0.5 * dy * B->xq[b], ue);
+      for (int m = 0; m < nm; ++m) {
+        const double w = 0.25 * B->wq[a] * B->wq[b] * B->phi[B->mi[m]][a] * B->phi[B->mj[m]][b];
+        for (int v = 0; v < 4; ++v) E[v][m] += w * ue[v];
+      }
+    }
+  for (int v = 0; v < 4; ++v)
+    for (int m = 0; m < nm; ++m) F[v][m] = E[v][m] + DM(d, v, i, j, m, nx, ny);
+  const double eps_r = fmax((double)1e-10f, 1e-5 * E[0][0]);
+  const double eps_p = fmax((double)1e-10f, 1e-5 * (p->gamma - 1.0) * E[3][0]);
+  /* the set: basis values per direction at Gauss-Lobatto (g) and Gauss (q) points */
+  double lg[3][3], lq[3][3], xg[3];
+  if (nq == 2) { xg[0] = -1.0; xg[1] = 1.0; } else { xg[0] = -1.0; xg[1] = 0.0; xg[2] = 1.0; }
+  for (int c = 0; c < nq; ++c)
+    for (int n = 0; n <= B->k; ++n) { lg[c][n] = orc_dg_legendre(xg[c], n); lq[c][n] = B->phi[n][c]; }
+  const int np = 3 * nq * nq;
+  double pts[27][4];
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int q = 0; q < nq; ++q)
+      for (int c = 0; c < nq; ++c)
+        for (int side = 0; side < 3; ++side) {
+          const double* lx = side == 0 ? lg[c] : lq[q];
+          const double* ly = side == 0 ? lq[q] : (side == 1 ? lg[c] : lq[c]);
+          double* o = pts[(q * nq + c) + side * nq * nq];
+          for (int v = 0; v < 4; ++v) {
+            double s = 0.0;
+            for (int m = 0; m < nm; ++m) s += F[v][m] * (lx[B->mi[m]] * ly[B->mj[m]]);
+            o[v] = s;
+          }
+        }
+    if (pass == 0) {
+      double rmin = pts[0][0];
+      for (int k = 1; k < np; ++k) rmin = fmin(rmin, pts[k][0]);
+      const double theta = fmin(fabs((F[0][0] - eps_r) / (F[0][0] - rmin)), 1.0);
+      if (theta < 1.0) {
+        for (int m = 1; m < nm; ++m) {
+          F[0][m] = theta * F[0][m];
+          DM(d, 0, i, j, m, nx, ny) = F[0][m] - E[0][m];
+        }
+      }
+    } else {
+      const double avg[4] = {F[0][0], F[1][0], F[2][0], F[3][0]};
+      double tmin = 1.0;
+      for (int k = 0; k < np; ++k) {
+        const double r = fmax(pts[k][0], (double)10e-10f);
+        const double vx = pts[k][1] / r, vy = pts[k][2] / r;
+        const double pr = (p->gamma - 1.0) * (pts[k][3] - 0.5 * r * (vx * vx + vy * vy));
+        const double t = pr > eps_p ? 1.0 : orc_ref_dg2d_solve_for_t(p->gamma, eps_p, pts[k], avg);
+        tmin = fmin(tmin, t);
+      }
+      if (tmin < 1.0)
+        for (int v = 0; v < 4; ++v)
+          for (int m = 1; m < nm; ++m) DM(d, v, i, j, m, nx, ny) = tmin * F[v][m] - E[v][m];
+    }
+  }
+}
+
 void orc_dg2d_limit(const fvwb_dg_params* p, double* d)
 {
-  if (p->limiter != FVWB_DG_LIMITER_TVB || p->degree < 1) return;
+  if (p->degree < 1) return;
+  if (p->limiter == FVWB_DG_LIMITER_POSITIVITY || p->limiter == FVWB_DG_LIMITER_TVB_POSITIVITY) {
+    if (p->limiter == FVWB_DG_LIMITER_TVB_POSITIVITY) {
+      fvwb_dg_params q = *p;
+      q.limiter = FVWB_DG_LIMITER_TVB;
+      orc_dg2d_limit(&q, d);
+    }
+    dg_basis B;
+    make_basis(p->degree, &B);
+    for (int j = 0; j < p->ny; ++j)
+      for (int i = 0; i < p->nx; ++i) positivity_cell(p, &B, i, j, d);
+    return;
+  }
+  if (p->limiter != FVWB_DG_LIMITER_TVB) return;
   const int nx = p->nx, ny = p->ny, nm = orc_dg2d_nmodes(p->degree);
   const double dx = p->boxlen_x / nx, dy = p->boxlen_y / ny;
   const double s3 = sqrt(3.0);
@@ -305,6 +394,109 @@ void orc_dg2d_limit(const fvwb_dg_params* p, double* d)
           DM(d, v, i, j, 1, nx, ny) = at / s3;
           DM(d, v, i, j, 2, nx, ny) = bt / s3;
           for (int m = 3; m < nm; ++m) DM(d, v, i, j, m, nx, ny) = 0.0;
+        }
+      }
+}
+
+/* ---- the reference's positivity limiter (2d/limiters.f90:312-654), as written ---- */
+/* 2d/legendre.f90:111-170 gll_quadrature: n = 2 gives the points (-1, 0), n = 3
+ * (-1, 0, 1), as written; only the points are used by compute_set */
+void orc_ref_dg2d_gll_points(int n, double* x)
+{
+  if (n == 2) { x[0] = -1.; x[1] = 0.; }
+  else if (n == 3) { x[0] = -1.; x[1] = 0.; x[2] = 1.; }
+  else for (int i = 0; i < n; ++i) x[i] = NAN;
+}
+
+/* 2d/limiters.f90:312-362 solve_for_t: the root in [0,1] of the quadratic whose
+ * zero puts the pressure of (1-t) u_avg + t u at eps, written with the
+ * reference's operation order (a, b, c, then b/a, c/a); no root in range gives 0 */
+double orc_ref_dg2d_solve_for_t(double gamma, double eps, const double u[4], const double ua[4])
+{
+  const double pa = ua[0], mxa = ua[1], mya = ua[2], ea = ua[3];
+  const double pj = u[0], mxj = u[1], myj = u[2], ej = u[3];
+  double a = 2.0 * (pj - pa) * (ej - ea) - (mxj - mxa) * (mxj - mxa) - (myj - mya) * (myj - mya);
+  double b = 2.0 * (pj - pa) * (ea - eps / (gamma - 1)) + 2.0 * pa * (ej - ea) -
+             2.0 * (mxa * (mxj - mxa) + mya * (myj - mya));
+  double c = 2.0 * pa * ea - (mxa * mxa + mya * mya) - 2.0 * eps * pa / (gamma - 1.0);
+  b = b / a;
+  c = c / a;
+  const double D = sqrt(fabs(b * b - 4 * c));
+  const double t1 = 0.5 * (-b - D), t2 = 0.5 * (-b + D);
+  double t;
+  if (t1 > -eps && t1 < 1.0 + eps) t = t1;
+  else if (t2 > -eps && t2 < 1.0 + eps) t = t2;
+  else t = 0.0;
+  t = fmin(1.0, t);
+  t = fmax(0.0, t);
+  return t;
+}
+
+/* 2d/limiters.f90:438-475 compute_set for one cell's modes m(v,a,b) (order o,
+ * gll points): the values at (x_gll(c), y_gl(q)) then at (x_gl(q), x_gll(c)),
+ * point index q*gll + c and then + o*gll */
+static void ref_positivity_set(const double* m, int o, int gll, double* pts)
+{
+  double xq[8], wq[8], xg[8];
+  orc_dg_gl_quadrature(o, xq, wq);
+  orc_ref_dg2d_gll_points(gll, xg);
+  for (int q = 0; q < o; ++q)
+    for (int c = 0; c < gll; ++c) {
+      double ul[4] = {0.0, 0.0, 0.0, 0.0}, ur[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int i = 0; i < o; ++i)
+        for (int j = 0; j < o; ++j)
+          for (int v = 0; v < 4; ++v) {
+            const double mv = m[v + 4 * (i + o * j)];
+            ul[v] = ul[v] + mv * orc_dg_legendre(xg[c], i) * orc_dg_legendre(xq[q], j);
+            ur[v] = ur[v] + mv * orc_dg_legendre(xq[q], i) * orc_dg_legendre(xg[c], j);
+          }
+      for (int v = 0; v < 4; ++v) {
+        pts[v + 4 * (q * gll + c)] = ul[v];
+        pts[v + 4 * (q * gll + c + o * gll)] = ur[v];
+      }
+    }
+}
+
+/* 2d/limiters.f90:478-654 compute_positivity on the tensor modes
+ * u(1:4,1:nx,1:ny,1:o,1:o) of the reference's own DG, in place: density first,
+ * theta = min(|(rho_avg - eps)/(rho_avg - rho_min)|, 1) over the set, scaling
+ * the non-mean density modes; then, on the density-limited modes, the smallest
+ * solve_for_t over the set points whose pressure (density floored at 1e-9) is
+ * not above eps, scaling every non-mean mode.  gll = (2(o-1)+3)/2. */
+void orc_ref_dg2d_positivity(int nx, int ny, int o, double gamma, double eps, double* u)
+{
+  if (o == 1) return;
+  const int gll = (2 * (o - 1) + 3) / 2, np = 2 * o * gll;
+  double m[4 * 64], pts[4 * 64];
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) {
+        for (int a = 0; a < o; ++a)
+          for (int b = 0; b < o; ++b)
+            for (int v = 0; v < 4; ++v) m[v + 4 * (a + o * b)] = Q5(u, v, i, j, a, b, nx, ny, o);
+        ref_positivity_set(m, o, gll, pts);
+        const double avg[4] = {m[0], m[1], m[2], m[3]};
+        if (pass == 0) {
+          double pmin = pts[0];
+          for (int k = 1; k < np; ++k) pmin = fmin(pmin, pts[4 * k]);
+          const double theta = fmin(fabs((avg[0] - eps) / (avg[0] - pmin)), 1.0);
+          for (int a = 0; a < o; ++a)
+            for (int b = 0; b < o; ++b)
+              if (a != 0 || b != 0) Q5(u, 0, i, j, a, b, nx, ny, o) = theta * m[4 * (a + o * b)];
+        } else {
+          double tmin = 1.;
+          for (int k = 0; k < np; ++k) {
+            const double* pk = &pts[4 * k];
+            const double r = fmax(pk[0], (double)10e-10f);
+            const double vx = pk[1] / r, vy = pk[2] / r;
+            const double pr = (gamma - 1.0) * (pk[3] - 0.5 * r * (vx * vx + vy * vy));
+            const double t = pr > eps ? 1. : orc_ref_dg2d_solve_for_t(gamma, eps, pk, avg);
+            if (tmin >= t) tmin = t;
+          }
+          for (int a = 0; a < o; ++a)
+            for (int b = 0; b < o; ++b)
+              if (a != 0 || b != 0)
+                for (int v = 0; v < 4; ++v) Q5(u, v, i, j, a, b, nx, ny, o) = tmin * m[v + 4 * (a + o * b)];
         }
       }
 }

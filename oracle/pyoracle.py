@@ -80,7 +80,7 @@ def make_dg1d_params(nx=128, *, n=3, riemann=DG1D_RIEMANN_HLLC, source=2, gamma=
     return DG1DParams(3, nx, n, riemann, source, 0, gamma, boxlen)
 
 
-DG_LIMITER_NONE, DG_LIMITER_TVB = 0, 1
+DG_LIMITER_NONE, DG_LIMITER_TVB, DG_LIMITER_POSITIVITY, DG_LIMITER_TVB_POSITIVITY = 0, 1, 2, 3
 DG_SSPRK3, DG_SSPRK54 = 0, 1
 
 
@@ -170,6 +170,10 @@ def lib():
         L.orc_dg_legendre_prime.argtypes = [ctypes.c_double, ctypes.c_int]
         L.orc_dg_legendre_prime.restype = ctypes.c_double
         L.orc_dg_gl_quadrature.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_ref_dg2d_solve_for_t.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp]
+        L.orc_ref_dg2d_solve_for_t.restype = ctypes.c_double
+        L.orc_ref_dg2d_positivity.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_double, dp]
         L.orc_dg1d_legendre.argtypes = [ctypes.c_double, ctypes.c_int]
         L.orc_dg1d_legendre.restype = ctypes.c_double
         L.orc_dg1d_legendre_prime.argtypes = [ctypes.c_double, ctypes.c_int]
@@ -496,6 +500,19 @@ def dg_gl_quadrature(n):
     x, w = np.zeros(n), np.zeros(n)
     lib().orc_dg_gl_quadrature(n, _p(x), _p(w))
     return x, w
+
+
+def ref_dg2d_solve_for_t(gamma, eps, u, u_avg):
+    u, ua = np.ascontiguousarray(u, dtype=float), np.ascontiguousarray(u_avg, dtype=float)
+    return lib().orc_ref_dg2d_solve_for_t(float(gamma), float(eps), _p(u), _p(ua))
+
+
+def ref_dg2d_positivity(u, gamma=GAMMA_REF, eps=float(np.float32(1e-10))):
+    """compute_positivity (2d/limiters.f90:478-654) on tensor modes AoS(4,nx,ny,o,o)."""
+    u = fortran(u).copy(order="F")
+    _, nx, ny, o, _ = u.shape
+    lib().orc_ref_dg2d_positivity(nx, ny, o, float(gamma), float(eps), _p(u))
+    return u
 
 
 # ---- 1D modal DG (dg_with_source.f90, legendre.f90) -----------------------------------

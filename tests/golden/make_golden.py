@@ -179,6 +179,32 @@ def dg1d_fixtures():
     return out
 
 
+def positivity_fixture():
+    """compute_positivity (2d/limiters.f90:478-654, limiter_type 'ONP') as the
+    reference compiles it (8x8 cells, mx = my = 2, eps = 1e-10) on tensor modes
+    whose cells range from untouched to density- and pressure-limited."""
+    assert rr.available("dg2d"), "build oracle/_ref first"
+    rng = np.random.default_rng(478)
+    nx = ny = 8
+    o = 2
+    u = np.zeros((4, nx, ny, o, o), order="F")
+    rho = rng.uniform(0.5, 1.5, (nx, ny))
+    vx, vy = rng.uniform(-1, 1, (nx, ny)), rng.uniform(-1, 1, (nx, ny))
+    pr = rng.uniform(0.2, 1.0, (nx, ny))
+    u[0, :, :, 0, 0] = rho
+    u[1, :, :, 0, 0] = rho * vx
+    u[2, :, :, 0, 0] = rho * vy
+    u[3, :, :, 0, 0] = pr / 0.4 + 0.5 * rho * (vx ** 2 + vy ** 2)
+    scale = rng.choice([0.05, 0.3, 0.8], size=(nx, ny))
+    for v in range(4):
+        for a in range(o):
+            for b in range(o):
+                if a or b:
+                    u[v, :, :, a, b] = rng.normal(size=(nx, ny)) * scale * np.abs(u[v, :, :, 0, 0] + 0.3)
+    r = rr.run("dg2d_positivity", u=u)
+    return {"u": u, "u_limited": r["u"], "eps": r["eps"], "gamma": r["gamma"]}
+
+
 def hllc_fixture():
     """riemann_hllc (dg_with_source.f90:1318-1374) on 1D state pairs chosen to
     reach all four sampled regions (supersonic either way, star states either
@@ -214,7 +240,7 @@ def main(only=None):
     assert rr.available("bench2d") and rr.available("dg2d"), "build oracle/_ref first"
     out = {}
     single = {"dg1d_riemann_hllc": hllc_fixture, "bench2d_update_plain_300": plain_fixture,
-              "bench1d_update_sr_128": sr_fixture}
+              "bench1d_update_sr_128": sr_fixture, "dg2d_positivity": positivity_fixture}
     if only in single:
         out[only] = single[only]()
     elif only == "fvm1d":
@@ -303,6 +329,9 @@ def main(only=None):
 
     # the plain 1D residuals: benchmark_1d 'FVM' and fvm.f90
     out.update(fvm1d_fixtures())
+
+    # the reference's shipped 2D DG limiter, compute_positivity
+    out["dg2d_positivity"] = positivity_fixture()
 
     # the reference's 1D DG: basis, quadrature, compute_update_exact_delta
     out.update(dg1d_fixtures())

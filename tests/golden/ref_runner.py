@@ -247,6 +247,12 @@ def _child(op, fin, fout):
             sx, sy, ox, oy = (ctypes.c_int(v) for v in (nx, ny, o, o))
             fn(_dp(a), _dp(res), ctypes.byref(sx), ctypes.byref(sy), ctypes.byref(ox), ctypes.byref(oy))
             out["r"] = res
+        elif op == "dg2d_positivity":
+            u = np.asfortranarray(z["u"]).copy(order="F")
+            L.compute_positivity_(_dp(u))
+            out["u"] = u
+            out["eps"] = np.array(ctypes.c_double.in_dll(L, "_QMparameters_dg_2dEeps").value)
+            out["gamma"] = np.array(ctypes.c_double.in_dll(L, "_QMparameters_dg_2dEgamma").value)
         elif op == "dg2d_gl_quadrature":
             res = {}
             for n in range(1, 7):

@@ -174,6 +174,103 @@ def test_tvb_limiter_keeps_means_and_smooth_data():
     assert np.all(np.abs(a) <= np.maximum(np.abs(dxp[1:]), np.abs(dxp[:-1])) + 1e-15)
 
 
+# ---- the positivity limiter (2d/limiters.f90:312-654) ---------------------------------
+def test_reference_positivity_matches_golden_bitwise():
+    """compute_positivity as the reference compiles it (its shipped limiter
+    'ONP'; 8x8 cells of tensor modes, mx = my = 2): the oracle's restatement is
+    bit for bit the reference's output, on a state where most cells are limited."""
+    g = load("dg2d_positivity")
+    out = po.ref_dg2d_positivity(g["u"], gamma=float(g["gamma"]), eps=float(g["eps"]))
+    np.testing.assert_array_equal(out, g["u_limited"])
+    assert np.any(out != g["u"])
+    np.testing.assert_array_equal(out[:, :, :, 0, 0], g["u"][:, :, :, 0, 0])   # means untouched
+
+
+def test_solve_for_t_puts_the_pressure_at_eps():
+    """The root of solve_for_t moves a point of negative pressure towards the
+    cell mean until its pressure is eps (the quadratic of (1-t) u_avg + t u)."""
+    g = 1.4
+    eps = float(np.float32(1e-10))
+    rng = np.random.default_rng(312)
+    for _ in range(200):
+        ua = np.array([rng.uniform(0.5, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), 0.0])
+        ua[3] = rng.uniform(0.3, 2) / (g - 1) + 0.5 * (ua[1] ** 2 + ua[2] ** 2) / ua[0]
+        u = np.array([rng.uniform(0.2, 2), rng.uniform(-3, 3), rng.uniform(-3, 3), 0.0])
+        u[3] = 0.5 * (u[1] ** 2 + u[2] ** 2) / u[0] - rng.uniform(0.01, 1) / (g - 1)   # p < 0
+        t = po.ref_dg2d_solve_for_t(g, eps, u, ua)
+        assert 0.0 <= t <= 1.0
+        w = (1 - t) * ua + t * u
+        pr = (g - 1) * (w[3] - 0.5 * (w[1] ** 2 + w[2] ** 2) / w[0])
+        assert abs(pr - eps) < 1e-9 * (1 + abs(ua[3]) + abs(u[3]))
+
+
+def set_values(p, F):
+    """Full-state modes F (4,nx,ny,6) at the limiter's set: (x_gll, y_gl),
+    (x_gl, y_gll) and (x_gl, y_gl), Gauss-Lobatto -1, 0, 1 and the three Gauss
+    points."""
+    xq, _ = po.dg_gl_quadrature(3)
+    xg = np.array([-1.0, 0.0, 1.0])
+    mi, mj = [0, 1, 0, 2, 1, 0], [0, 0, 1, 0, 1, 2]
+    ph = lambda n, x: po.dg_legendre(x, n)
+    out = []
+    for q in range(3):
+        for c in range(3):
+            for (x, y) in ((xg[c], xq[q]), (xq[q], xg[c]), (xq[q], xq[c])):
+                out.append(sum(F[..., m] * ph(mi[m], x) * ph(mj[m], y) for m in range(6)))
+    return np.stack(out, axis=-1)
+
+
+def equilibrium_modes(p):
+    X, Y = po.dg2d_points(p)
+    k = p.rho0 * p.g / p.p0
+    e = np.exp(-k * (X + Y))
+    nodal = np.zeros((4,) + e.shape, order="F")
+    nodal[0] = p.rho0 * e
+    nodal[3] = p.p0 * e / (p.gamma - 1)
+    return po.dg2d_project(p, nodal)
+
+
+def test_positivity_limiter_keeps_means_and_makes_the_set_positive():
+    """The configured limiter on a state with vacuum-like cells: the means of
+    every variable are untouched bit for bit, and afterwards density and
+    pressure of the full state are at least about the cell's thresholds (1e-5
+    of its equilibrium density and pressure) at every set point."""
+    p = po.make_dg_params(12, 12, degree=2, limiter=po.DG_LIMITER_POSITIVITY)
+    rng = np.random.default_rng(478)
+    E = equilibrium_modes(p)
+    d = np.asfortranarray(rng.normal(size=(4, 12, 12, 6)) * 0.6 * np.abs(E[..., :1]))
+    d[..., 0] = 0.0                               # means: the equilibrium's, positive
+    before = set_values(p, E + d)
+    assert (before[0] < 0).any()
+    lim = po.dg2d_limit(p, d)
+    np.testing.assert_array_equal(lim[..., 0], d[..., 0])
+    after = set_values(p, E + lim)
+    r = after[0]
+    pr = (p.gamma - 1) * (after[3] - 0.5 * (after[1] ** 2 + after[2] ** 2) / r)
+    eq = set_values(p, E)
+    assert np.all(r >= 1e-5 * E[0, :, :, 0][..., None] * (1 - 1e-9))
+    assert np.all(pr >= 1e-5 * (p.gamma - 1) * E[3, :, :, 0][..., None] * (1 - 1e-6))
+    assert eq[0].min() > 0
+    assert np.any(lim != d)
+
+
+def test_positivity_limiter_leaves_positive_states_alone():
+    p = po.make_dg_params(16, 16, degree=2, limiter=po.DG_LIMITER_POSITIVITY)
+    d = po.dg2d_fourier_state(p, amp=1e-2)
+    np.testing.assert_array_equal(po.dg2d_limit(p, d), d)
+    z = np.zeros((4, 16, 16, 6), order="F")
+    np.testing.assert_array_equal(po.dg2d_limit(p, z), z)
+
+
+def test_tvb_then_positivity():
+    p3 = po.make_dg_params(12, 12, degree=2, limiter=po.DG_LIMITER_TVB_POSITIVITY, tvb_m=0.0)
+    p1 = po.make_dg_params(12, 12, degree=2, limiter=po.DG_LIMITER_TVB, tvb_m=0.0)
+    p2 = po.make_dg_params(12, 12, degree=2, limiter=po.DG_LIMITER_POSITIVITY, tvb_m=0.0)
+    E = equilibrium_modes(p3)
+    d = np.asfortranarray(np.random.default_rng(5).normal(size=(4, 12, 12, 6)) * 0.5 * np.abs(E[..., :1]))
+    np.testing.assert_array_equal(po.dg2d_limit(p3, d), po.dg2d_limit(p2, po.dg2d_limit(p1, d)))
+
+
 # ---- the reference's SSP(5,4) (2d/benchmark_2d_dg.f90:683-708) ------------------------
 
 def _ssp54_butcher():
