@@ -102,7 +102,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
     dt = 0.5 * a.h / __longlong_as_double((long long)m) * a.cfl;
     if (tile == 0 && lane < 8) {
       Ctrl* cw = a.ctrl;
-      if (a.first) cw->cmax[q][lane].v = 0ull;
+      // a step past t_end changes nothing and carries the wave-speed maximum to
+      // the next parity, so a later call with a larger t_end resumes from it
+      if (a.first) cw->cmax[q][lane].v = active ? 0ull : c->cmax[p][lane].v;
       if (a.last && lane == 0) {
         cw->t[q] = active ? t + dt : t;
         if (active) { cw->steps_done += 1; cw->dt_last = dt; }

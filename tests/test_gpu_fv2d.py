@@ -310,3 +310,21 @@ def test_2048_fp32_tracks_fp64_and_stays_balanced():
             s.init_gaussian(0.0, 0.3, 0.3, 100.0)
             s.advance(20)
             assert np.count_nonzero(s.download_deviation()) == 0
+
+
+@pytest.mark.parametrize("precision", [fvwb.FP32, fvwb.FP64])
+def test_advance_past_t_end_then_further(precision):
+    """Steps past t_end are no-ops that keep the state, the clock and the
+    wave-speed maximum, so advancing to t1 and then to t2 ends exactly where
+    one advance to t2 ends (same steps, same time, same state)."""
+    print(f"case: advance to t1 then t2, precision={precision}", flush=True)
+    p = fvwb.make_params(48, 40, bc=fvwb.BC_REFLECT, recon=fvwb.RECON_MUSCL_MINMOD, flux=fvwb.FLUX_HLL,
+                         integrator=fvwb.SSPRK3, precision=precision, rho0=1.0)
+    with fvwb.Solver2D(p) as a, fvwb.Solver2D(p) as b:
+        for s in (a, b):
+            s.init_gaussian(1e-2, 0.5, 0.5, 100.0)
+        a.advance(1000, 0.01)
+        a.advance(1000, 0.025)
+        b.advance(2000, 0.025)
+        assert a.time() == b.time()
+        assert np.array_equal(a.download_deviation(), b.download_deviation())
