@@ -12,6 +12,7 @@
 #include <string>
 
 #include "../../include/fvwb.h"
+#include "fvwb_checkpoint.h"
 #include "fvwb_internal.h"
 
 namespace fvwb {
@@ -451,6 +452,38 @@ extern "C" int fvwb_download_deviation(fvwb_handle h, double* du) {
   FVWB_HIP(h, hipMemcpyAsync(du, h->stage_u, 4 * (size_t)p.nx * p.ny * sizeof(double), hipMemcpyDeviceToHost,
                              h->stream));
   FVWB_HIP(h, hipStreamSynchronize(h->stream));
+  return FVWB_OK;
+}
+
+// ---- checkpoint / resume: the deviation planes at t^n, the equilibrium and the clock
+extern "C" int fvwb_checkpoint_save(fvwb_handle h, const char* path) {
+  if (!h || !path) return fail(h, FVWB_ERR_ARG, "NULL argument");
+  if (!h->uploaded) return fail(h, FVWB_ERR_STATE, "checkpoint before upload/init");
+  const fvwb_params& p = h->p;
+  const fvwb_ckpt::Segment seg[3] = {{h->buf[0], 4 * (size_t)h->plane * h->esize},
+                                     {h->w_eq, 4 * (size_t)p.nx * p.ny * sizeof(double)},
+                                     {h->ctrl, sizeof(Ctrl)}};
+  std::string err;
+  const int rc = fvwb_ckpt::save(path, fvwb_ckpt::FV2D, &h->p, sizeof h->p, h->parity, seg, 3, h->stream, err);
+  if (rc) h->err = err;
+  return rc;
+}
+
+extern "C" int fvwb_checkpoint_load(fvwb_handle h, const char* path) {
+  if (!h || !path) return fail(h, FVWB_ERR_ARG, "NULL argument");
+  const fvwb_params& p = h->p;
+  const fvwb_ckpt::Segment seg[3] = {{h->buf[0], 4 * (size_t)h->plane * h->esize},
+                                     {h->w_eq, 4 * (size_t)p.nx * p.ny * sizeof(double)},
+                                     {h->ctrl, sizeof(Ctrl)}};
+  std::string err;
+  int32_t parity = 0;
+  const int rc = fvwb_ckpt::load(path, fvwb_ckpt::FV2D, &h->p, sizeof h->p, &parity, seg, 3, h->stream, err);
+  if (rc) {
+    h->err = err;
+    return rc;
+  }
+  h->parity = parity;
+  h->uploaded = true;
   return FVWB_OK;
 }
 

@@ -18,6 +18,7 @@
 #include <string>
 
 #include "../../include/fvwb.h"
+#include "fvwb_checkpoint.h"
 #include "fvwb_euler1d.h"
 
 namespace dg1d {
@@ -634,4 +635,37 @@ extern "C" int fvwb_dg1d_get_time(fvwb_dg1d_handle h, double* t, double* dt_last
   if (dt_last) *dt_last = clk.dt_last;
   if (steps) *steps = clk.steps;
   return FVWB_OK;
+}
+
+// ---- checkpoint / resume: the modes, u_init, the equilibrium at the Gauss points, the clock
+static int dg1d_segments(fvwb_dg1d* h, fvwb_ckpt::Segment seg[4]) {
+  const size_t bytes = sizeof(double) * 3 * (size_t)h->p.n * (size_t)h->p.nx;
+  seg[0] = {run_array(h, 0), bytes};
+  seg[1] = {run_array(h, 6), bytes};
+  seg[2] = {h->d_ueq, bytes};
+  seg[3] = {h->d_clk, sizeof(Clock)};
+  return 4;
+}
+
+extern "C" int fvwb_dg1d_checkpoint_save(fvwb_dg1d_handle h, const char* path) {
+  if (!h || !path) return faildg1d(h, FVWB_ERR_ARG, "NULL argument");
+  if (!h->loaded) return faildg1d(h, FVWB_ERR_STATE, "checkpoint before upload");
+  fvwb_ckpt::Segment seg[4];
+  const int n = dg1d_segments(h, seg);
+  std::string err;
+  const int rc = fvwb_ckpt::save(path, fvwb_ckpt::DG1D, &h->p, sizeof h->p, 0, seg, n, h->stream, err);
+  if (rc) h->err = err;
+  return rc;
+}
+
+extern "C" int fvwb_dg1d_checkpoint_load(fvwb_dg1d_handle h, const char* path) {
+  if (!h || !path) return faildg1d(h, FVWB_ERR_ARG, "NULL argument");
+  fvwb_ckpt::Segment seg[4];
+  const int n = dg1d_segments(h, seg);
+  std::string err;
+  int32_t parity = 0;
+  const int rc = fvwb_ckpt::load(path, fvwb_ckpt::DG1D, &h->p, sizeof h->p, &parity, seg, n, h->stream, err);
+  h->loaded = rc == FVWB_OK;
+  if (rc) h->err = err;
+  return rc;
 }

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "../../include/fvwb.h"
+#include "fvwb_checkpoint.h"
 #include "fvwb_device.h"
 
 namespace fvwb {
@@ -1112,6 +1113,33 @@ extern "C" int fvwb_dg_advance(fvwb_dg_handle h, int32_t nsteps, double t_end) {
   DGHIP(h, hipMemcpyAsync(&c, h->ctrl, sizeof c, hipMemcpyDeviceToHost, h->stream));
   DGHIP(h, hipStreamSynchronize(h->stream));
   if (c.err) return dg_fail(h, FVWB_ERR_NONPHYSICAL, "rho<=0, p<=0 or non-finite state reached");
+  return FVWB_OK;
+}
+
+// ---- checkpoint / resume: the modes at t^n and the clock
+extern "C" int fvwb_dg_checkpoint_save(fvwb_dg_handle h, const char* path) {
+  if (!h || !path) return dg_fail(h, FVWB_ERR_ARG, "NULL argument");
+  if (!h->ready) return dg_fail(h, FVWB_ERR_STATE, "no state uploaded");
+  const fvwb_ckpt::Segment seg[2] = {{h->buf[0], sizeof(float) * 4 * NM * (size_t)h->plane}, {h->ctrl, sizeof(Ctrl)}};
+  std::string err;
+  const int rc = fvwb_ckpt::save(path, fvwb_ckpt::DG2D, &h->p, sizeof h->p, h->parity, seg, 2, h->stream, err);
+  if (rc) h->err = err;
+  return rc;
+}
+
+extern "C" int fvwb_dg_checkpoint_load(fvwb_dg_handle h, const char* path) {
+  if (!h || !path) return dg_fail(h, FVWB_ERR_ARG, "NULL argument");
+  const fvwb_ckpt::Segment seg[2] = {{h->buf[0], sizeof(float) * 4 * NM * (size_t)h->plane}, {h->ctrl, sizeof(Ctrl)}};
+  std::string err;
+  int32_t parity = 0;
+  const int rc = fvwb_ckpt::load(path, fvwb_ckpt::DG2D, &h->p, sizeof h->p, &parity, seg, 2, h->stream, err);
+  if (rc) {
+    h->err = err;
+    h->ready = false;
+    return rc;
+  }
+  h->parity = parity;
+  h->ready = true;
   return FVWB_OK;
 }
 

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/fvwb.h"
+#include "fvwb_checkpoint.h"
 #include "fvwb_euler1d.h"
 
 // Every expression in this file rounds as written (the oracle and the
@@ -874,6 +875,35 @@ extern "C" int fvwb1d_get_time(fvwb1d_handle h, int32_t inst, double* t, double*
   if (dt_last) *dt_last = c.dt_last;
   if (steps_done) *steps_done = c.steps;
   return FVWB_OK;
+}
+
+// ---- checkpoint / resume: every instance's deviation, centre equilibrium and clock
+extern "C" int fvwb1d_checkpoint_save(fvwb1d_handle h, const char* path) {
+  if (!h || !path) return fail1d(h, FVWB_ERR_ARG, "NULL argument");
+  for (int b = 0; b < h->ninst; ++b)
+    if (!h->ready[b]) return fail1d(h, FVWB_ERR_STATE, "checkpoint before every instance was uploaded");
+  const fvwb_ckpt::Segment seg[3] = {{h->d_dev, sizeof(double) * 3 * (size_t)h->cells},
+                                     {h->d_ueqc, sizeof(double) * 3 * (size_t)h->cells},
+                                     {h->d_clk, sizeof(Clock) * (size_t)h->ninst}};
+  std::string err;
+  const int rc = fvwb_ckpt::save(path, fvwb_ckpt::FV1D, h->p.data(), sizeof(fvwb1d_params) * h->p.size(), 0, seg, 3,
+                                 h->stream, err);
+  if (rc) h->err = err;
+  return rc;
+}
+
+extern "C" int fvwb1d_checkpoint_load(fvwb1d_handle h, const char* path) {
+  if (!h || !path) return fail1d(h, FVWB_ERR_ARG, "NULL argument");
+  const fvwb_ckpt::Segment seg[3] = {{h->d_dev, sizeof(double) * 3 * (size_t)h->cells},
+                                     {h->d_ueqc, sizeof(double) * 3 * (size_t)h->cells},
+                                     {h->d_clk, sizeof(Clock) * (size_t)h->ninst}};
+  std::string err;
+  int32_t parity = 0;
+  const int rc = fvwb_ckpt::load(path, fvwb_ckpt::FV1D, h->p.data(), sizeof(fvwb1d_params) * h->p.size(), &parity,
+                                 seg, 3, h->stream, err);
+  for (int b = 0; b < h->ninst; ++b) h->ready[b] = rc == FVWB_OK;
+  if (rc) h->err = err;
+  return rc;
 }
 
 extern "C" int fvwb1d_get_stream(fvwb1d_handle h, void** stream) {

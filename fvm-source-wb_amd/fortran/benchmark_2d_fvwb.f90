@@ -7,12 +7,15 @@
 ! while t < tend, no clipping of the last step) and the ASCII snapshot format of
 ! output_file (:115-143: x, y, p - p_eq per cell as '(7(1PE12.5,1X))').
 !
-! usage: benchmark_2d_fvwb [stage|resident] [tend] [prefix]
+! usage: benchmark_2d_fvwb [stage|resident|resume] [tend] [prefix] [checkpoint]
 !   stage     the reference's loop on the host; every residual and wave-speed
 !             evaluation is the drop-in compute_update_exact / compute_max_speed
 !             (state crosses PCIe twice per stage, as the reference's caller owns it)
 !   resident  the state is uploaded once and advanced on the device to tend
-!             (fvwb_advance), then downloaded
+!             (fvwb_advance), then downloaded; with a checkpoint path, the
+!             device state at tend is also saved there (fvwb_checkpoint_save)
+!   resume    the device state is loaded from the checkpoint path
+!             (fvwb_checkpoint_load) and advanced from its time to tend
 !   prefix    when given: <prefix>_init.dat and <prefix>_final.dat snapshots and
 !             <prefix>.bin (int32 nx, ny, nvar, steps; real(8) t; u0, w_eq, u)
 program benchmark_2d_fvwb
@@ -22,7 +25,7 @@ program benchmark_2d_fvwb
   use fvwb_dropin_2d
   implicit none
   real(kind=8), allocatable :: x(:,:), y(:,:), u(:,:,:), u0(:,:,:), w_eq(:,:,:)
-  character(len=512) :: arg, mode, prefix
+  character(len=512) :: arg, mode, prefix, ckpt
   integer :: nargs, iter, ios
   real(kind=8) :: t
 
@@ -35,7 +38,9 @@ program benchmark_2d_fvwb
      read(arg, *, iostat=ios) tend
      if (ios /= 0) error stop 'benchmark_2d_fvwb: tend must be a number'
   end if
+  ckpt = ''
   if (nargs >= 3) call get_command_argument(3, prefix)
+  if (nargs >= 4) call get_command_argument(4, ckpt)
 
   allocate(x(nx, ny), y(nx, ny), u(nvar, nx, ny), u0(nvar, nx, ny), w_eq(nvar, nx, ny))
   call cell_centres(x, y)
@@ -47,10 +52,10 @@ program benchmark_2d_fvwb
   select case (trim(mode))
   case ('stage')
      call evolve_stages(u, w_eq, t, iter)
-  case ('resident')
+  case ('resident', 'resume')
      call evolve_resident(u, w_eq, t, iter)
   case default
-     error stop 'benchmark_2d_fvwb: mode must be stage or resident'
+     error stop 'benchmark_2d_fvwb: mode must be stage, resident or resume'
   end select
   write(*, '(a,a,a,i0,a,i0,a,i0,a,1pe24.16)') 'benchmark_2d_fvwb mode=', trim(mode), ' nx=', nx, &
        ' ny=', ny, ' steps=', iter, ' t=', t
@@ -181,8 +186,15 @@ contains
     type(c_ptr) :: h
     integer(c_int32_t) :: steps
     h = fvwb_dropin_handle()
-    call fvwb_check(fvwb_upload(h, u, w_eq), h, 'fvwb_upload')
+    if (trim(mode) == 'resume') then
+       if (len_trim(ckpt) == 0) error stop 'benchmark_2d_fvwb: resume needs a checkpoint path'
+       call fvwb_check(fvwb_checkpoint_load(h, trim(ckpt)//c_null_char), h, 'fvwb_checkpoint_load')
+    else
+       call fvwb_check(fvwb_upload(h, u, w_eq), h, 'fvwb_upload')
+    end if
     call fvwb_check(fvwb_advance(h, huge(steps), tend, t, steps), h, 'fvwb_advance')
+    if (trim(mode) == 'resident' .and. len_trim(ckpt) > 0) &
+         call fvwb_check(fvwb_checkpoint_save(h, trim(ckpt)//c_null_char), h, 'fvwb_checkpoint_save')
     call fvwb_check(fvwb_download(h, u), h, 'fvwb_download')
     iter = steps
   end subroutine evolve_resident

@@ -34,6 +34,8 @@ module fvwb_iface
   public :: fvwb_batch_upload, fvwb_batch_reset_time, fvwb_batch_advance, fvwb_batch_download
   public :: fvwb_batch_download_deviation, fvwb_batch_get_time
   public :: fvwb_check, fvwb_batch_check, fvwb_message
+  public :: fvwb_checkpoint_save, fvwb_checkpoint_load, fvwb1d_checkpoint_save, fvwb1d_checkpoint_load
+  public :: fvwb_dg_checkpoint_save, fvwb_dg_checkpoint_load, fvwb_dg1d_checkpoint_save, fvwb_dg1d_checkpoint_load
 
   ! ---- 1D (benchmark_1d.f90) ----
   integer(c_int32_t), parameter, public :: FVWB1D_BC_PERIODIC = 1, FVWB1D_BC_ZERO_GRADIENT = 2, &
@@ -326,6 +328,48 @@ module fvwb_iface
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s
      end function c_strlen
+     ! ---- checkpoint / resume (path: a NUL-terminated string, trim(name)//c_null_char) ----
+     integer(c_int) function fvwb_checkpoint_save(h, path) bind(C, name='fvwb_checkpoint_save')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function fvwb_checkpoint_save
+     integer(c_int) function fvwb_checkpoint_load(h, path) bind(C, name='fvwb_checkpoint_load')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function fvwb_checkpoint_load
+     integer(c_int) function fvwb1d_checkpoint_save(h, path) bind(C, name='fvwb1d_checkpoint_save')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function fvwb1d_checkpoint_save
+     integer(c_int) function fvwb1d_checkpoint_load(h, path) bind(C, name='fvwb1d_checkpoint_load')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function fvwb1d_checkpoint_load
+     integer(c_int) function fvwb_dg_checkpoint_save(h, path) bind(C, name='fvwb_dg_checkpoint_save')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function fvwb_dg_checkpoint_save
+     integer(c_int) function fvwb_dg_checkpoint_load(h, path) bind(C, name='fvwb_dg_checkpoint_load')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function fvwb_dg_checkpoint_load
+     integer(c_int) function fvwb_dg1d_checkpoint_save(h, path) bind(C, name='fvwb_dg1d_checkpoint_save')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function fvwb_dg1d_checkpoint_save
+     integer(c_int) function fvwb_dg1d_checkpoint_load(h, path) bind(C, name='fvwb_dg1d_checkpoint_load')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function fvwb_dg1d_checkpoint_load
+
      ! ---- 1D modal DG ----
      integer(c_int) function fvwb_dg1d_params_reference(p, nx) bind(C, name='fvwb_dg1d_params_reference')
        import :: c_int, c_int32_t, fvwb_dg1d_params

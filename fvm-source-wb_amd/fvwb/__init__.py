@@ -138,7 +138,12 @@ def lib():
     dp = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int32)
     pp = ctypes.POINTER(Params)
+    cs = ctypes.c_char_p
     sig = {
+        "fvwb_checkpoint_save": [H, cs], "fvwb_checkpoint_load": [H, cs],
+        "fvwb1d_checkpoint_save": [H, cs], "fvwb1d_checkpoint_load": [H, cs],
+        "fvwb_dg_checkpoint_save": [H, cs], "fvwb_dg_checkpoint_load": [H, cs],
+        "fvwb_dg1d_checkpoint_save": [H, cs], "fvwb_dg1d_checkpoint_load": [H, cs],
         "fvwb_params_reference_2d": [pp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
         "fvwb_create": [ctypes.POINTER(H), pp],
         "fvwb_destroy": [H],
@@ -286,6 +291,14 @@ class Solver2D:
         self.close()
 
     # ---- reference external procedures ------------------------------------------
+    def checkpoint_save(self, path):
+        """Write the device state (deviation at t^n, equilibrium, clock) to path."""
+        _check(lib().fvwb_checkpoint_save(self._h, os.fsencode(path)), self._h)
+
+    def checkpoint_load(self, path):
+        """Restore a state written by checkpoint_save of a handle with the same parameters."""
+        _check(lib().fvwb_checkpoint_load(self._h, os.fsencode(path)), self._h)
+
     def compute_update_exact(self, u, w_eq):
         """dudt of compute_update_exact(u, w_eq, dudt) (benchmark_2d.f90:465-618)."""
         u, w_eq = _fortran(u, self.shape), _fortran(w_eq, self.shape)
@@ -534,6 +547,12 @@ class Solver1D:
             raise FvwbError(ERR_ARG, f"expected shape (3, {self._nx(inst)}), got {a.shape}")
         return a
 
+    def checkpoint_save(self, path):
+        _check1d(lib().fvwb1d_checkpoint_save(self._h, os.fsencode(path)), self._h)
+
+    def checkpoint_load(self, path):
+        _check1d(lib().fvwb1d_checkpoint_load(self._h, os.fsencode(path)), self._h)
+
     def compute_update(self, u, w_eq):
         """dudt of compute_update(u, w_eq, dudt) (benchmark_1d.f90:263-377), instance 0."""
         u, w_eq = self._arr(u), self._arr(w_eq)
@@ -673,6 +692,12 @@ class SolverDG:
         _checkdg(lib().fvwb_dg_compute_residual(self._h, _dp(m), _dp(out)), self._h)
         return out
 
+    def checkpoint_save(self, path):
+        _checkdg(lib().fvwb_dg_checkpoint_save(self._h, os.fsencode(path)), self._h)
+
+    def checkpoint_load(self, path):
+        _checkdg(lib().fvwb_dg_checkpoint_load(self._h, os.fsencode(path)), self._h)
+
     def upload(self, modes):
         m = self._arr(modes)
         _checkdg(lib().fvwb_dg_upload(self._h, _dp(m)), self._h)
@@ -780,3 +805,9 @@ class SolverDG1D:
         n = ctypes.c_int32()
         _checkdg1d(lib().fvwb_dg1d_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), self._h)
         return t.value, dt.value, n.value
+
+    def checkpoint_save(self, path):
+        _checkdg1d(lib().fvwb_dg1d_checkpoint_save(self._h, os.fsencode(path)), self._h)
+
+    def checkpoint_load(self, path):
+        _checkdg1d(lib().fvwb_dg1d_checkpoint_load(self._h, os.fsencode(path)), self._h)

@@ -436,6 +436,25 @@ int fvwb_dg1d_advance(fvwb_dg1d_handle h, int32_t nsteps, double t_end);
 int fvwb_dg1d_download(fvwb_dg1d_handle h, double* delta_u, double* u_init);
 int fvwb_dg1d_get_time(fvwb_dg1d_handle h, double* t, double* dt_last, int32_t* steps_done);
 
+/* ---- checkpoint / resume ----------------------------------------------------
+ * No counterpart in the reference, whose snapshots are six-digit ASCII and not
+ * restartable (benchmark_2d.f90:115-143).  save writes the device state a time
+ * loop carries between steps (the state at t^n, the equilibrium tables loaded
+ * with it, the clock and the step parity) to a binary file; load restores it
+ * into a handle created with the same parameters (checked byte for byte;
+ * FVWB_ERR_ARG otherwise, or for a missing, truncated or foreign file).  A
+ * loaded handle continues exactly as the saved one would have: the same
+ * steps, bit for bit.  save needs uploaded state (FVWB_ERR_STATE otherwise);
+ * a failed load leaves the handle without state. */
+int fvwb_checkpoint_save(fvwb_handle h, const char* path);
+int fvwb_checkpoint_load(fvwb_handle h, const char* path);
+int fvwb1d_checkpoint_save(fvwb1d_handle h, const char* path);
+int fvwb1d_checkpoint_load(fvwb1d_handle h, const char* path);
+int fvwb_dg_checkpoint_save(fvwb_dg_handle h, const char* path);
+int fvwb_dg_checkpoint_load(fvwb_dg_handle h, const char* path);
+int fvwb_dg1d_checkpoint_save(fvwb_dg1d_handle h, const char* path);
+int fvwb_dg1d_checkpoint_load(fvwb_dg1d_handle h, const char* path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,3 +65,11 @@ def test_dg1d_null_handle_calls_are_argument_errors():
     L = fvwb.lib()
     assert L.fvwb_dg1d_compute_update_exact_delta(None, None, None, None) == fvwb.ERR_ARG
     assert L.fvwb_dg1d_destroy(None) == fvwb.OK
+
+
+def test_checkpoint_null_arguments_are_argument_errors():
+    L = fvwb.lib()
+    for name in ("fvwb_checkpoint_save", "fvwb_checkpoint_load", "fvwb1d_checkpoint_save", "fvwb1d_checkpoint_load",
+                 "fvwb_dg_checkpoint_save", "fvwb_dg_checkpoint_load", "fvwb_dg1d_checkpoint_save",
+                 "fvwb_dg1d_checkpoint_load"):
+        assert getattr(L, name)(None, b"/tmp/x") == fvwb.ERR_ARG, name

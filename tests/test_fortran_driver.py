@@ -188,6 +188,12 @@ def test_reference_setup_stays_exactly_at_rest(tmp_path, mode):
     assert np.max(np.abs(vals)) <= 4e-16
     x0, y0 = (float(v) for v in lines[1].split()[:2])
     assert abs(x0 - 0.5 / 300) < 1e-8 and abs(y0 - 1.5 / 300) < 1e-8
+    # the reader's contract (post_processing2d.py:10-14): C-order reshape to
+    # (nx, ny) puts x along the first axis and y along the second
+    data = np.loadtxt(prefix + "_final.dat")
+    X, Y = data[:, 0].reshape((300, 300), order="C"), data[:, 1].reshape((300, 300), order="C")
+    assert np.all(np.diff(X, axis=0) > 0) and np.all(np.diff(X, axis=1) == 0)
+    assert np.all(np.diff(Y, axis=1) > 0) and np.all(np.diff(Y, axis=0) == 0)
 
 
 def read_state_1d(path):
@@ -361,3 +367,23 @@ def test_dg1d_driver_reproduces_reference_program(tmp_path, mode):
         np.testing.assert_array_equal([float(b) for _, b, _ in rows], g["t"])
         np.testing.assert_array_equal([float(c) for _, _, c in rows], g["dt"])
     np.testing.assert_array_equal(np.loadtxt(prefix + "recon99999.dat"), g["recon"])
+
+
+@pytest.mark.gpu
+def test_driver_resume_from_checkpoint_equals_one_run(tmp_path):
+    """The resident driver saves its device state at t = 0.01 through the
+    Fortran binding of fvwb_checkpoint_save; a second run resumes from that
+    file to t = 0.02 and ends exactly where one run to 0.02 ends."""
+    ck = str(tmp_path / "s.ckpt")
+    r = run([exe("benchmark_2d_fvwb_64x48"), "resident", "0.01", str(tmp_path / "a"), ck])
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = run([exe("benchmark_2d_fvwb_64x48"), "resume", "0.02", str(tmp_path / "b"), ck])
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = run([exe("benchmark_2d_fvwb_64x48"), "resident", "0.02", str(tmp_path / "c")])
+    assert r.returncode == 0, r.stdout + r.stderr
+    sb, tb, _, _, ub = read_state(str(tmp_path / "b.bin"))
+    sc, tc, _, _, uc = read_state(str(tmp_path / "c.bin"))
+    sa, ta, _, _, _ = read_state(str(tmp_path / "a.bin"))
+    print(f"checkpoint at step {sa} (t {ta}); resumed {sb} steps to {tb}; one run {sc} to {tc}", flush=True)
+    assert 0 < sa < sb and sb == sc and tb == tc
+    assert np.array_equal(ub, uc)
