@@ -417,7 +417,7 @@ struct LimArgs {
   long long plane;
   int pitch, nx, ny, parity;
   int nrec;            // bytes from a variable's mode-0 cell (0,0) to the end of its six planes
-  int limit, posit, cfl, bookkeeping, init;
+  int limit, charlim, posit, cfl, bookkeeping, init;
   const float* eqa;    // A_n(x cell) [3*nx] and B_n(y cell) [3*ny]: equilibrium modes
   const float* eqb;    //   are rho0 A_a B_b (density) and eover A_a B_b (energy)
   double t_end, cfl_over;
@@ -432,6 +432,32 @@ __device__ inline float gminmod(float x, float y, float z, float mh2) {
   if (signbit(y) != sx || signbit(z) != sx) return 0.f;
   const float m = fminf(fminf(fabsf(x), fabsf(y)), fabsf(z));
   return sx ? -m : m;
+}
+
+// Eigenvectors of the Euler flux Jacobian along x (DIR 0) or y at the state u:
+// columns of R for the speeds vn - c, vn, vn (shear), vn + c; L = R^-1.
+template <int DIR>
+__device__ inline void euler_eigen(float gamma, const float u[4], float R[4][4], float L[4][4]) {
+  const float irho = 1.0f / u[0];
+  const float vx = u[1] * irho, vy = u[2] * irho, q2 = vx * vx + vy * vy;
+  const float p = (gamma - 1.0f) * (u[3] - 0.5f * u[0] * q2);
+  const float c = sqrtf(gamma * fmaxf(p, 1e-10f) * irho), ic = 1.0f / c;
+  const float H = (u[3] + p) * irho;
+  const float vn = DIR == 0 ? vx : vy, vt = DIR == 0 ? vy : vx;
+  constexpr int n = DIR == 0 ? 1 : 2, t = DIR == 0 ? 2 : 1;
+  const float b1 = (gamma - 1.0f) * ic * ic, b2 = 0.5f * b1 * q2;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) R[i][k] = L[i][k] = 0.f;
+  R[0][0] = 1.f;         R[0][1] = 1.f;       R[0][3] = 1.f;
+  R[n][0] = vn - c;      R[n][1] = vn;        R[n][3] = vn + c;
+  R[t][0] = vt;          R[t][1] = vt;        R[t][2] = 1.f;  R[t][3] = vt;
+  R[3][0] = H - vn * c;  R[3][1] = 0.5f * q2; R[3][2] = vt;   R[3][3] = H + vn * c;
+  L[0][0] = 0.5f * (b2 + vn * ic); L[0][n] = -0.5f * (b1 * vn + ic); L[0][t] = -0.5f * b1 * vt; L[0][3] = 0.5f * b1;
+  L[1][0] = 1.f - b2;              L[1][n] = b1 * vn;                L[1][t] = b1 * vt;         L[1][3] = -b1;
+  L[2][0] = -vt;                                                     L[2][t] = 1.f;
+  L[3][0] = 0.5f * (b2 - vn * ic); L[3][n] = -0.5f * (b1 * vn - ic); L[3][t] = -0.5f * b1 * vt; L[3][3] = 0.5f * b1;
 }
 
 // solve_for_t (2d/limiters.f90:312-362) in FP64: the t in [0,1] at which the
@@ -577,8 +603,72 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
       }
     }
   }
+  if (a.charlim && in) {
+    // characteristic TVB: the neighbour means' differences and the linear modes
+    // in the characteristic variables of the cell's full-state mean
+    const float s3 = 1.7320508075688772f;
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    const unsigned ow = off(in && hw, ob - 4u, 0), oe = off(in && he, ob + 4u, 0);
+    const unsigned os = off(in && hs, ob - (unsigned)a.pitch * 4u, 0), on = off(in && hn, ob + (unsigned)a.pitch * 4u, 0);
+    float nbw[4], nbe[4], nbs[4], nbn[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float u0 = md[v][0];
+      const float mx = v == 1 ? -u0 : u0, my = v == 2 ? -u0 : u0;
+      const float lw = ldm(rv[v], ow), le = ldm(rv[v], oe), ls = ldm(rv[v], os), ln = ldm(rv[v], on);
+      nbw[v] = hw ? lw : mx; nbe[v] = he ? le : mx; nbs[v] = hs ? ls : my; nbn[v] = hn ? ln : my;
+    }
+    const float e00 = a.eqa[3 * i] * a.eqb[3 * j];
+    const float ubar[4] = {a.rho0 * e00 + md[0][0], md[1][0], md[2][0], a.eover * e00 + md[3][0]};
+    float lin[2][4];
+    bool changed = false;
+    auto limit_dir = [&](auto dirc, const float* up, const float* um, float mh2) {
+      constexpr int D = decltype(dirc)::value;
+      float R[4][4], L[4][4], av[4], dp[4], dm[4];
+      euler_eigen<D>(a.gamma, ubar, R, L);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        av[v] = s3 * md[v][1 + D];
+        dp[v] = up[v] - md[v][0];
+        dm[v] = md[v][0] - um[v];
+      }
+      float cl[4];
+      bool lim = false;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float ca = 0.f, cp = 0.f, cm = 0.f;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          ca += L[k][v] * av[v];
+          cp += L[k][v] * dp[v];
+          cm += L[k][v] * dm[v];
+        }
+        cl[k] = gminmod(ca, cp, cm, mh2);
+        lim |= cl[k] != ca;
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float s = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s += R[v][k] * cl[k];
+        lin[D][v] = lim ? s * (1.0f / 1.7320508075688772f) : md[v][1 + D];
+      }
+      changed |= lim;
+    };
+    limit_dir(std::integral_constant<int, 0>{}, nbe, nbw, a.mdx2);
+    limit_dir(std::integral_constant<int, 1>{}, nbn, nbs, a.mdy2);
+    if (changed) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        md[v][1] = lin[0][v];
+        md[v][2] = lin[1][v];
+        md[v][3] = md[v][4] = md[v][5] = 0.f;
+        chg[v] = true;
+      }
+    }
+  }
   if (a.posit && in) positivity_cell(md, chg, a, i, j);
-  if (a.limit || a.posit) {
+  if (a.limit || a.charlim || a.posit) {
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
@@ -743,7 +833,7 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   if (p->degree != 2) return dg_fail(nullptr, FVWB_ERR_ARG, "the device path implements degree 2 (P2)");
   if (p->flux != FVWB_FLUX_RUSANOV && p->flux != FVWB_FLUX_HLL && p->flux != FVWB_FLUX_HLLC)
     return dg_fail(nullptr, FVWB_ERR_ARG, "bad flux");
-  if (p->limiter < FVWB_DG_LIMITER_NONE || p->limiter > FVWB_DG_LIMITER_TVB_POSITIVITY)
+  if (p->limiter < FVWB_DG_LIMITER_NONE || p->limiter > FVWB_DG_LIMITER_CHARACTERISTIC_POSITIVITY)
     return dg_fail(nullptr, FVWB_ERR_ARG, "bad limiter");
   if (p->bc != FVWB_BC_REFLECT) return dg_fail(nullptr, FVWB_ERR_ARG, "the DG path implements walls (FVWB_BC_REFLECT)");
   if (p->integrator != FVWB_DG_SSPRK3 && p->integrator != FVWB_DG_SSPRK54)
@@ -912,7 +1002,9 @@ static LimArgs lim_args(fvwb_dg* h, float* st) {
   a.mdx2 = (float)(p.tvb_m * (dx * dx)); a.mdy2 = (float)(p.tvb_m * (dy * dy));
   a.rho0 = (float)p.rho0; a.eover = (float)(p.p0 / (p.gamma - 1.0));
   a.limit = p.limiter == FVWB_DG_LIMITER_TVB || p.limiter == FVWB_DG_LIMITER_TVB_POSITIVITY;
-  a.posit = p.limiter == FVWB_DG_LIMITER_POSITIVITY || p.limiter == FVWB_DG_LIMITER_TVB_POSITIVITY;
+  a.charlim = p.limiter == FVWB_DG_LIMITER_CHARACTERISTIC || p.limiter == FVWB_DG_LIMITER_CHARACTERISTIC_POSITIVITY;
+  a.posit = p.limiter == FVWB_DG_LIMITER_POSITIVITY || p.limiter == FVWB_DG_LIMITER_TVB_POSITIVITY ||
+            p.limiter == FVWB_DG_LIMITER_CHARACTERISTIC_POSITIVITY;
   a.eqa = h->eqa32; a.eqb = h->eqb32;
   a.t_end = INFINITY;
   for (int q = 0; q < 3; ++q) {
@@ -1048,24 +1140,24 @@ static int enqueue_step(fvwb_dg* h, double t_end) {
     a.in = A; a.un = nullptr; a.out = B; a.alpha = 0.f; a.beta = 1.f; a.dtc = c[0]; a.first = 1;
     launch_stage(h, a);
     la.st = h->cell(1); la.cfl = 0; la.bookkeeping = 0;
-    if (la.limit || la.posit) launch_limit(h, la);
+    if (la.limit || la.charlim || la.posit) launch_limit(h, la);
     // w2 = a20 u + a21 (w1 + (c21/a21) dt L(w1))
     a.in = B; a.un = A; a.out = Cc; a.alpha = c[1]; a.beta = c[2]; a.dtc = (double)c[3] / c[2]; a.first = 0;
     launch_stage(h, a);
     la.st = h->cell(2);
-    if (la.limit || la.posit) launch_limit(h, la);
+    if (la.limit || la.charlim || la.posit) launch_limit(h, la);
     // w3 = a30 u + a32 (w2 + (c32/a32) dt L(w2)), over w1
     a.in = Cc; a.un = A; a.out = B; a.alpha = c[4]; a.beta = c[5]; a.dtc = (double)c[6] / c[5];
     launch_stage(h, a);
     la.st = h->cell(1);
-    if (la.limit || la.posit) launch_limit(h, la);
+    if (la.limit || la.charlim || la.posit) launch_limit(h, la);
     // w4 = a40 u + a43 w3 + c43 dt L(w3) and P = a50 u + a52 w2 + a53 w3 + c53 dt L(w3)
     a.in = B; a.un = A; a.out = D; a.alpha = c[7]; a.beta = c[8]; a.dtc = 1.0;
     a.xin = Cc; a.xout = Cc; a.gam = c[9]; a.pu = c[10]; a.px = c[11]; a.pin = c[12]; a.pl = c[14];
     launch_stage(h, a);
     a.xin = nullptr; a.xout = nullptr;
     la.st = h->cell(3);
-    if (la.limit || la.posit) launch_limit(h, la);
+    if (la.limit || la.charlim || la.posit) launch_limit(h, la);
     // u = P + a54 (w4 + (c54/a54) dt L(w4))
     a.in = D; a.un = Cc; a.out = A; a.alpha = 1.f; a.beta = c[13]; a.dtc = (double)c[15] / c[13];
     launch_stage(h, a);
@@ -1079,12 +1171,12 @@ static int enqueue_step(fvwb_dg* h, double t_end) {
   a.in = A; a.un = nullptr; a.out = B; a.alpha = 0.f; a.beta = 1.f; a.first = 1;
   launch_stage(h, a);
   la.st = h->cell(1); la.cfl = 0; la.bookkeeping = 0;
-  if (la.limit || la.posit) launch_limit(h, la);
+  if (la.limit || la.charlim || la.posit) launch_limit(h, la);
   // stage 2
   a.in = B; a.un = A; a.out = Cc; a.alpha = 0.75f; a.beta = 0.25f; a.first = 0;
   launch_stage(h, a);
   la.st = h->cell(2);
-  if (la.limit || la.posit) launch_limit(h, la);
+  if (la.limit || la.charlim || la.posit) launch_limit(h, la);
   // stage 3 (in place over U^n: each owned cell's U^n is read by the lane that writes it, before the write)
   a.in = Cc; a.un = A; a.out = A; a.alpha = (float)(1.0 / 3.0); a.beta = (float)(2.0 / 3.0);
   launch_stage(h, a);

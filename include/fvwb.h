@@ -331,6 +331,19 @@ int fvwb1d_get_stream(fvwb1d_handle h, void** stream);
                                      equilibrium's own modes are its Gauss
                                      projection; means never change            */
 #define FVWB_DG_LIMITER_TVB_POSITIVITY 3   /* TVB, then positivity              */
+#define FVWB_DG_LIMITER_CHARACTERISTIC 4   /* the TVB minmod of the linear modes
+                                     taken in characteristic variables: the
+                                     x slope and the x differences of the
+                                     neighbour means (of the deviation) through
+                                     the left eigenvectors of dF/dU at the cell's
+                                     full-state mean, the y ones through those of
+                                     dG/dU, limited componentwise and mapped back
+                                     with the right eigenvectors; a limited cell
+                                     drops its quadratic modes (the per-direction
+                                     form of the characteristic limiting the
+                                     reference sketches in 2d/benchmark_2d_dg.f90:
+                                     2021-2089 and 2d/limiters.f90:1200-1295)   */
+#define FVWB_DG_LIMITER_CHARACTERISTIC_POSITIVITY 5   /* characteristic, then positivity */
 #define FVWB_DG_SSPRK3        0   /* Shu-Osher SSP-RK3 (dg_with_source.f90:194-204)  */
 #define FVWB_DG_SSPRK54       1   /* five-stage fourth-order SSP Runge-Kutta of
                                      2d/benchmark_2d_dg.f90:683-710 (solver 'RK4',

@@ -135,6 +135,9 @@ void   orc_ref_dg2d_nodes_from_modes(int nx, int ny, int order, const double* mo
 void   orc_ref_dg2d_gll_points(int n, double* x);
 double orc_ref_dg2d_solve_for_t(double gamma, double eps, const double u[4], const double u_avg[4]);
 void   orc_ref_dg2d_positivity(int nx, int ny, int order, double gamma, double eps, double* u);
+/* right (columns) and left eigenvectors of the x (dir 0) or y (dir 1) Euler
+ * flux Jacobian at the conservative state u, row-major 4x4, L = R^-1 */
+void   orc_euler_eigenvectors(double gamma, int dir, const double u[4], double R[16], double L[16]);
 /* configured scheme; modes AoS(4,nx,ny,nmodes) of the deviation */
 int    orc_dg2d_nmodes(int degree);
 void   orc_dg2d_residual(const fvwb_dg_params* p, const double* d, double* L);

@@ -19,6 +19,7 @@
 #include "fvwb_oracle.h"
 
 #include <math.h>
+#include <stdbool.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -358,9 +359,120 @@ static void positivity_cell(const fvwb_dg_params* p, const dg_basis* B, int i, i
   }
 }
 
+/* Eigenvectors of the 2D Euler flux Jacobian along x (dir 0) or y (dir 1) at
+ * the conservative state u: columns of R are the right eigenvectors for the
+ * speeds vn - c, vn, vn (shear), vn + c; L = R^-1 written in closed form. */
+void orc_euler_eigenvectors(double gamma, int dir, const double u[4], double R[16], double L[16])
+{
+  const double rho = u[0], vx = u[1] / rho, vy = u[2] / rho;
+  const double q2 = vx * vx + vy * vy;
+  const double p = (gamma - 1.0) * (u[3] - 0.5 * rho * q2);
+  const double c = sqrt(gamma * p / rho), H = (u[3] + p) / rho;
+  const double vn = dir == 0 ? vx : vy, vt = dir == 0 ? vy : vx;
+  const int n = dir == 0 ? 1 : 2, t = dir == 0 ? 2 : 1;   /* normal and tangential momentum rows */
+  const double b1 = (gamma - 1.0) / (c * c), b2 = 0.5 * b1 * q2;
+  double r[4][4] = {{0}}, l[4][4] = {{0}};
+  /* right eigenvectors (rows = conserved variable, columns = wave) */
+  r[0][0] = 1.0;     r[0][1] = 1.0;      r[0][2] = 0.0; r[0][3] = 1.0;
+  r[n][0] = vn - c;  r[n][1] = vn;       r[n][2] = 0.0; r[n][3] = vn + c;
+  r[t][0] = vt;      r[t][1] = vt;       r[t][2] = 1.0; r[t][3] = vt;
+  r[3][0] = H - vn * c; r[3][1] = 0.5 * q2; r[3][2] = vt; r[3][3] = H + vn * c;
+  /* left eigenvectors (rows = wave) */
+  l[0][0] = 0.5 * (b2 + vn / c); l[0][n] = -0.5 * (b1 * vn + 1.0 / c); l[0][t] = -0.5 * b1 * vt; l[0][3] = 0.5 * b1;
+  l[1][0] = 1.0 - b2;            l[1][n] = b1 * vn;                  l[1][t] = b1 * vt;         l[1][3] = -b1;
+  l[2][0] = -vt;                 l[2][n] = 0.0;                      l[2][t] = 1.0;             l[2][3] = 0.0;
+  l[3][0] = 0.5 * (b2 - vn / c); l[3][n] = -0.5 * (b1 * vn - 1.0 / c); l[3][t] = -0.5 * b1 * vt; l[3][3] = 0.5 * b1;
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 4; ++k) { R[4 * i + k] = r[i][k]; L[4 * i + k] = l[i][k]; }
+}
+
+static void mat4(const double* M, const double* x, double* y)
+{
+  for (int i = 0; i < 4; ++i) y[i] = M[4 * i] * x[0] + M[4 * i + 1] * x[1] + M[4 * i + 2] * x[2] + M[4 * i + 3] * x[3];
+}
+
+/* characteristic TVB limiting of the linear modes (limiter 4, 5) */
+static void characteristic_limit(const fvwb_dg_params* p, double* d)
+{
+  const int nx = p->nx, ny = p->ny, nm = orc_dg2d_nmodes(p->degree);
+  const double dx = p->boxlen_x / nx, dy = p->boxlen_y / ny;
+  const double s3 = sqrt(3.0);
+  dg_basis B;
+  make_basis(p->degree, &B);
+  double* out = malloc(sizeof(double) * 4 * (size_t)nx * ny * nm);
+  memcpy(out, d, sizeof(double) * 4 * (size_t)nx * ny * nm);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      /* the full-state mean: the equilibrium's Gauss-projected mean plus the deviation's */
+      double ubar[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int b = 0; b < B.nq; ++b)
+        for (int a = 0; a < B.nq; ++a) {
+          double ue[4];
+          ueq_at(p, (i + 0.5) * dx + 0.5 * dx * B.xq[a], (j + 0.5) * dy + 0.5 * dy * B.xq[b], ue);
+          for (int v = 0; v < 4; ++v) ubar[v] += 0.25 * B.wq[a] * B.wq[b] * ue[v];
+        }
+      for (int v = 0; v < 4; ++v) ubar[v] += DM(d, v, i, j, 0, nx, ny);
+      bool changed = false;
+      double lin[2][4];
+      for (int dir = 0; dir < 2; ++dir) {
+        double R[16], L[16], a[4], dp[4], dm[4], ca[4], cp[4], cm[4];
+        orc_euler_eigenvectors(p->gamma, dir, ubar, R, L);
+        for (int v = 0; v < 4; ++v) {
+          const double u0 = DM(d, v, i, j, 0, nx, ny);
+          double up, um;
+          if (dir == 0) {
+            up = i < nx - 1 ? DM(d, v, i + 1, j, 0, nx, ny) : (v == 1 ? -u0 : u0);
+            um = i > 0 ? DM(d, v, i - 1, j, 0, nx, ny) : (v == 1 ? -u0 : u0);
+          } else {
+            up = j < ny - 1 ? DM(d, v, i, j + 1, 0, nx, ny) : (v == 2 ? -u0 : u0);
+            um = j > 0 ? DM(d, v, i, j - 1, 0, nx, ny) : (v == 2 ? -u0 : u0);
+          }
+          a[v] = s3 * DM(d, v, i, j, 1 + dir, nx, ny);
+          dp[v] = up - u0;
+          dm[v] = u0 - um;
+        }
+        mat4(L, a, ca);
+        mat4(L, dp, cp);
+        mat4(L, dm, cm);
+        bool lim = false;
+        double cl[4];
+        for (int k = 0; k < 4; ++k) {
+          cl[k] = orc_generalized_minmod(ca[k], cp[k], cm[k], p->tvb_m, dir == 0 ? dx : dy);
+          lim |= cl[k] != ca[k];
+        }
+        if (lim) {
+          double a2[4];
+          mat4(R, cl, a2);
+          for (int v = 0; v < 4; ++v) lin[dir][v] = a2[v] / s3;
+          changed = true;
+        } else {
+          for (int v = 0; v < 4; ++v) lin[dir][v] = DM(d, v, i, j, 1 + dir, nx, ny);
+        }
+      }
+      if (changed)
+        for (int v = 0; v < 4; ++v) {
+          DM(out, v, i, j, 1, nx, ny) = lin[0][v];
+          DM(out, v, i, j, 2, nx, ny) = lin[1][v];
+          for (int m = 3; m < nm; ++m) DM(out, v, i, j, m, nx, ny) = 0.0;
+        }
+    }
+  memcpy(d, out, sizeof(double) * 4 * (size_t)nx * ny * nm);
+  free(out);
+}
+
 void orc_dg2d_limit(const fvwb_dg_params* p, double* d)
 {
   if (p->degree < 1) return;
+  if (p->limiter == FVWB_DG_LIMITER_CHARACTERISTIC || p->limiter == FVWB_DG_LIMITER_CHARACTERISTIC_POSITIVITY) {
+    characteristic_limit(p, d);
+    if (p->limiter == FVWB_DG_LIMITER_CHARACTERISTIC_POSITIVITY) {
+      dg_basis B;
+      make_basis(p->degree, &B);
+      for (int j = 0; j < p->ny; ++j)
+        for (int i = 0; i < p->nx; ++i) positivity_cell(p, &B, i, j, d);
+    }
+    return;
+  }
   if (p->limiter == FVWB_DG_LIMITER_POSITIVITY || p->limiter == FVWB_DG_LIMITER_TVB_POSITIVITY) {
     if (p->limiter == FVWB_DG_LIMITER_TVB_POSITIVITY) {
       fvwb_dg_params q = *p;

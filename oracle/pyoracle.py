@@ -81,6 +81,7 @@ def make_dg1d_params(nx=128, *, n=3, riemann=DG1D_RIEMANN_HLLC, source=2, gamma=
 
 
 DG_LIMITER_NONE, DG_LIMITER_TVB, DG_LIMITER_POSITIVITY, DG_LIMITER_TVB_POSITIVITY = 0, 1, 2, 3
+DG_LIMITER_CHARACTERISTIC, DG_LIMITER_CHARACTERISTIC_POSITIVITY = 4, 5
 DG_SSPRK3, DG_SSPRK54 = 0, 1
 
 
@@ -170,6 +171,7 @@ def lib():
         L.orc_dg_legendre_prime.argtypes = [ctypes.c_double, ctypes.c_int]
         L.orc_dg_legendre_prime.restype = ctypes.c_double
         L.orc_dg_gl_quadrature.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_euler_eigenvectors.argtypes = [ctypes.c_double, ctypes.c_int, dp, dp, dp]
         L.orc_ref_dg2d_solve_for_t.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp]
         L.orc_ref_dg2d_solve_for_t.restype = ctypes.c_double
         L.orc_ref_dg2d_positivity.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -500,6 +502,15 @@ def dg_gl_quadrature(n):
     x, w = np.zeros(n), np.zeros(n)
     lib().orc_dg_gl_quadrature(n, _p(x), _p(w))
     return x, w
+
+
+def euler_eigenvectors(gamma, direction, u):
+    """(R, L): right eigenvectors as columns and their inverse, of the x (0) or y (1)
+    Euler flux Jacobian at the conservative state u."""
+    u = np.ascontiguousarray(u, dtype=float)
+    R, L = np.zeros(16), np.zeros(16)
+    lib().orc_euler_eigenvectors(float(gamma), int(direction), _p(u), _p(R), _p(L))
+    return R.reshape(4, 4), L.reshape(4, 4)
 
 
 def ref_dg2d_solve_for_t(gamma, eps, u, u_avg):

@@ -124,7 +124,7 @@ def test_limiter_matches_oracle():
 
 
 @pytest.mark.parametrize("integrator", (po.DG_SSPRK3, po.DG_SSPRK54))
-@pytest.mark.parametrize("limiter", (po.DG_LIMITER_NONE, po.DG_LIMITER_TVB))
+@pytest.mark.parametrize("limiter", (po.DG_LIMITER_NONE, po.DG_LIMITER_TVB, po.DG_LIMITER_CHARACTERISTIC))
 def test_run_matches_oracle(limiter, integrator):
     say(f"case: DG 30 steps limiter={limiter} integrator={integrator}")
     op = po.make_dg_params(48, 40, limiter=limiter, g=1.0, integrator=integrator)
@@ -328,3 +328,40 @@ def test_positivity_limiter_keeps_a_near_vacuum_run_alive():
         f"min p {pr.min():.3e}")
     assert got0 is None and got1 is None
     assert rho.min() > 0 and pr.min() > 0
+
+
+
+# ---- characteristic limiting --------------------------------------------------------------
+@pytest.mark.parametrize("tvb_m", [0.0, 50.0])
+def test_characteristic_limiter_matches_oracle(tvb_m):
+    """One pass (on upload) over steps in density, pressure and velocity: the
+    device's FP32 characteristic limiter equals the oracle's on the
+    FP32-rounded input to FP32 rounding, cell by cell (the same cells limited)."""
+    say(f"case: DG characteristic limiter M={tvb_m}")
+    op = po.make_dg_params(32, 24, tvb_m=tvb_m, limiter=po.DG_LIMITER_CHARACTERISTIC)
+    X, Y = po.dg2d_points(op)
+    nodal = np.zeros((4, 32, 24, 9), order="F")
+    nodal[3] = np.where(X < 0.37, 0.5, 0.0)
+    nodal[0] = np.where(X + Y < 0.8, 0.05, 0.0)
+    nodal[1] = 0.2 * np.where(Y > 0.5, 1.0, -1.0)
+    d = np.asfortranarray(po.dg2d_project(op, nodal).astype(np.float32).astype(np.float64))
+    ref = po.dg2d_limit(op, d)
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        s.upload(d)
+        got = s.download()
+    lim_ref = np.any(ref != d, axis=(0, 3))
+    lim_got = np.any(got != d, axis=(0, 3))
+    err = np.max(np.abs(got - ref))
+    say(f"   limited cells {lim_ref.sum()} (device {lim_got.sum()}), max err {err:.3e}")
+    assert lim_ref.sum() > 0 and np.array_equal(lim_ref, lim_got)
+    assert err <= 2e-6 * np.max(np.abs(d))
+
+
+def test_characteristic_limiter_keeps_rest_exact():
+    say("case: DG rest with the characteristic limiter")
+    op = po.make_dg_params(24, 24, limiter=po.DG_LIMITER_CHARACTERISTIC_POSITIVITY, integrator=po.DG_SSPRK54)
+    z = np.zeros((4, 24, 24, 6), order="F")
+    with fvwb.SolverDG(gpu_p(op)) as s:
+        s.upload(z)
+        s.advance(20)
+        assert np.count_nonzero(s.download()) == 0

@@ -271,6 +271,85 @@ def test_tvb_then_positivity():
     np.testing.assert_array_equal(po.dg2d_limit(p3, d), po.dg2d_limit(p2, po.dg2d_limit(p1, d)))
 
 
+# ---- characteristic limiting -----------------------------------------------------------
+def euler_flux(gamma, U, direction):
+    r, mx, my, E = U
+    vx, vy = mx / r, my / r
+    p = (gamma - 1) * (E - 0.5 * r * (vx * vx + vy * vy))
+    if direction == 0:
+        return np.array([mx, mx * vx + p, my * vx, (E + p) * vx])
+    return np.array([my, mx * vy, my * vy + p, (E + p) * vy])
+
+
+def test_eigenvectors_diagonalise_the_flux_jacobian():
+    """R's columns are eigenvectors of dF/dU (by central differences) for the
+    speeds vn-c, vn, vn, vn+c, and L is R's inverse, in both directions."""
+    g = 1.4
+    rng = np.random.default_rng(21)
+    for _ in range(20):
+        rho, vx, vy, pr = rng.uniform(0.5, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.3, 2)
+        U = np.array([rho, rho * vx, rho * vy, pr / (g - 1) + 0.5 * rho * (vx * vx + vy * vy)])
+        c = np.sqrt(g * pr / rho)
+        for d in (0, 1):
+            R, L = po.euler_eigenvectors(g, d, U)
+            np.testing.assert_allclose(L @ R, np.eye(4), atol=1e-13)
+            h = 1e-6
+            A = np.stack([(euler_flux(g, U + h * e, d) - euler_flux(g, U - h * e, d)) / (2 * h)
+                          for e in np.eye(4)], axis=1)
+            vn = vx if d == 0 else vy
+            np.testing.assert_allclose(A @ R, R * np.array([vn - c, vn, vn, vn + c]), atol=1e-7)
+
+
+def test_characteristic_limiter_basics():
+    """Smooth data below M dx^2 and a resting atmosphere are left alone bit
+    for bit; on a step every mean stays, a limited cell keeps only its
+    (characteristically limited) linear modes, and at M = 0 the limited x
+    slope in characteristic variables is within the neighbour differences."""
+    p = po.make_dg_params(16, 16, degree=2, tvb_m=50.0, limiter=po.DG_LIMITER_CHARACTERISTIC)
+    d = po.dg2d_fourier_state(p, amp=1e-4)
+    np.testing.assert_array_equal(po.dg2d_limit(p, d), d)
+    z = np.zeros((4, 16, 16, 6), order="F")
+    np.testing.assert_array_equal(po.dg2d_limit(p, z), z)
+    X, Y = po.dg2d_points(p)
+    nodal = np.zeros((4, 16, 16, 9), order="F")
+    nodal[3] = np.where(X < 0.5, 0.3, 0.0)
+    nodal[0] = np.where(X + Y < 1.0, 0.2, 0.0)
+    nodal[1] = 0.1 * np.sin(3 * Y)
+    d = po.dg2d_project(p, nodal)
+    p0 = po.make_dg_params(16, 16, degree=2, tvb_m=0.0, limiter=po.DG_LIMITER_CHARACTERISTIC)
+    lim = po.dg2d_limit(p0, d)
+    np.testing.assert_array_equal(lim[..., 0], d[..., 0])
+    changed = np.any(lim != d, axis=(0, 3))
+    assert changed.any()
+    assert np.all(lim[:, changed][:, :, 3:] == 0.0)
+    E = equilibrium_modes(p0)
+    s3 = np.sqrt(3.0)
+    for i in range(1, 15):
+        for j in range(16):
+            ubar = E[:, i, j, 0] + d[:, i, j, 0]
+            _, L = po.euler_eigenvectors(p0.gamma, 0, ubar)
+            a = L @ (s3 * lim[:, i, j, 1])
+            dp = L @ (d[:, i + 1, j, 0] - d[:, i, j, 0])
+            dm = L @ (d[:, i, j, 0] - d[:, i - 1, j, 0])
+            assert np.all(np.abs(a) <= np.maximum(np.abs(dp), np.abs(dm)) * (1 + 1e-12) + 1e-15)
+
+
+def test_characteristic_limiter_on_an_entropy_wave_is_the_scalar_one():
+    """A density step at rest with no pressure or velocity deviation excites
+    only the entropy field, whose characteristic variable is the density
+    deviation itself: the characteristic limiter then equals the scalar TVB
+    limiter (to the rounding of the two transforms)."""
+    pc = po.make_dg_params(16, 12, degree=2, tvb_m=0.0, limiter=po.DG_LIMITER_CHARACTERISTIC)
+    ps = po.make_dg_params(16, 12, degree=2, tvb_m=0.0, limiter=po.DG_LIMITER_TVB)
+    X, Y = po.dg2d_points(pc)
+    nodal = np.zeros((4, 16, 12, 9), order="F")
+    nodal[0] = np.where(X < 0.4, 0.2, 0.0) + np.where(Y > 0.6, 0.1, 0.0)
+    d = po.dg2d_project(pc, nodal)
+    lc, ls = po.dg2d_limit(pc, d), po.dg2d_limit(ps, d)
+    assert np.any(ls != d)
+    np.testing.assert_allclose(lc, ls, rtol=0, atol=1e-14)
+
+
 # ---- the reference's SSP(5,4) (2d/benchmark_2d_dg.f90:683-708) ------------------------
 
 def _ssp54_butcher():
