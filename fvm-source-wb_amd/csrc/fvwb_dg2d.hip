@@ -550,6 +550,9 @@ __device__ inline void positivity_cell(float md[4][NM], bool chg[4], const LimAr
   }
 }
 
+// TVB, CHAR and POS select the limiters compiled in (each launch carries only
+// its own code); the CFL rate and the bookkeeping stay runtime switches.
+template <bool TVB, bool CHAR, bool POS>
 __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
   const int par = a.parity;
   const int blin = blockIdx.y * gridDim.x + blockIdx.x;
@@ -576,10 +579,10 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
 #pragma unroll
   for (int v = 0; v < 4; ++v)
 #pragma unroll
-    for (int m = 0; m < NM; ++m) md[v][m] = (m < 3 || a.cfl || a.posit) ? ldm(rv[v], off(in, ob, m)) : 0.f;
+    for (int m = 0; m < NM; ++m) md[v][m] = (m < 3 || a.cfl || POS) ? ldm(rv[v], off(in, ob, m)) : 0.f;
   bool chg[4] = {false, false, false, false};
 
-  if (a.limit) {
+  if (TVB) {
     const float s3 = 1.7320508075688772f;
     const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
     const unsigned ow = off(in && hw, ob - 4u, 0), oe = off(in && he, ob + 4u, 0);
@@ -603,7 +606,7 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
       }
     }
   }
-  if (a.charlim && in) {
+  if (CHAR && in) {
     // characteristic TVB: the neighbour means' differences and the linear modes
     // in the characteristic variables of the cell's full-state mean
     const float s3 = 1.7320508075688772f;
@@ -667,8 +670,8 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
       }
     }
   }
-  if (a.posit && in) positivity_cell(md, chg, a, i, j);
-  if (a.limit || a.charlim || a.posit) {
+  if (POS && in) positivity_cell(md, chg, a, i, j);
+  if (TVB || CHAR || POS) {
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
@@ -1033,8 +1036,26 @@ static void launch_stage(fvwb_dg* h, const Args& a) {
 }
 
 static void launch_limit(fvwb_dg* h, const LimArgs& a) {
-  hipLaunchKernelGGL(dg_limit_kernel, dim3((unsigned)((h->p.nx + 255) / 256), (unsigned)h->p.ny), dim3(256), 0,
-                     h->stream, a);
+  const dim3 grid((unsigned)((h->p.nx + 255) / 256), (unsigned)h->p.ny), block(256);
+  switch (h->p.limiter) {
+    case FVWB_DG_LIMITER_TVB:
+      hipLaunchKernelGGL((dg_limit_kernel<true, false, false>), grid, block, 0, h->stream, a);
+      break;
+    case FVWB_DG_LIMITER_POSITIVITY:
+      hipLaunchKernelGGL((dg_limit_kernel<false, false, true>), grid, block, 0, h->stream, a);
+      break;
+    case FVWB_DG_LIMITER_TVB_POSITIVITY:
+      hipLaunchKernelGGL((dg_limit_kernel<true, false, true>), grid, block, 0, h->stream, a);
+      break;
+    case FVWB_DG_LIMITER_CHARACTERISTIC:
+      hipLaunchKernelGGL((dg_limit_kernel<false, true, false>), grid, block, 0, h->stream, a);
+      break;
+    case FVWB_DG_LIMITER_CHARACTERISTIC_POSITIVITY:
+      hipLaunchKernelGGL((dg_limit_kernel<false, true, true>), grid, block, 0, h->stream, a);
+      break;
+    default:
+      hipLaunchKernelGGL((dg_limit_kernel<false, false, false>), grid, block, 0, h->stream, a);
+  }
 }
 
 static int put_modes(fvwb_dg* h, const double* modes, float* dst) {
