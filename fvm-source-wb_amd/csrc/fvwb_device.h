@@ -119,13 +119,14 @@ template <typename T>
 struct Phys {
   T gamma, gm1, half;   // gamma, gamma-1, 0.5
   T g;                  // |grad phi| per axis
+  T ggm1, gfloor;       // gamma*(gamma-1) and gamma*1e-10 (sound speed of DEV sides)
 };
 
 // Face equilibrium at rest: density, pressure, energy p/(gamma-1) (the
-// reference's compute_conservative of a resting state, benchmark_2d.f90:169)
-// and E+p.
+// reference's compute_conservative of a resting state, benchmark_2d.f90:169),
+// E+p, and gamma*p.
 template <typename T>
-struct FaceEq { T rho, p, e, ep; };
+struct FaceEq { T rho, p, e, ep, gp; };
 
 enum { FLUX_RUSANOV = 0, FLUX_HLL = 1, FLUX_HLLC = 2 };
 enum { FORM_PLAIN = 0, FORM_DEV = 1 };
@@ -167,13 +168,16 @@ __device__ __forceinline__ Side<T> side_plain(const Phys<T>& P, const FaceEq<T>&
 
 // DEV form: with m_eq = 0, F(U_eq+dU) - F(U_eq) along x is
 //   (dm_x, m_x u + dp, m_y u, (E+p) u),  dp = (gamma-1)(dE - |m|^2/(2 rho)),
-// and symmetrically along y; no cancellation of O(1) terms occurs.
+// and symmetrically along y; no cancellation of O(1) terms occurs.  The sound
+// speed takes gamma*p = gamma*p_eq + gamma(gamma-1)(dE - |m|^2/(2 rho)) in one
+// fused step, floored at gamma*1e-10 (= gamma*max(p, 1e-10)).
 template <typename T, int DIR>
 __device__ __forceinline__ Side<T> side_dev(const Phys<T>& P, const FaceEq<T>& q, Vec4<T> d) {
   T rho = q.rho + d.r;
   T inv = trcp(tmax(rho, T(1e-10)));
   T u = d.mx * inv, v = d.my * inv;
-  T dp = P.gm1 * (d.e - P.half * (d.mx * u + d.my * v));
+  T de = d.e - P.half * (d.mx * u + d.my * v);
+  T dp = P.gm1 * de;
   T ep = (q.ep + d.e) + dp;      // E + p
   Side<T> s;
   if (DIR == 0) {
@@ -183,7 +187,7 @@ __device__ __forceinline__ Side<T> side_dev(const Phys<T>& P, const FaceEq<T>& q
     s.f = {d.my, d.mx * v, d.my * v + dp, ep * v};
     s.vn = v;
   }
-  s.c = tsqrt(P.gamma * tmax(q.p + dp, T(1e-10)) * inv);
+  s.c = tsqrt(tmax(P.ggm1 * de + q.gp, P.gfloor) * inv);
   s.vmag = tsqrt(u * u + v * v);
   return s;
 }
@@ -284,11 +288,20 @@ __device__ __forceinline__ Vec4<T> face_flux(const Phys<T>& P, const FaceEq<T>& 
     T sr = tmax(L.vn + L.c, R.vn + R.c);
     T ap = tmax(T(0), sr), am = tmax(T(0), -sl);
     T inv = trcp(ap + am);
-    T w = am * inv, qq = ap * am * inv;
-    fh = {L.f.r + w * (R.f.r - L.f.r) - qq * (jr.r - jl.r),
-          L.f.mx + w * (R.f.mx - L.f.mx) - qq * (jr.mx - jl.mx),
-          L.f.my + w * (R.f.my - L.f.my) - qq * (jr.my - jl.my),
-          L.f.e + w * (R.f.e - L.f.e) - qq * (jr.e - jl.e)};
+    T w = am * inv;
+    if (FORM == FORM_DEV) {
+      // the same flux with q = a+ w factored out: F_L + w ((F_R - F_L) - a+ (U_R - U_L))
+      fh = {L.f.r + w * ((R.f.r - L.f.r) - ap * (jr.r - jl.r)),
+            L.f.mx + w * ((R.f.mx - L.f.mx) - ap * (jr.mx - jl.mx)),
+            L.f.my + w * ((R.f.my - L.f.my) - ap * (jr.my - jl.my)),
+            L.f.e + w * ((R.f.e - L.f.e) - ap * (jr.e - jl.e))};
+    } else {
+      T qq = ap * am * inv;
+      fh = {L.f.r + w * (R.f.r - L.f.r) - qq * (jr.r - jl.r),
+            L.f.mx + w * (R.f.mx - L.f.mx) - qq * (jr.mx - jl.mx),
+            L.f.my + w * (R.f.my - L.f.my) - qq * (jr.my - jl.my),
+            L.f.e + w * (R.f.e - L.f.e) - qq * (jr.e - jl.e)};
+    }
   }
   if (FORM == FORM_PLAIN) fh = fh - fe;
   return fh;

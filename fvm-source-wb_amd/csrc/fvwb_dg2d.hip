@@ -102,6 +102,8 @@ struct Args {
   float gam, pu, px, pin, pl;
   float gamma, g, odx, ody;
   float rho0, p0, e0, ep0;   // rho0, p0, p0/(gamma-1), p0 gamma/(gamma-1): equilibrium rho, p, E, E+p per e^{-k(x+y)}
+  float gp0;                 // gamma p0: equilibrium gamma*p per e^{-k(x+y)}
+  float ggm1, gfloor;        // gamma (gamma-1) and gamma*1e-10
 };
 
 __device__ inline float ctrl_rate(const Ctrl* c, int par) {
@@ -130,6 +132,7 @@ __device__ inline FaceEq<float> eq_at(const Args& a, float e) {
   q.p = a.p0 * e;
   q.e = a.e0 * e;      // used by HLLC only
   q.ep = a.ep0 * e;
+  q.gp = a.gp0 * e;
   return q;
 }
 
@@ -202,6 +205,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
 
   Phys<float> P;
   P.gamma = a.gamma; P.gm1 = a.gamma - 1.0f; P.half = 0.5f; P.g = a.g;
+  P.ggm1 = a.ggm1; P.gfloor = a.gfloor;
 
   // lane offsets (bytes) of modes g and g+4 of column i inside a row of a variable
   const unsigned colb = (unsigned)(i + 1) * 4u;
@@ -986,6 +990,9 @@ static Args stage_args(fvwb_dg* h) {
   a.odx = (float)(p.nx / p.boxlen_x); a.ody = (float)(p.ny / p.boxlen_y);
   a.rho0 = (float)p.rho0; a.p0 = (float)p.p0; a.e0 = (float)(p.p0 / (p.gamma - 1.0));
   a.ep0 = (float)(p.p0 * p.gamma / (p.gamma - 1.0));
+  a.gp0 = (float)(p.p0 * p.gamma);
+  a.ggm1 = (float)(p.gamma * (p.gamma - 1.0));
+  a.gfloor = (float)(p.gamma * 1e-10);
   a.t_end = INFINITY;
   a.dtc = 1.0;
   a.xin = nullptr; a.xout = nullptr;

@@ -184,9 +184,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
   P.gm1 = T(a.gamma - 1.0);
   P.half = T(0.5);
   P.g = T(a.g);
+  P.ggm1 = T(a.gamma * (a.gamma - 1.0));
+  P.gfloor = T(a.gamma * 1e-10);
   const T odx = T(a.odx), ody = T(a.ody);
   // stage combination out = alpha*U^n + beta*D + (beta*dt)*L
   const T alpha = T(a.alpha), beta = T(a.beta), bdt = T(a.beta * dt);
+  // the deviation form folds beta*dt into the residual's factors
+  const T kx = T(a.beta * dt * a.odx), ky = T(a.beta * dt * a.ody), kg = T(a.beta * dt * a.g);
 
   // separable FP64 equilibrium: rho = (rho0 e^{-kx}) e^{-ky} is formed in FP64
   // and rounded once; p, E and E+p scale with rho by constant ratios.
@@ -198,16 +202,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
   }
   const T por = T(a.p_over_rho), eor = T(a.p_over_rho * a.inv_gm1);
   const T epor = T(a.p_over_rho * a.gamma * a.inv_gm1);
+  const T gpor = T(a.p_over_rho * a.gamma);
   auto face_eq = [&](double rx, double ey) -> FaceEq<T> {
     FaceEq<T> f;
     f.rho = T(rx * ey);
     f.p = f.rho * por;
     f.e = f.rho * eor;
     f.ep = f.rho * epor;
+    f.gp = f.rho * gpor;
     return f;
   };
 
-  double smax = 0.0;
+  // the wave-speed maximum in the working precision (exact when widened)
+  T smax = T(0);
   bool bad = false;
 
   // ---- the sweep ------------------------------------------------------------------
@@ -317,21 +324,32 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
         // the same expression in both directions, on the flux above (Ga) and below (Gb)
         const Vec4<T>& Ga = DIR > 0 ? Gn.c[k] : Gp.c[k];
         const Vec4<T>& Gb = DIR > 0 ? Gp.c[k] : Gn.c[k];
-        Vec4<T> L;
-        L.r = -(F[k + 1].r - F[k].r) * odx - (Ga.r - Gb.r) * ody;
-        L.mx = (-(F[k + 1].mx - F[k].mx) * odx - (Ga.mx - Gb.mx) * ody) - P.g * D.r;
-        L.my = (-(F[k + 1].my - F[k].my) * odx - (Ga.my - Gb.my) * ody) - P.g * D.r;
-        L.e = (-(F[k + 1].e - F[k].e) * odx - (Ga.e - Gb.e) * ody) - P.g * (D.mx + D.my);
-        if (BC == BC_FROZEN) {
-          const int col = x0 + k;
-          if (col == 0 || col == nx - 1 || j == 0 || j == ny - 1) L = {T(0), T(0), T(0), T(0)};
-        }
-        if (MODE == MODE_RESIDUAL) {
-          O.c[k] = L;
-        } else if (MODE == MODE_STAGE1) {
-          O.c[k] = D + scale(bdt, L);
+        const bool frozen =
+            BC == BC_FROZEN && (x0 + k == 0 || x0 + k == nx - 1 || j == 0 || j == ny - 1);
+        if (FORM == FORM_DEV && MODE != MODE_RESIDUAL) {
+          // out = base - kx dF - ky dG + kg s, base = D (stage 1) or alpha U^n + beta D
+          Vec4<T> B = D;
+          if (MODE == MODE_STAGE) B = scale(alpha, UA.c[k]) + scale(beta, D);
+          Vec4<T> Q;
+          Q.r = B.r - kx * (F[k + 1].r - F[k].r) - ky * (Ga.r - Gb.r);
+          Q.mx = B.mx - kx * (F[k + 1].mx - F[k].mx) - ky * (Ga.mx - Gb.mx) - kg * D.r;
+          Q.my = B.my - kx * (F[k + 1].my - F[k].my) - ky * (Ga.my - Gb.my) - kg * D.r;
+          Q.e = B.e - kx * (F[k + 1].e - F[k].e) - ky * (Ga.e - Gb.e) - kg * (D.mx + D.my);
+          O.c[k] = frozen ? B : Q;
         } else {
-          O.c[k] = scale(alpha, UA.c[k]) + (scale(beta, D) + scale(bdt, L));
+          Vec4<T> L;
+          L.r = -(F[k + 1].r - F[k].r) * odx - (Ga.r - Gb.r) * ody;
+          L.mx = (-(F[k + 1].mx - F[k].mx) * odx - (Ga.mx - Gb.mx) * ody) - P.g * D.r;
+          L.my = (-(F[k + 1].my - F[k].my) * odx - (Ga.my - Gb.my) * ody) - P.g * D.r;
+          L.e = (-(F[k + 1].e - F[k].e) * odx - (Ga.e - Gb.e) * ody) - P.g * (D.mx + D.my);
+          if (frozen) L = {T(0), T(0), T(0), T(0)};
+          if (MODE == MODE_RESIDUAL) {
+            O.c[k] = L;
+          } else if (MODE == MODE_STAGE1) {
+            O.c[k] = D + scale(bdt, L);
+          } else {
+            O.c[k] = scale(alpha, UA.c[k]) + (scale(beta, D) + scale(bdt, L));
+          }
         }
       }
       Gp = Gn;
@@ -354,7 +372,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
               bool ok;
               T rct = T(rxc[k] * eyc);
               T sp = cell_speed(P, rct, rct * eor, O.c[k], ok);
-              smax = fmax(smax, (double)sp);
+              smax = tmax(smax, sp);
               bad |= !ok;
             }
           }
@@ -377,9 +395,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
 
   if (MODE != MODE_RESIDUAL && a.last) {
     if (bad) atomicOr(&a.ctrl->err, 1u);
+    double sm = (double)smax;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
-    if (lane == 0) atomicMax(&a.ctrl->cmax[q][slot].v, (unsigned long long)__double_as_longlong(smax));
+    for (int off = 32; off > 0; off >>= 1) sm = fmax(sm, __shfl_xor(sm, off));
+    if (lane == 0) atomicMax(&a.ctrl->cmax[q][slot].v, (unsigned long long)__double_as_longlong(sm));
   }
 }
 
@@ -496,6 +515,7 @@ __global__ void cfl_init_kernel(const T* __restrict__ in, const double* __restri
                                 double inv_gm1) {
   Phys<T> P;
   P.gamma = T(gamma); P.gm1 = T(gamma - 1.0); P.half = T(0.5); P.g = T(0);
+  P.ggm1 = T(gamma * (gamma - 1.0)); P.gfloor = T(gamma * 1e-10);
   double m = 0.0;
   bool bad = false;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < (long long)nx * ny;
