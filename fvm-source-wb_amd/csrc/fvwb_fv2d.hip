@@ -26,6 +26,14 @@ enum { MODE_RESIDUAL = 0, MODE_STAGE1 = 1, MODE_STAGE = 2 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Read of a table that no kernel writes while a stage runs, through the
+// constant address space: at a wave-uniform index it becomes a scalar load,
+// which waits on its own counter and so never holds up on the row loads in
+// flight (a vector load would, as vector loads complete in order).
+__device__ __forceinline__ double table_at(const double* t, int k) {
+  return ((const __attribute__((address_space(4))) double*)t)[k];
+}
+
 // Blocks are dealt round-robin to the 8 XCDs; renumber them so that each XCD
 // receives a contiguous range of strips (neighbouring halos share its L2).
 __device__ __forceinline__ int xcd_block(int b, int nb) {
@@ -168,6 +176,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
     }
     return b;
   };
+  // interior strips: every row and column read lies inside the grid, so a row
+  // is four plain block loads with nothing applied to the loaded values (no
+  // mirror, no per-column path), which lets the loads stay in flight until the
+  // row that uses them
+  auto load_row_interior = [&](const T* base, int j) -> Blk<T, CPL> {
+    const T* r = base + (long long)clampi(j, 0, ny - 1) * pitch;
+    Lanes<T, CPL> v0 = ldv<T, CPL>(r, vboff, rowb), v1 = ldv<T, CPL>(r + plane, vboff, rowb),
+                  v2 = ldv<T, CPL>(r + 2 * plane, vboff, rowb), v3 = ldv<T, CPL>(r + 3 * plane, vboff, rowb);
+    Blk<T, CPL> b;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) b.c[k] = Vec4<T>{v0.v[k], v1.v[k], v2.v[k], v3.v[k]};
+    return b;
+  };
   // cell-centred rows (clamped into the grid; rows past the strip are masked)
   auto load_cell = [&](const T* base, int j) -> Blk<T, CPL> {
     const T* r = base + (long long)clampi(j, 0, ny - 1) * pitch;
@@ -228,8 +249,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
   // of URING.  The body is unrolled RING times with rotated roles, so the rings
   // never move data.  (A deeper ring measured no faster: the strip sweep is
   // limited by its memory access pattern, not by loads in flight per wave.)
-  auto sweep = [&](auto dirc) {
+  auto sweep = [&](auto dirc, auto interiorc) {
     constexpr int DIR = decltype(dirc)::value;
+    auto load_row_s = [&](const T* base, int j) -> Blk<T, CPL> {
+      if constexpr (decltype(interiorc)::value) return load_row_interior(base, j);
+      else return load_row(base, j);
+    };
     constexpr int RING = 4;
     constexpr int URING = 2;
     // slopes are differences along the sweep (minmod2(-a,-b) = -minmod2(a,b)), so a
@@ -243,12 +268,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
       else return face_flux<T, FLUX, 1, FORM>(P, qy, Da, Db);
     };
     {
-      const double eyf0 = a.eyf[clampi(js + (DIR < 0), 0, ny)];
-      Blk<T, CPL> Dm1 = load_row(a.in, js - DIR);
+      const double eyf0 = table_at(a.eyf, clampi(js + (DIR < 0), 0, ny));
+      Blk<T, CPL> Dm1 = load_row_s(a.in, js - DIR);
 #pragma unroll
-      for (int k = 0; k < RING - 1; ++k) R[k] = load_row(a.in, js + k * DIR);
+      for (int k = 0; k < RING - 1; ++k) R[k] = load_row_s(a.in, js + k * DIR);
       Blk<T, CPL> Dm2;
-      if (RECON) Dm2 = load_row(a.in, js - 2 * DIR);
+      if (RECON) Dm2 = load_row_s(a.in, js - 2 * DIR);
 #pragma unroll
       for (int k = 0; k < CPL; ++k) {
         FaceEq<T> qy = face_eq(rxc[k], eyf0);
@@ -272,11 +297,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
     // j+(RING-1)DIR; UA holds U^n(j) and US receives U^n(j+(URING-1)DIR).
     auto row_step = [&](int j, const Blk<T, CPL>& A, const Blk<T, CPL>& B, const Blk<T, CPL>& C,
                         Blk<T, CPL>& S, const Blk<T, CPL>& UA, Blk<T, CPL>& US) {
-      S = load_row(a.in, j + (RING - 1) * DIR);
+      S = load_row_s(a.in, j + (RING - 1) * DIR);
       if (MODE == MODE_STAGE) US = load_cell(a.un, j + (URING - 1) * DIR);
       const int jt = clampi(j, 0, ny - 1);
-      const double eyc = a.eyc[jt];
-      const double eyf = a.eyf[jt + (DIR > 0)];
+      const double eyc = table_at(a.eyc, jt);
+      const double eyf = table_at(a.eyf, jt + (DIR > 0));
 
       // y faces between row j and row j+DIR
       Blk<T, CPL> Gn;
@@ -388,10 +413,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
                  Ur[k % URING], Ur[(k + URING - 1) % URING]);
     }
   };
-  if (ty & 1)
-    sweep(std::integral_constant<int, -1>{});
-  else
-    sweep(std::integral_constant<int, 1>{});
+  // rows read: y0-3 .. y1+2 (the ring runs one row past each end of the strip)
+  const bool interior = fast && y0 >= 3 && y1 + 3 <= ny;
+  if (ty & 1) {
+    if (interior) sweep(std::integral_constant<int, -1>{}, std::true_type{});
+    else sweep(std::integral_constant<int, -1>{}, std::false_type{});
+  } else {
+    if (interior) sweep(std::integral_constant<int, 1>{}, std::true_type{});
+    else sweep(std::integral_constant<int, 1>{}, std::false_type{});
+  }
 
   if (MODE != MODE_RESIDUAL && a.last) {
     if (bad) atomicOr(&a.ctrl->err, 1u);
