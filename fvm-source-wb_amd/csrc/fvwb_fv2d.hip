@@ -200,6 +200,23 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
     return b;
   };
 
+  // a face state D + h*s of a reconstruction (h = +-1/2).  The deviation form
+  // writes it as one fused multiply-add per variable, so every launch shape
+  // rounds it the same way: h*s is exact for normal numbers, but not for
+  // subnormal ones, where a separate product would round first.  The FP64
+  // form keeps the reference's product-then-sum.
+  auto face_state = [](const Vec4<T>& D, T h, const Vec4<T>& s) -> Vec4<T> {
+    if constexpr (FORM == FORM_DEV) {
+      auto fm = [](T x, T y, T z) -> T {
+        if constexpr (sizeof(T) == 4) return __builtin_fmaf(x, y, z);
+        else return __builtin_fma(x, y, z);
+      };
+      return {fm(h, s.r, D.r), fm(h, s.mx, D.mx), fm(h, s.my, D.my), fm(h, s.e, D.e)};
+    } else {
+      return D + scale(h, s);
+    }
+  };
+
   Phys<T> P;
   P.gamma = T(a.gamma);
   P.gm1 = T(a.gamma - 1.0);
@@ -280,7 +297,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
         if (RECON) {
           Vec4<T> sym1 = minmod2(Dm1.c[k] - Dm2.c[k], R[0].c[k] - Dm1.c[k]);
           sy.c[k] = minmod2(R[0].c[k] - Dm1.c[k], R[1].c[k] - R[0].c[k]);
-          Gp.c[k] = yface(qy, Dm1.c[k] + scale(hd, sym1), R[0].c[k] - scale(hd, sy.c[k]));
+          Gp.c[k] = yface(qy, face_state(Dm1.c[k], hd, sym1), face_state(R[0].c[k], -hd, sy.c[k]));
         } else {
           Gp.c[k] = yface(qy, Dm1.c[k], R[0].c[k]);
         }
@@ -310,7 +327,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
         FaceEq<T> qy = face_eq(rxc[k], eyf);
         if (RECON) {
           Vec4<T> syn = minmod2(B.c[k] - A.c[k], C.c[k] - B.c[k]);
-          Gn.c[k] = yface(qy, A.c[k] + scale(hd, sy.c[k]), B.c[k] - scale(hd, syn));
+          Gn.c[k] = yface(qy, face_state(A.c[k], hd, sy.c[k]), face_state(B.c[k], -hd, syn));
           sy.c[k] = syn;
         } else {
           Gn.c[k] = yface(qy, A.c[k], B.c[k]);
@@ -327,8 +344,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
           Vec4<T> am = k == 0 ? lm : A.c[k - 1];
           Vec4<T> ap = k == CPL - 1 ? rp : A.c[k + 1];
           Vec4<T> sx = minmod2(A.c[k] - am, ap - A.c[k]);
-          xr[k] = A.c[k] + scale(P.half, sx);
-          xl[k] = A.c[k] - scale(P.half, sx);
+          xr[k] = face_state(A.c[k], P.half, sx);
+          xl[k] = face_state(A.c[k], -P.half, sx);
         }
       } else {
 #pragma unroll
