@@ -273,7 +273,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
       else return load_row(base, j);
     };
     constexpr int RING = 4;
-    constexpr int URING = 2;
+    // FP64 holds U^n for one row only (loaded as the row starts, used as it
+    // ends): that keeps its stage-2/3 kernel under 256 VGPRs, two waves per
+    // SIMD instead of one.  FP32 keeps two rows at three waves, which measured
+    // faster than one row at four.
+    constexpr int URING = sizeof(T) == 8 ? 1 : 2;
     // slopes are differences along the sweep (minmod2(-a,-b) = -minmod2(a,b)), so a
     // row's state toward the next row is D + sy/2 and toward the previous D - sy/2
     const T hd = P.half;
