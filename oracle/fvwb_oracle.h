@@ -170,6 +170,31 @@ void   orc_dg1d_initial(const fvwb_dg1d_params* p, int ninit, double pert, doubl
 int    orc_dg1d_run_rki(const fvwb_dg1d_params* p, double* delta_u, const double* u_eq, double* u_init,
                         double t_end, int max_steps, double* t_out, double* ts, double* dts);
 
+/* ---- 2D residual distribution (benchmark_rd.f90) --------------------------- */
+/* the values the reference reads from its (unshipped) module parameters_rd */
+typedef struct {
+  int nx, ny;          /* nodes per side (x_i = (i-1) boxlen_x/(nx-1)) */
+  int bc;              /* 2 "periodic" copies, 3 "transmissive" copies, 0 none (oracle only) */
+  int pad;
+  double gamma, boxlen_x, boxlen_y, cfl;
+} orc_rd_params;
+void   orc_rd_primitive(double gamma, const double u[4], double w[4]);          /* :521-532 */
+void   orc_rd_conservative(double gamma, const double w[4], double u[4]);       /* :535-547 */
+void   orc_rd_flux(double gamma, const double u[4], double f1[4], double f2[4]); /* :549-571 */
+void   orc_rd_source(const double w[4], double s[4]);                           /* :747-767 */
+double orc_rd_speed(double gamma, const double u[4]);                           /* :478-493 */
+double orc_rd_max_speed(const orc_rd_params* p, const double* u);               /* :448-475 */
+/* corner residuals 11, 12, 21, 22 of cell (i, j) (:667-708) */
+void   orc_rd_cell(const orc_rd_params* p, const double* u, const double* diff, double dt, int i, int j,
+                   double phi[4][4]);
+/* compute_update_rd (:612-745); u, diff, dudt AoS(4,nx,ny); -1 on bad parameters */
+int    orc_rd_update(const orc_rd_params* p, const double* u, const double* diff, double dt, double* dudt);
+/* one 'RD1' step of evolve (:346-364); returns dt, -1 on bad parameters */
+double orc_rd_step(const orc_rd_params* p, double* u, double t, double t_end);
+int    orc_rd_run(const orc_rd_params* p, double* u, double t_end, int max_steps, double* t_out, int* n_out);
+/* get_initial_conditions cases 1, 2, 8 (:53-265); -1 for another case */
+int    orc_rd_initial(const orc_rd_params* p, int ninit, double eta, double* u);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,22 @@ def make_params(nx, ny, *, bc=BC_FROZEN_RING, recon=RECON_FIRST_ORDER, flux=FLUX
                   boxlen_y, cfl, rho0, p0, g)
 
 
+class RDParams(ctypes.Structure):
+    """orc_rd_params: the values benchmark_rd.f90 reads from its module
+    parameters_rd (bc 2 periodic copies, 3 transmissive copies, 0 none)."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("bc", ctypes.c_int), ("pad", ctypes.c_int),
+                ("gamma", ctypes.c_double), ("boxlen_x", ctypes.c_double), ("boxlen_y", ctypes.c_double),
+                ("cfl", ctypes.c_double)]
+
+
+RD_BC_NONE, RD_BC_PERIODIC, RD_BC_TRANSMISSIVE = 0, 2, 3
+
+
+def make_rd_params(nx, ny, *, bc=RD_BC_TRANSMISSIVE, gamma=GAMMA_REF, boxlen_x=1.0, boxlen_y=1.0,
+                   cfl=0.5) -> RDParams:
+    return RDParams(nx, ny, bc, 0, gamma, boxlen_x, boxlen_y, cfl)
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-B", "-C", HERE, "liboracle_fvwb.so"], check=True)
 
@@ -199,6 +215,19 @@ def lib():
         L.orc_dg2d_ssp54_coeffs.argtypes = [dp]
         L.orc_dg2d_nodal.argtypes = [pd, dp, dp]
         L.orc_dg2d_project.argtypes = [pd, dp, dp]
+        pr = ctypes.POINTER(RDParams)
+        L.orc_rd_update.argtypes = [pr, dp, dp, ctypes.c_double, dp]
+        L.orc_rd_update.restype = ctypes.c_int
+        L.orc_rd_cell.argtypes = [pr, dp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_rd_max_speed.argtypes = [pr, dp]
+        L.orc_rd_max_speed.restype = ctypes.c_double
+        L.orc_rd_step.argtypes = [pr, dp, ctypes.c_double, ctypes.c_double]
+        L.orc_rd_step.restype = ctypes.c_double
+        L.orc_rd_run.argtypes = [pr, dp, ctypes.c_double, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_int)]
+        L.orc_rd_run.restype = ctypes.c_int
+        L.orc_rd_initial.argtypes = [pr, ctypes.c_int, ctypes.c_double, dp]
+        L.orc_rd_initial.restype = ctypes.c_int
+        L.orc_rd_conservative.argtypes = [ctypes.c_double, dp, dp]
         _lib = L
     return _lib
 
@@ -704,3 +733,59 @@ def dg2d_fourier_state(p: DGParams, amp=1e-4, modes=None):
     nodal = np.zeros((4, p.nx, p.ny, X.shape[2]), order="F")
     nodal[3] = dp / (p.gamma - 1.0)
     return dg2d_project(p, nodal)
+
+
+# ---- residual distribution (benchmark_rd.f90) ----------------------------------
+def rd_update(p: RDParams, u, diff, dt):
+    """compute_update_rd: dudt AoS(4,nx,ny) of the state u and the correction diff."""
+    u, diff = fortran(u), fortran(diff)
+    out = np.zeros((4, p.nx, p.ny), order="F")
+    rc = lib().orc_rd_update(ctypes.byref(p), _p(u), _p(diff), float(dt), _p(out))
+    if rc != 0:
+        raise ValueError("orc_rd_update: bad parameters")
+    return out
+
+
+def rd_cell(p: RDParams, u, diff, dt, i, j):
+    """Corner residuals (11, 12, 21, 22) x 4 variables of cell (i, j)."""
+    u, diff = fortran(u), fortran(diff)
+    phi = np.zeros(16)
+    lib().orc_rd_cell(ctypes.byref(p), _p(u), _p(diff), float(dt), int(i), int(j), _p(phi))
+    return phi.reshape(4, 4)
+
+
+def rd_max_speed(p: RDParams, u):
+    return lib().orc_rd_max_speed(ctypes.byref(p), _p(fortran(u)))
+
+
+def rd_step(p: RDParams, u, t=0.0, t_end=float("inf")):
+    """One 'RD1' step in place; returns dt."""
+    assert u.flags.f_contiguous and u.dtype == np.float64
+    dt = lib().orc_rd_step(ctypes.byref(p), _p(u), float(t), float(t_end))
+    if dt < 0:
+        raise ValueError("orc_rd_step: bad parameters")
+    return dt
+
+
+def rd_run(p: RDParams, u, t_end, max_steps=1 << 30):
+    """evolve's loop; returns (u, t, steps)."""
+    u = fortran(u).copy(order="F")
+    t = ctypes.c_double(0.0)
+    n = ctypes.c_int(0)
+    if lib().orc_rd_run(ctypes.byref(p), _p(u), float(t_end), int(max_steps), ctypes.byref(t), ctypes.byref(n)):
+        raise ValueError("orc_rd_run: bad parameters")
+    return u, t.value, n.value
+
+
+def rd_initial(p: RDParams, ninit, eta=1e-3):
+    u = np.zeros((4, p.nx, p.ny), order="F")
+    if lib().orc_rd_initial(ctypes.byref(p), int(ninit), float(eta), _p(u)):
+        raise ValueError(f"orc_rd_initial: no case {ninit}")
+    return u
+
+
+def rd_conservative(gamma, w):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    u = np.zeros(4)
+    lib().orc_rd_conservative(float(gamma), _p(w), _p(u))
+    return u
