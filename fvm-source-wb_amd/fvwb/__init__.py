@@ -109,6 +109,25 @@ class DG1DParams(ctypes.Structure):
     ]
 
 
+class RDParams(ctypes.Structure):
+    """``fvwb_rd_params`` of include/fvwb.h (benchmark_rd.f90's parameters_rd)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("bc", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("boxlen_x", ctypes.c_double), ("boxlen_y", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+    ]
+
+
+RD_BC_PERIODIC, RD_BC_TRANSMISSIVE = 2, 3
+
+
+def make_rd_params(nx, ny, *, bc=RD_BC_TRANSMISSIVE, gamma=float(np.float32(1.4)), boxlen_x=1.0, boxlen_y=1.0,
+                   cfl=0.5) -> RDParams:
+    """gamma defaults to the reference's default-real 1.4, as its other parameter modules set it."""
+    return RDParams(nx, ny, bc, 0, gamma, boxlen_x, boxlen_y, cfl)
+
+
 def make_dg_params(nx, ny, *, degree=2, flux=FLUX_HLL, limiter=DG_LIMITER_TVB, gamma=1.4, boxlen_x=1.0,
                    boxlen_y=1.0, cfl=0.8, rho0=1.0, p0=1.0, g=1.0, tvb_m=50.0, integrator=0) -> DGParams:
     """Defaults: the DG gravity workload (P2, HLL, TVB with M = 50, walls, CFL 0.8)."""
@@ -202,6 +221,17 @@ def lib():
         "fvwb_dg1d_download": [H, dp, dp],
         "fvwb_dg1d_get_time": [H, dp, dp, ip],
     })
+    prd = ctypes.POINTER(RDParams)
+    sig.update({
+        "fvwb_rd_create": [ctypes.POINTER(H), prd],
+        "fvwb_rd_destroy": [H],
+        "fvwb_rd_compute_update": [H, dp, dp, ctypes.c_double, dp],
+        "fvwb_rd_compute_max_speed": [H, dp, dp],
+        "fvwb_rd_upload": [H, dp],
+        "fvwb_rd_advance": [H, ctypes.c_int32, ctypes.c_double],
+        "fvwb_rd_download": [H, dp],
+        "fvwb_rd_get_time": [H, dp, dp, ctypes.POINTER(ctypes.c_int64)],
+    })
     p1 = ctypes.POINTER(Params1D)
     sig.update({
         "fvwb1d_params_reference": [p1, ctypes.c_int32, ctypes.c_int32],
@@ -234,6 +264,8 @@ def lib():
     L.fvwb_dg_last_error.restype = ctypes.c_char_p
     L.fvwb_dg1d_last_error.argtypes = [H]
     L.fvwb_dg1d_last_error.restype = ctypes.c_char_p
+    L.fvwb_rd_last_error.argtypes = [H]
+    L.fvwb_rd_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -811,3 +843,75 @@ class SolverDG1D:
 
     def checkpoint_load(self, path):
         _checkdg1d(lib().fvwb_dg1d_checkpoint_load(self._h, os.fsencode(path)), self._h)
+
+
+def _checkrd(rc, h=None):
+    if rc != OK:
+        msg = lib().fvwb_rd_last_error(h)
+        raise FvwbError(rc, msg.decode() if msg else "")
+
+
+class SolverRD:
+    """The reference's 2D residual distribution (benchmark_rd.f90, solver 'RD1')
+    on the GPU (FP64).  States are float64 AoS(4,nx,ny): numpy shape (4, nx, ny),
+    order="F", the reference's u(1:4,1:nx,1:ny) with mx = my = 1."""
+
+    def __init__(self, params: RDParams):
+        self.params = params
+        self.shape = (4, params.nx, params.ny)
+        self._h = ctypes.c_void_p()
+        _checkrd(lib().fvwb_rd_create(ctypes.byref(self._h), ctypes.byref(params)))
+
+    def close(self):
+        if self._h:
+            lib().fvwb_rd_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _arr(self, a):
+        a = np.asfortranarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise FvwbError(ERR_ARG, f"expected shape {self.shape}, got {a.shape}")
+        return a
+
+    def compute_update(self, u, diff, dt):
+        """dudt of compute_update_rd(u, diff, dt, x, y, dudt) (benchmark_rd.f90:612-745)."""
+        a, d = self._arr(u), self._arr(diff)
+        out = np.zeros(self.shape, order="F")
+        _checkrd(lib().fvwb_rd_compute_update(self._h, _dp(a), _dp(d), float(dt), _dp(out)), self._h)
+        return out
+
+    def compute_max_speed(self, u):
+        """compute_max_speed(u, speed_max) (benchmark_rd.f90:448-475)."""
+        c = ctypes.c_double()
+        _checkrd(lib().fvwb_rd_compute_max_speed(self._h, _dp(self._arr(u)), ctypes.byref(c)), self._h)
+        return c.value
+
+    def upload(self, u):
+        _checkrd(lib().fvwb_rd_upload(self._h, _dp(self._arr(u))), self._h)
+
+    def advance(self, nsteps=(1 << 31) - 1, t_end=math.inf):
+        """evolve's loop (benchmark_rd.f90:356-385) while t < t_end, at most nsteps steps."""
+        _checkrd(lib().fvwb_rd_advance(self._h, int(nsteps), float(t_end)), self._h)
+
+    def download(self):
+        out = np.zeros(self.shape, order="F")
+        _checkrd(lib().fvwb_rd_download(self._h, _dp(out)), self._h)
+        return out
+
+    def time(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int64()
+        _checkrd(lib().fvwb_rd_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), self._h)
+        return t.value, dt.value, n.value

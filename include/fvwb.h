@@ -468,6 +468,42 @@ int fvwb_dg_checkpoint_load(fvwb_dg_handle h, const char* path);
 int fvwb_dg1d_checkpoint_save(fvwb_dg1d_handle h, const char* path);
 int fvwb_dg1d_checkpoint_load(fvwb_dg1d_handle h, const char* path);
 
+/* ---- 2D residual distribution (benchmark_rd.f90, solver 'RD1') ---------------
+ * A vertex scheme on nx x ny nodes x_i = (i-1) boxlen_x/(nx-1), y_j likewise,
+ * state u(1:4, 1:nx, 1:ny) as the reference holds it with mx = my = 1 (AoS,
+ * FP64, x fastest after the variable).  The parameters are those the
+ * reference reads from its module parameters_rd, which it does not ship. */
+#define FVWB_RD_BC_PERIODIC     2   /* benchmark_rd.f90:723-727: copies, x then y */
+#define FVWB_RD_BC_TRANSMISSIVE 3   /* benchmark_rd.f90:729-733 */
+
+typedef struct {
+  int32_t nx, ny;      /* nodes per side, >= 2 */
+  int32_t bc;          /* FVWB_RD_BC_PERIODIC or FVWB_RD_BC_TRANSMISSIVE (others stop the reference) */
+  int32_t pad;
+  double gamma, boxlen_x, boxlen_y, cfl;
+} fvwb_rd_params;
+
+typedef struct fvwb_rd* fvwb_rd_handle;
+
+int fvwb_rd_create(fvwb_rd_handle* out, const fvwb_rd_params* p);
+int fvwb_rd_destroy(fvwb_rd_handle h);
+const char* fvwb_rd_last_error(fvwb_rd_handle h);
+/* benchmark_rd.f90:612-745 compute_update_rd(u, diff, dt, x, y, dudt): host
+ * arrays AoS(4,nx,ny); the reference's x and y only feed get_source's unused
+ * coordinates.  dudt equals the reference's expressions bit for bit. */
+int fvwb_rd_compute_update(fvwb_rd_handle h, const double* u, const double* diff, double dt, double* dudt);
+/* benchmark_rd.f90:448-475 compute_max_speed(u, speed_max) */
+int fvwb_rd_compute_max_speed(fvwb_rd_handle h, const double* u, double* cmax);
+/* benchmark_rd.f90:333-388 evolve with solver 'RD1', state resident on the
+ * device: upload resets the clock; advance takes at most nsteps steps while
+ * t < t_end (dt = min(cfl dx dy/cmax/3, t_end - t), two update passes per
+ * step); FVWB_ERR_NONPHYSICAL if the wave-speed maximum is not finite and
+ * positive. */
+int fvwb_rd_upload(fvwb_rd_handle h, const double* u);
+int fvwb_rd_advance(fvwb_rd_handle h, int32_t nsteps, double t_end);
+int fvwb_rd_download(fvwb_rd_handle h, double* u);
+int fvwb_rd_get_time(fvwb_rd_handle h, double* t, double* dt_last, int64_t* steps_done);
+
 #ifdef __cplusplus
 }
 #endif
