@@ -54,6 +54,19 @@ module fvwb_iface
   public :: fvwb_dg1d_compute_update_exact_delta, fvwb_dg1d_upload, fvwb_dg1d_advance
   public :: fvwb_dg1d_download, fvwb_dg1d_get_time, fvwb_dg1d_check
 
+  ! ---- 2D residual distribution (benchmark_rd.f90) ----
+  integer(c_int32_t), parameter, public :: FVWB_RD_BC_PERIODIC = 2, FVWB_RD_BC_TRANSMISSIVE = 3
+
+  ! struct fvwb_rd_params: four int32 then four doubles (16 + 32 bytes)
+  type, bind(C), public :: fvwb_rd_params
+     integer(c_int32_t) :: nx, ny, bc, pad
+     real(c_double) :: gamma, boxlen_x, boxlen_y, cfl
+  end type fvwb_rd_params
+
+  public :: fvwb_rd_create, fvwb_rd_destroy, fvwb_rd_last_error, fvwb_rd_compute_update
+  public :: fvwb_rd_compute_max_speed, fvwb_rd_upload, fvwb_rd_advance, fvwb_rd_download
+  public :: fvwb_rd_get_time, fvwb_rd_check
+
   ! struct fvwb1d_params: six int32 then four doubles (24 + 32 bytes)
   type, bind(C), public :: fvwb1d_params
      integer(c_int32_t) :: nvar, nx, bc, recon, integrator, nequilibrium
@@ -428,6 +441,67 @@ module fvwb_iface
        real(c_double), intent(out) :: t, dt_last
        integer(c_int32_t), intent(out) :: steps
      end function fvwb_dg1d_get_time
+
+     ! ---- 2D residual distribution ----
+     integer(c_int) function fvwb_rd_create(h, p) bind(C, name='fvwb_rd_create')
+       import :: c_int, c_ptr, fvwb_rd_params
+       type(c_ptr), intent(out) :: h
+       type(fvwb_rd_params), intent(in) :: p
+     end function fvwb_rd_create
+
+     integer(c_int) function fvwb_rd_destroy(h) bind(C, name='fvwb_rd_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_rd_destroy
+
+     type(c_ptr) function fvwb_rd_last_error(h) bind(C, name='fvwb_rd_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_rd_last_error
+
+     ! drop-in for benchmark_rd.f90:612-745 (compute_update_rd(u,diff,dt,x,y,dudt))
+     integer(c_int) function fvwb_rd_compute_update(h, u, diff, dt, dudt) bind(C, name='fvwb_rd_compute_update')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*), diff(*)
+       real(c_double), value :: dt
+       real(c_double), intent(out) :: dudt(*)
+     end function fvwb_rd_compute_update
+
+     ! drop-in for benchmark_rd.f90:448-475 (compute_max_speed(u,speed_max))
+     integer(c_int) function fvwb_rd_compute_max_speed(h, u, cmax) bind(C, name='fvwb_rd_compute_max_speed')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*)
+       real(c_double), intent(out) :: cmax
+     end function fvwb_rd_compute_max_speed
+
+     ! evolve with solver 'RD1' (benchmark_rd.f90:333-388), resident on the device
+     integer(c_int) function fvwb_rd_upload(h, u) bind(C, name='fvwb_rd_upload')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*)
+     end function fvwb_rd_upload
+
+     integer(c_int) function fvwb_rd_advance(h, nsteps, t_end) bind(C, name='fvwb_rd_advance')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nsteps
+       real(c_double), value :: t_end
+     end function fvwb_rd_advance
+
+     integer(c_int) function fvwb_rd_download(h, u) bind(C, name='fvwb_rd_download')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: u(*)
+     end function fvwb_rd_download
+
+     integer(c_int) function fvwb_rd_get_time(h, t, dt_last, steps) bind(C, name='fvwb_rd_get_time')
+       import :: c_int, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: t, dt_last
+       integer(c_int64_t), intent(out) :: steps
+     end function fvwb_rd_get_time
   end interface
 
 contains
@@ -497,5 +571,16 @@ contains
        error stop 1
     end if
   end subroutine fvwb_dg1d_check
+
+  subroutine fvwb_rd_check(status, h, what)
+    integer(c_int), intent(in) :: status
+    type(c_ptr), intent(in) :: h
+    character(len=*), intent(in) :: what
+    if (status /= FVWB_OK) then
+       write(0, '(a,a,a,i0,a,a)') 'fvwb: ', what, ' failed with status ', status, ': ', &
+            fvwb_message(fvwb_rd_last_error(h))
+       error stop 1
+    end if
+  end subroutine fvwb_rd_check
 
 end module fvwb_iface

@@ -112,6 +112,16 @@ def test_dg1d_driver_binds_dropin_to_library():
         assert sym in undefined and sym in fvwb.declared_symbols()
 
 
+def test_rd_driver_binds_dropin_to_library():
+    nm = subprocess.run(["nm", exe("rd_fvwb")], capture_output=True, text=True, check=True).stdout
+    defined = set(re.findall(r"^[0-9a-f]+ T (\S+)$", nm, re.M))
+    undefined = set(re.findall(r"^\s+U (\S+)$", nm, re.M))
+    assert "compute_update_rd_" in defined and "compute_max_speed_" in defined
+    for sym in ("fvwb_rd_compute_update", "fvwb_rd_compute_max_speed", "fvwb_rd_create", "fvwb_rd_upload",
+                "fvwb_rd_advance"):
+        assert sym in undefined and sym in fvwb.declared_symbols()
+
+
 def test_driver_without_device_fails_loudly():
     """Without a usable GPU the driver stops with the library's error instead
     of computing anything on the host."""
@@ -387,3 +397,28 @@ def test_driver_resume_from_checkpoint_equals_one_run(tmp_path):
     print(f"checkpoint at step {sa} (t {ta}); resumed {sb} steps to {tb}; one run {sc} to {tc}", flush=True)
     assert 0 < sa < sb and sb == sc and tb == tc
     assert np.array_equal(ub, uc)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["stage", "resident"])
+def test_rd_driver_matches_oracle(tmp_path, mode):
+    """The residual-distribution program (65x65 nodes, ninit = 8, transmissive
+    copies, 'RD1', tend = 0.005) with the drop-ins on the host loop (stage) or
+    the device-resident loop (resident): from the driver's own initial state
+    the oracle's restatement of evolve takes the same steps to the same t and
+    state, bit for bit; the initial state itself equals the oracle's to libm
+    rounding."""
+    prefix = str(tmp_path / "rd")
+    r = run([exe("rd_fvwb"), mode, "0.005", prefix])
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(prefix + ".bin", "rb") as f:
+        nx, ny, steps, _ = np.frombuffer(f.read(16), dtype=np.int32)
+        t = np.frombuffer(f.read(8), dtype=np.float64)[0]
+        u0 = np.frombuffer(f.read(8 * 4 * nx * ny), dtype=np.float64).reshape((4, nx, ny), order="F")
+        u = np.frombuffer(f.read(8 * 4 * nx * ny), dtype=np.float64).reshape((4, nx, ny), order="F")
+    op = po.make_rd_params(int(nx), int(ny), bc=po.RD_BC_TRANSMISSIVE, cfl=0.5)
+    np.testing.assert_allclose(u0, po.rd_initial(op, 8, float(np.float32(0.01))), rtol=1e-15, atol=0)
+    uo, to, no = po.rd_run(op, np.asfortranarray(u0), 0.005)
+    print(f"{mode}: steps {steps} vs {no}, t {t!r} vs {to!r}", flush=True)
+    assert (nx, ny) == (65, 65) and steps == no == 162 and t == to
+    np.testing.assert_array_equal(u, uo)
