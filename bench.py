@@ -189,6 +189,31 @@ def time_fv1d(ev, ninst, n=256, t_end=0.25):
     return ms / 1e3, nsteps
 
 
+def time_rd(n, steps, eta=1e-2):
+    """Residual distribution (benchmark_rd.f90, solver 'RD1'): n x n nodes on
+    the unit square, transmissive copies, CFL 0.5, the rest state rho = 1.21
+    exp(-1.21 (x+y)) with a pressure bump eta exp(-121 ((x-0.3)^2 + (y-0.3)^2))
+    (initial condition 8, built here in numpy).  Each step reads the speed
+    maximum back to the host for dt, as the reference's loop does, so the
+    time is taken on the host clock around the synchronous advance."""
+    gamma = float(np.float32(1.4))
+    x = np.linspace(0.0, 1.0, n)[:, None]
+    y = np.linspace(0.0, 1.0, n)[None, :]
+    r = float(np.float32(1.21))
+    u = np.zeros((4, n, n), order="F")
+    u[0] = r * np.exp(-r * (x + y))
+    u[3] = (np.exp(-r * (x + y)) + eta * np.exp(-121.0 * ((x - 0.3) ** 2 + (y - 0.3) ** 2))) / (gamma - 1.0)
+    with fvwb.SolverRD(fvwb.make_rd_params(n, n, gamma=gamma)) as s:
+        s.upload(u)
+        s.advance(2)                         # code objects loaded outside the timed region
+        s.upload(u)
+        t0 = time.perf_counter()
+        s.advance(steps)
+        sec = time.perf_counter() - t0
+        assert s.time()[2] == steps
+    return sec
+
+
 def cpu_baseline():
     """FP64 CPU oracle (plain C port, one core) on a bounded sample: one
     2048x2048 problem of the same scheme, 2 SSP-RK3 steps."""
@@ -221,6 +246,8 @@ def main():
     ap.add_argument("--dg-size", type=int, default=1024, help="cells per side of the DG run (0: skip)")
     ap.add_argument("--fv1d-batch", type=int, default=4096, help="1D instances in the batched 1D run (0: skip)")
     ap.add_argument("--dg-only", action="store_true", help="time the DG workload only (no JSON line)")
+    ap.add_argument("--rd-size", type=int, default=2049, help="nodes per side of the residual-distribution run (0: skip)")
+    ap.add_argument("--rd-only", action="store_true", help="time the residual-distribution workload only (no JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -228,6 +255,11 @@ def main():
         sys.exit(2)
     ev = HipEvents()
     K, W = args.steps, args.warmup
+    if args.rd_only:       # profiling runs of the residual-distribution kernels alone
+        secr = time_rd(args.rd_size, K)
+        log(f"rd-only {args.rd_size}^2: {args.rd_size ** 2 * K / secr:.4e} node-updates/s, "
+            f"{secr * 1e3 / K:.3f} ms/step")
+        return
     if args.dg_only:       # profiling runs of the DG kernels alone
         sec_dg = time_dg(ev, args.dg_size, K, W)
         log(f"dg-only {args.dg_size}^2: {args.dg_size ** 2 * K / sec_dg:.4e} cell-updates/s, "
@@ -318,6 +350,17 @@ def main():
             f"SSP(5,4) {sec54 * 1e3 / kdg:.3f} ms/step")
 
     # ---- 1D workload (256 cells to t = 0.25, whole run in one launch) ---------------------
+    rd = None
+    if args.rd_size > 0:
+        nrd, krd = args.rd_size, 50
+        secr = time_rd(nrd, krd)
+        # per step: two updates reading u, diff (64 B) and writing dudt (32 B),
+        # the two combinations (128 B, 96 B) and the speed pass (32 B) per node
+        rd = {"workload": f"{nrd}x{nrd} nodes, residual distribution 'RD1', transmissive, CFL 0.5, FP64",
+              "steps": krd, "ms_per_step": secr * 1e3 / krd, "value": nrd * nrd * krd / secr,
+              "unit": "node-updates/s", "bytes_per_node_update": 448,
+              "hbm_frac": nrd * nrd * krd * 448 / secr / 8e12}
+        log(f"rd {nrd}^2: {rd['value']:.4e} node-updates/s, {rd['ms_per_step']:.3f} ms/step")
     fv1d = None
     if args.fv1d_batch > 0:
         sec1, n1s = time_fv1d(ev, 1)
@@ -362,6 +405,7 @@ def main():
         "hydrostatic_drift": drift,
         "dg_p2": dg,
         "fv1d": fv1d,
+        "rd": rd,
     }
     print(json.dumps(out), flush=True)
 
