@@ -2,11 +2,11 @@
 // solver 'RD1') on MI355X.
 //
 // compute_update_rd (:612-745) scatters four corner residuals of every quad
-// cell into its nodes.  Here a cell pass writes each cell's sixteen corner
-// values once (SoA planes, one thread per cell) and a node pass gathers them,
-// one thread per node, adding the contributions in the order the reference's
-// cell loop (x outer, y inner) adds them, so the sums round identically.  The
-// two boundary passes apply the reference's copies (:722-737) in their order.
+// cell into its nodes.  Here each workgroup computes the corner values of the
+// cells around a tile of nodes into LDS and every node gathers its own, adding
+// them in the order the reference's cell loop (x outer, y inner) adds them, so
+// the sums round identically without atomics.  The two boundary passes apply
+// the reference's copies (:722-737) in their order.
 // FP64 with contraction off and the reference's expression order throughout:
 // the results equal the oracle's bit for bit.
 #include <hip/hip_runtime.h>
@@ -48,13 +48,11 @@ __device__ __forceinline__ void load4(const double* a, int nx, int i, int j, dou
   v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
 }
 
-// Corner residuals of cell (i, j): phi[(k*4 + v)*ncell + cell], k = 11, 12, 21, 22.
-__global__ void rd_cell_kernel(Geo g, const double* __restrict__ u, const double* __restrict__ diff, double dt,
-                               double* __restrict__ phi) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int cx = g.nx - 1;
-  if (i >= cx || j >= g.ny - 1) return;
+// Corner residuals of cell (i, j) (nodes i..i+1, j..j+1): out[k*4 + v] for
+// corners k = 11, 12, 21, 22 (benchmark_rd.f90:653-697).
+__device__ __forceinline__ void cell_phi(const Geo& g, const double* __restrict__ u,
+                                         const double* __restrict__ diff, double dt, int i, int j,
+                                         double out[16]) {
   double uu[4][4], ww[4][4], ff[4][4], gg[4][4], ss[4][4], sp[4];
   load4(u, g.nx, i, j, uu[0]);
   load4(u, g.nx, i + 1, j, uu[1]);
@@ -92,7 +90,6 @@ __global__ void rd_cell_kernel(Geo g, const double* __restrict__ u, const double
   load4(diff, g.nx, i, j + 1, dd[2]);
   load4(diff, g.nx, i + 1, j + 1, dd[3]);
   const double dx = g.dx, dy = g.dy;
-  const long long ncell = (long long)cx * (g.ny - 1), cell = (long long)j * cx + i;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     // :673-678, corners 11, 12, 21, 22 = uu[0..3]
@@ -101,31 +98,52 @@ __global__ void rd_cell_kernel(Geo g, const double* __restrict__ u, const double
                             0.25 * (ss[0][v] + ss[2][v] + ss[1][v] + ss[3][v]) * dx * dy;
     const double ubar = 0.25 * (uu[0][v] + uu[3][v] + uu[2][v] + uu[1][v]);
     // :688-697
-    phi[(0 * 4 + v) * ncell + cell] = dy * dx * dd[0][v] * 0.25 + dt * (totalphi * 0.25 + alpha * (uu[0][v] - ubar) * dy);
-    phi[(1 * 4 + v) * ncell + cell] = dy * dx * dd[1][v] * 0.25 + dt * (totalphi * 0.25 + alpha * (uu[1][v] - ubar) * dy);
-    phi[(2 * 4 + v) * ncell + cell] = dy * dx * dd[2][v] * 0.25 + dt * (totalphi * 0.25 + alpha * (uu[2][v] - ubar) * dx);
-    phi[(3 * 4 + v) * ncell + cell] = dy * dx * dd[3][v] * 0.25 + dt * (totalphi * 0.25 + alpha * (uu[3][v] - ubar) * dx);
+    out[0 * 4 + v] = dy * dx * dd[0][v] * 0.25 + dt * (totalphi * 0.25 + alpha * (uu[0][v] - ubar) * dy);
+    out[1 * 4 + v] = dy * dx * dd[1][v] * 0.25 + dt * (totalphi * 0.25 + alpha * (uu[1][v] - ubar) * dy);
+    out[2 * 4 + v] = dy * dx * dd[2][v] * 0.25 + dt * (totalphi * 0.25 + alpha * (uu[2][v] - ubar) * dx);
+    out[3 * 4 + v] = dy * dx * dd[3][v] * 0.25 + dt * (totalphi * 0.25 + alpha * (uu[3][v] - ubar) * dx);
   }
 }
 
-// dudt at node (a, b): the corner values of cells (a-1,b-1), (a-1,b), (a,b-1),
-// (a,b) in that order (the order the x-outer, y-inner cell loop adds them).
-__global__ void rd_node_kernel(Geo g, const double* __restrict__ phi, double* __restrict__ dudt) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y * blockDim.y + threadIdx.y;
+// One workgroup per tile of TX x TY nodes: the corner residuals of the
+// (TX+1) x (TY+1) cells that touch the tile (its cells and a one-cell halo,
+// which the neighbouring tiles compute as well) go to LDS, then every node
+// gathers its up to four corner values from there in the order the
+// reference's x-outer, y-inner cell loop adds them (cells (a-1,b-1), (a-1,b),
+// (a,b-1), (a,b)), so the sums round as the reference's do and no corner value
+// travels through HBM.
+constexpr int TX = 32, TY = 8, CX = TX + 1, CY = TY + 1, NC = CX * CY;
+
+__global__ __launch_bounds__(256) void rd_update_kernel(Geo g, const double* __restrict__ u,
+                                                        const double* __restrict__ diff, double dt,
+                                                        double* __restrict__ dudt) {
+  __shared__ double lds[16 * NC];           // [corner*4 + v][cell], 38 KB
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+  for (int c = threadIdx.x; c < NC; c += 256) {
+    const int ci = x0 - 1 + c % CX, cj = y0 - 1 + c / CX;
+    if (ci >= 0 && ci < g.nx - 1 && cj >= 0 && cj < g.ny - 1) {
+      double phi[16];
+      cell_phi(g, u, diff, dt, ci, cj, phi);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) lds[q * NC + c] = phi[q];
+    }
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int a = x0 + tx, b = y0 + ty;
   if (a >= g.nx || b >= g.ny) return;
   const int cx = g.nx - 1, cy = g.ny - 1;
-  const long long ncell = (long long)cx * cy;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-  auto add = [&](int ci, int cj, int corner) {
-    const long long cell = (long long)cj * cx + ci;
+  // local cell index of cell (a-1+di, b-1+dj) is (ty+dj)*CX + tx+di
+  auto add = [&](int di, int dj, int corner) {
+    const int c = (ty + dj) * CX + tx + di;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) s[v] = s[v] + phi[(corner * 4 + v) * ncell + cell];
+    for (int v = 0; v < 4; ++v) s[v] = s[v] + lds[(corner * 4 + v) * NC + c];
   };
-  if (a >= 1 && b >= 1) add(a - 1, b - 1, 3);
-  if (a >= 1 && b < cy) add(a - 1, b, 1);
-  if (a < cx && b >= 1) add(a, b - 1, 2);
-  if (a < cx && b < cy) add(a, b, 0);
+  if (a >= 1 && b >= 1) add(0, 0, 3);
+  if (a >= 1 && b < cy) add(0, 1, 1);
+  if (a < cx && b >= 1) add(1, 0, 2);
+  if (a < cx && b < cy) add(1, 1, 0);
   *reinterpret_cast<double4*>(dudt + 4 * ((long long)b * g.nx + a)) = double4{s[0], s[1], s[2], s[3]};
 }
 
@@ -205,7 +223,6 @@ struct fvwb_rd {
   double* d_up = nullptr;
   double* d_diff = nullptr;
   double* d_dudt = nullptr;
-  double* d_phi = nullptr;    // 16 planes of (nx-1)(ny-1) corner values
   unsigned long long* d_smax = nullptr;
   bool loaded = false;
   double t = 0.0, dt_last = 0.0;
@@ -231,7 +248,7 @@ extern "C" const char* fvwb_rd_last_error(fvwb_rd_handle h) { return h ? h->err.
 extern "C" int fvwb_rd_destroy(fvwb_rd_handle h) {
   if (!h) return FVWB_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* ps[] = {h->d_u, h->d_up, h->d_diff, h->d_dudt, h->d_phi, h->d_smax};
+  void* ps[] = {h->d_u, h->d_up, h->d_diff, h->d_dudt, h->d_smax};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -244,7 +261,7 @@ extern "C" int fvwb_rd_create(fvwb_rd_handle* out, const fvwb_rd_params* p) {
   *out = nullptr;
   if (p->nx < 2 || p->ny < 2) return failrd(nullptr, FVWB_ERR_ARG, "nx and ny must be >= 2");
   if ((long long)p->nx * p->ny > (1ll << 28)) return failrd(nullptr, FVWB_ERR_ARG, "more than 2^28 nodes");
-  if (p->ny > 4 * 65535) return failrd(nullptr, FVWB_ERR_ARG, "ny must be <= 262140 (one grid row per 4 node rows)");
+  if (p->ny > TY * 65535) return failrd(nullptr, FVWB_ERR_ARG, "ny must be <= 524280 (one grid row per 8 node rows)");
   if (p->bc != FVWB_RD_BC_PERIODIC && p->bc != FVWB_RD_BC_TRANSMISSIVE)
     return failrd(nullptr, FVWB_ERR_ARG, "bc must be 2 (periodic copies) or 3 (transmissive copies)");
   if (!(p->gamma > 1.0) || !(p->boxlen_x > 0.0) || !(p->boxlen_y > 0.0) || !(p->cfl > 0.0))
@@ -254,10 +271,9 @@ extern "C" int fvwb_rd_create(fvwb_rd_handle* out, const fvwb_rd_params* p) {
   h->p = *p;
   h->g = Geo{p->nx, p->ny, p->bc, p->gamma, p->boxlen_x / (double)(p->nx - 1), p->boxlen_y / (double)(p->ny - 1)};
   const size_t nb = sizeof(double) * 4 * (size_t)p->nx * (size_t)p->ny;
-  const size_t pb = sizeof(double) * 16 * (size_t)(p->nx - 1) * (size_t)(p->ny - 1);
   bool ok = hipStreamCreate(&h->stream) == hipSuccess && hipMalloc(&h->d_u, nb) == hipSuccess &&
             hipMalloc(&h->d_up, nb) == hipSuccess && hipMalloc(&h->d_diff, nb) == hipSuccess &&
-            hipMalloc(&h->d_dudt, nb) == hipSuccess && hipMalloc(&h->d_phi, pb) == hipSuccess &&
+            hipMalloc(&h->d_dudt, nb) == hipSuccess &&
             hipMalloc(&h->d_smax, sizeof(unsigned long long)) == hipSuccess;
   if (!ok) {
     fvwb_rd_destroy(h);
@@ -269,11 +285,8 @@ extern "C" int fvwb_rd_create(fvwb_rd_handle* out, const fvwb_rd_params* p) {
 
 // dudt of compute_update_rd for the device state u and correction diff.
 static int update(fvwb_rd* h, const double* u, const double* diff, double dt, double* dudt) {
-  const dim3 blk(64, 4);
-  const dim3 gc((unsigned)((h->p.nx - 1 + 63) / 64), (unsigned)((h->p.ny - 1 + 3) / 4));
-  const dim3 gn((unsigned)((h->p.nx + 63) / 64), (unsigned)((h->p.ny + 3) / 4));
-  hipLaunchKernelGGL(rd_cell_kernel, gc, blk, 0, h->stream, h->g, u, diff, dt, h->d_phi);
-  hipLaunchKernelGGL(rd_node_kernel, gn, blk, 0, h->stream, h->g, h->d_phi, dudt);
+  const dim3 grid((unsigned)((h->p.nx + TX - 1) / TX), (unsigned)((h->p.ny + TY - 1) / TY));
+  hipLaunchKernelGGL(rd_update_kernel, grid, dim3(256), 0, h->stream, h->g, u, diff, dt, dudt);
   hipLaunchKernelGGL(rd_bc_x_kernel, dim3((unsigned)((h->p.ny + 255) / 256)), dim3(256), 0, h->stream, h->g, dudt);
   hipLaunchKernelGGL(rd_bc_y_kernel, dim3((unsigned)((h->p.nx + 255) / 256)), dim3(256), 0, h->stream, h->g, dudt);
   HIPR(h, hipGetLastError());
