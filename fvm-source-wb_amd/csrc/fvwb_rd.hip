@@ -112,26 +112,27 @@ __device__ __forceinline__ void cell_phi(const Geo& g, const double* __restrict_
 // reference's x-outer, y-inner cell loop adds them (cells (a-1,b-1), (a-1,b),
 // (a,b-1), (a,b)), so the sums round as the reference's do and no corner value
 // travels through HBM.
-constexpr int TX = 32, TY = 8, CX = TX + 1, CY = TY + 1, NC = CX * CY;
+// 31 x 7 nodes make the 32 x 8 = 256 cells one per thread (one fill round).
+constexpr int TX = 31, TY = 7, CX = TX + 1, CY = TY + 1, NC = CX * CY;
+static_assert(NC == 256, "one cell per thread of the workgroup");
 
 __global__ __launch_bounds__(256) void rd_update_kernel(Geo g, const double* __restrict__ u,
                                                         const double* __restrict__ diff, double dt,
                                                         double* __restrict__ dudt) {
-  __shared__ double lds[16 * NC];           // [corner*4 + v][cell], 38 KB
+  __shared__ double lds[16 * NC];           // [corner*4 + v][cell], 32 KB
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-  for (int c = threadIdx.x; c < NC; c += 256) {
-    const int ci = x0 - 1 + c % CX, cj = y0 - 1 + c / CX;
-    if (ci >= 0 && ci < g.nx - 1 && cj >= 0 && cj < g.ny - 1) {
-      double phi[16];
-      cell_phi(g, u, diff, dt, ci, cj, phi);
+  const int c = threadIdx.x;                // cell (x0-1 + c%CX, y0-1 + c/CX)
+  const int ci = x0 - 1 + c % CX, cj = y0 - 1 + c / CX;
+  if (ci >= 0 && ci < g.nx - 1 && cj >= 0 && cj < g.ny - 1) {
+    double phi[16];
+    cell_phi(g, u, diff, dt, ci, cj, phi);
 #pragma unroll
-      for (int q = 0; q < 16; ++q) lds[q * NC + c] = phi[q];
-    }
+    for (int q = 0; q < 16; ++q) lds[q * NC + c] = phi[q];
   }
   __syncthreads();
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int tx = threadIdx.x % CX, ty = threadIdx.x / CX;   // node (x0+tx, y0+ty)
   const int a = x0 + tx, b = y0 + ty;
-  if (a >= g.nx || b >= g.ny) return;
+  if (tx >= TX || ty >= TY || a >= g.nx || b >= g.ny) return;
   const int cx = g.nx - 1, cy = g.ny - 1;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
   // local cell index of cell (a-1+di, b-1+dj) is (ty+dj)*CX + tx+di
@@ -179,8 +180,15 @@ __global__ void rd_bc_y_kernel(Geo g, double* __restrict__ d) {
 
 // compute_max_speed (:448-475): the maximum of the nodes' speeds (a NaN speed
 // never passes the reference's ">=" test, and fmax ignores it likewise).
-// Speeds are >= 0, so their bit patterns order as unsigned integers.
-__global__ void rd_speed_kernel(Geo g, const double* __restrict__ u, unsigned long long* __restrict__ smax) {
+// Speeds are >= 0, so their bit patterns order as unsigned integers.  Each
+// workgroup reduces its nodes and adds one atomicMax to one of SLOTS slots, a
+// cache line apart, so the atomics do not queue on one line; the host takes
+// the maximum of the slots.
+constexpr int SLOTS = 64, SLOT_STRIDE = 16;     // 16 x 8 B = one 128-byte line
+
+__global__ __launch_bounds__(256) void rd_speed_kernel(Geo g, const double* __restrict__ u,
+                                                       unsigned long long* __restrict__ smax) {
+  __shared__ double part[4];
   const long long n = (long long)g.nx * g.ny;
   double m = 0.0;
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
@@ -190,7 +198,12 @@ __global__ void rd_speed_kernel(Geo g, const double* __restrict__ u, unsigned lo
     m = fmax(m, speed(g.gamma, v));
   }
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(smax, (unsigned long long)__double_as_longlong(m));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+    atomicMax(smax + (blockIdx.x % SLOTS) * SLOT_STRIDE, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 // up = u - dudt/(dx dy), diff = up - u (evolve, :351-354)
@@ -261,7 +274,7 @@ extern "C" int fvwb_rd_create(fvwb_rd_handle* out, const fvwb_rd_params* p) {
   *out = nullptr;
   if (p->nx < 2 || p->ny < 2) return failrd(nullptr, FVWB_ERR_ARG, "nx and ny must be >= 2");
   if ((long long)p->nx * p->ny > (1ll << 28)) return failrd(nullptr, FVWB_ERR_ARG, "more than 2^28 nodes");
-  if (p->ny > TY * 65535) return failrd(nullptr, FVWB_ERR_ARG, "ny must be <= 524280 (one grid row per 8 node rows)");
+  if (p->ny > TY * 65535) return failrd(nullptr, FVWB_ERR_ARG, "ny must be <= 458745 (one grid row per 7 node rows)");
   if (p->bc != FVWB_RD_BC_PERIODIC && p->bc != FVWB_RD_BC_TRANSMISSIVE)
     return failrd(nullptr, FVWB_ERR_ARG, "bc must be 2 (periodic copies) or 3 (transmissive copies)");
   if (!(p->gamma > 1.0) || !(p->boxlen_x > 0.0) || !(p->boxlen_y > 0.0) || !(p->cfl > 0.0))
@@ -274,7 +287,7 @@ extern "C" int fvwb_rd_create(fvwb_rd_handle* out, const fvwb_rd_params* p) {
   bool ok = hipStreamCreate(&h->stream) == hipSuccess && hipMalloc(&h->d_u, nb) == hipSuccess &&
             hipMalloc(&h->d_up, nb) == hipSuccess && hipMalloc(&h->d_diff, nb) == hipSuccess &&
             hipMalloc(&h->d_dudt, nb) == hipSuccess &&
-            hipMalloc(&h->d_smax, sizeof(unsigned long long)) == hipSuccess;
+            hipMalloc(&h->d_smax, sizeof(unsigned long long) * SLOTS * SLOT_STRIDE) == hipSuccess;
   if (!ok) {
     fvwb_rd_destroy(h);
     return failrd(nullptr, FVWB_ERR_OOM, "device allocation failed");
@@ -294,15 +307,18 @@ static int update(fvwb_rd* h, const double* u, const double* diff, double dt, do
 }
 
 static int max_speed(fvwb_rd* h, const double* u, double* cmax) {
-  HIPR(h, hipMemsetAsync(h->d_smax, 0, sizeof(unsigned long long), h->stream));
+  const size_t sb = sizeof(unsigned long long) * SLOTS * SLOT_STRIDE;
+  HIPR(h, hipMemsetAsync(h->d_smax, 0, sb, h->stream));
   const long long n = (long long)h->p.nx * h->p.ny;
-  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 2048);
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 16384);
   hipLaunchKernelGGL(rd_speed_kernel, dim3(blocks), dim3(256), 0, h->stream, h->g, u, h->d_smax);
   HIPR(h, hipGetLastError());
-  unsigned long long bits = 0;
-  HIPR(h, hipMemcpyAsync(&bits, h->d_smax, sizeof bits, hipMemcpyDeviceToHost, h->stream));
+  unsigned long long bits[SLOTS * SLOT_STRIDE];
+  HIPR(h, hipMemcpyAsync(bits, h->d_smax, sb, hipMemcpyDeviceToHost, h->stream));
   HIPR(h, hipStreamSynchronize(h->stream));
-  std::memcpy(cmax, &bits, sizeof bits);
+  unsigned long long m = 0;
+  for (int k = 0; k < SLOTS; ++k) m = std::max(m, bits[k * SLOT_STRIDE]);
+  std::memcpy(cmax, &m, sizeof m);
   return FVWB_OK;
 }
 
