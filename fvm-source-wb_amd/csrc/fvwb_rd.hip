@@ -105,6 +105,15 @@ __device__ __forceinline__ void cell_phi(const Geo& g, const double* __restrict_
   }
 }
 
+// The resident time loop's clock, on the device so that steps follow each
+// other without a host round trip: dt of the current step, whether it runs
+// (t < t_end and no failure), and the running totals.
+struct Clock {
+  double t, dt, dt_last;
+  long long steps;
+  int active, err;
+};
+
 // One workgroup per tile of TX x TY nodes: the corner residuals of the
 // (TX+1) x (TY+1) cells that touch the tile (its cells and a one-cell halo,
 // which the neighbouring tiles compute as well) go to LDS, then every node
@@ -116,9 +125,14 @@ __device__ __forceinline__ void cell_phi(const Geo& g, const double* __restrict_
 constexpr int TX = 31, TY = 7, CX = TX + 1, CY = TY + 1, NC = CX * CY;
 static_assert(NC == 256, "one cell per thread of the workgroup");
 
+// dt is the argument, or the clock's when ck is given (the resident loop).
 __global__ __launch_bounds__(256) void rd_update_kernel(Geo g, const double* __restrict__ u,
                                                         const double* __restrict__ diff, double dt,
-                                                        double* __restrict__ dudt) {
+                                                        const Clock* __restrict__ ck, double* __restrict__ dudt) {
+  if (ck) {
+    if (!ck->active) return;
+    dt = ck->dt;
+  }
   __shared__ double lds[16 * NC];           // [corner*4 + v][cell], 32 KB
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
   const int c = threadIdx.x;                // cell (x0-1 + c%CX, y0-1 + c/CX)
@@ -206,21 +220,53 @@ __global__ __launch_bounds__(256) void rd_speed_kernel(Geo g, const double* __re
   }
 }
 
+// dt = min(cfl dx dy/cmax/3, t_end - t) (evolve, :348-349) from the speed
+// slots, which it clears for the next step; a maximum that is not finite and
+// positive stops the run with the error flag set.
+__global__ void rd_dt_kernel(Geo g, double cfl, double t_end, unsigned long long* __restrict__ smax,
+                             Clock* __restrict__ ck) {
+  if (threadIdx.x != 0) return;
+  unsigned long long m = 0;
+  for (int k = 0; k < SLOTS; ++k) {
+    m = max(m, smax[k * SLOT_STRIDE]);
+    smax[k * SLOT_STRIDE] = 0;
+  }
+  const double cmax = __longlong_as_double((long long)m);
+  if (ck->err || !(ck->t < t_end)) {
+    ck->active = 0;
+    return;
+  }
+  if (!(cmax > 0.0) || !isfinite(cmax)) {
+    ck->err = 1;
+    ck->active = 0;
+    return;
+  }
+  ck->dt = fmin(cfl * g.dy * g.dx / cmax / (2.0 * 1.0 + 1.0), t_end - ck->t);
+  ck->active = 1;
+}
+// t = t + dt (:366), after the step
+__global__ void rd_tick_kernel(Clock* __restrict__ ck) {
+  if (threadIdx.x != 0 || !ck->active) return;
+  ck->t = ck->t + ck->dt;
+  ck->dt_last = ck->dt;
+  ck->steps += 1;
+}
+
 // up = u - dudt/(dx dy), diff = up - u (evolve, :351-354)
 __global__ void rd_pass1_kernel(long long n, double area, const double* __restrict__ u,
                                 const double* __restrict__ dudt, double* __restrict__ up,
-                                double* __restrict__ diff) {
+                                double* __restrict__ diff, const Clock* __restrict__ ck) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  if (k >= n || !ck->active) return;
   const double x = u[k] - dudt[k] / area;
   up[k] = x;
   diff[k] = x - u[k];
 }
 // u = up - dudt/(dx dy) (:356)
 __global__ void rd_pass2_kernel(long long n, double area, double* __restrict__ u, const double* __restrict__ up,
-                                const double* __restrict__ dudt) {
+                                const double* __restrict__ dudt, const Clock* __restrict__ ck) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  if (k >= n || !ck->active) return;
   u[k] = up[k] - dudt[k] / area;
 }
 
@@ -237,9 +283,8 @@ struct fvwb_rd {
   double* d_diff = nullptr;
   double* d_dudt = nullptr;
   unsigned long long* d_smax = nullptr;
+  Clock* d_clk = nullptr;
   bool loaded = false;
-  double t = 0.0, dt_last = 0.0;
-  int64_t steps = 0;
   std::string err;
 };
 
@@ -261,7 +306,7 @@ extern "C" const char* fvwb_rd_last_error(fvwb_rd_handle h) { return h ? h->err.
 extern "C" int fvwb_rd_destroy(fvwb_rd_handle h) {
   if (!h) return FVWB_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* ps[] = {h->d_u, h->d_up, h->d_diff, h->d_dudt, h->d_smax};
+  void* ps[] = {h->d_u, h->d_up, h->d_diff, h->d_dudt, h->d_smax, h->d_clk};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -287,7 +332,8 @@ extern "C" int fvwb_rd_create(fvwb_rd_handle* out, const fvwb_rd_params* p) {
   bool ok = hipStreamCreate(&h->stream) == hipSuccess && hipMalloc(&h->d_u, nb) == hipSuccess &&
             hipMalloc(&h->d_up, nb) == hipSuccess && hipMalloc(&h->d_diff, nb) == hipSuccess &&
             hipMalloc(&h->d_dudt, nb) == hipSuccess &&
-            hipMalloc(&h->d_smax, sizeof(unsigned long long) * SLOTS * SLOT_STRIDE) == hipSuccess;
+            hipMalloc(&h->d_smax, sizeof(unsigned long long) * SLOTS * SLOT_STRIDE) == hipSuccess &&
+            hipMalloc(&h->d_clk, sizeof(Clock)) == hipSuccess;
   if (!ok) {
     fvwb_rd_destroy(h);
     return failrd(nullptr, FVWB_ERR_OOM, "device allocation failed");
@@ -297,21 +343,25 @@ extern "C" int fvwb_rd_create(fvwb_rd_handle* out, const fvwb_rd_params* p) {
 }
 
 // dudt of compute_update_rd for the device state u and correction diff.
-static int update(fvwb_rd* h, const double* u, const double* diff, double dt, double* dudt) {
+static int update(fvwb_rd* h, const double* u, const double* diff, double dt, const Clock* ck, double* dudt) {
   const dim3 grid((unsigned)((h->p.nx + TX - 1) / TX), (unsigned)((h->p.ny + TY - 1) / TY));
-  hipLaunchKernelGGL(rd_update_kernel, grid, dim3(256), 0, h->stream, h->g, u, diff, dt, dudt);
+  hipLaunchKernelGGL(rd_update_kernel, grid, dim3(256), 0, h->stream, h->g, u, diff, dt, ck, dudt);
   hipLaunchKernelGGL(rd_bc_x_kernel, dim3((unsigned)((h->p.ny + 255) / 256)), dim3(256), 0, h->stream, h->g, dudt);
   hipLaunchKernelGGL(rd_bc_y_kernel, dim3((unsigned)((h->p.nx + 255) / 256)), dim3(256), 0, h->stream, h->g, dudt);
   HIPR(h, hipGetLastError());
   return FVWB_OK;
 }
 
-static int max_speed(fvwb_rd* h, const double* u, double* cmax) {
-  const size_t sb = sizeof(unsigned long long) * SLOTS * SLOT_STRIDE;
-  HIPR(h, hipMemsetAsync(h->d_smax, 0, sb, h->stream));
+static void launch_speed(fvwb_rd* h, const double* u) {
   const long long n = (long long)h->p.nx * h->p.ny;
   const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 16384);
   hipLaunchKernelGGL(rd_speed_kernel, dim3(blocks), dim3(256), 0, h->stream, h->g, u, h->d_smax);
+}
+
+static int max_speed(fvwb_rd* h, const double* u, double* cmax) {
+  const size_t sb = sizeof(unsigned long long) * SLOTS * SLOT_STRIDE;
+  HIPR(h, hipMemsetAsync(h->d_smax, 0, sb, h->stream));
+  launch_speed(h, u);
   HIPR(h, hipGetLastError());
   unsigned long long bits[SLOTS * SLOT_STRIDE];
   HIPR(h, hipMemcpyAsync(bits, h->d_smax, sb, hipMemcpyDeviceToHost, h->stream));
@@ -329,7 +379,7 @@ extern "C" int fvwb_rd_compute_update(fvwb_rd_handle h, const double* u, const d
   // the update works in the step's scratch arrays; an uploaded state stays as it is
   HIPR(h, hipMemcpyAsync(h->d_up, u, nb, hipMemcpyHostToDevice, h->stream));
   HIPR(h, hipMemcpyAsync(h->d_diff, diff, nb, hipMemcpyHostToDevice, h->stream));
-  int rc = update(h, h->d_up, h->d_diff, dt, h->d_dudt);
+  int rc = update(h, h->d_up, h->d_diff, dt, nullptr, h->d_dudt);
   if (rc) return rc;
   HIPR(h, hipMemcpyAsync(dudt, h->d_dudt, nb, hipMemcpyDeviceToHost, h->stream));
   HIPR(h, hipStreamSynchronize(h->stream));
@@ -346,41 +396,57 @@ extern "C" int fvwb_rd_compute_max_speed(fvwb_rd_handle h, const double* u, doub
 extern "C" int fvwb_rd_upload(fvwb_rd_handle h, const double* u) {
   if (!h || !u) return failrd(h, FVWB_ERR_ARG, "NULL argument");
   const size_t nb = sizeof(double) * 4 * (size_t)h->p.nx * (size_t)h->p.ny;
+  const Clock zero{};
   HIPR(h, hipMemcpyAsync(h->d_u, u, nb, hipMemcpyHostToDevice, h->stream));
+  HIPR(h, hipMemcpyAsync(h->d_clk, &zero, sizeof zero, hipMemcpyHostToDevice, h->stream));
   HIPR(h, hipStreamSynchronize(h->stream));
   h->loaded = true;
-  h->t = 0.0;
-  h->dt_last = 0.0;
-  h->steps = 0;
   return FVWB_OK;
 }
 
+static int read_clock(fvwb_rd* h, Clock* c) {
+  HIPR(h, hipMemcpyAsync(c, h->d_clk, sizeof *c, hipMemcpyDeviceToHost, h->stream));
+  HIPR(h, hipStreamSynchronize(h->stream));
+  return FVWB_OK;
+}
+
+// Steps are queued in groups of this many between reads of the clock.
+static constexpr int32_t kGroup = 64;
+
 // evolve (:356-385) with solver 'RD1': while t < t_end, at most nsteps steps.
+// dt, t and the step count live on the device; the host reads them once per
+// group of steps to stop early, and each kernel of a step past t_end (or
+// after a failure) does nothing.
 extern "C" int fvwb_rd_advance(fvwb_rd_handle h, int32_t nsteps, double t_end) {
   if (!h) return failrd(h, FVWB_ERR_ARG, "NULL handle");
   if (!h->loaded) return failrd(h, FVWB_ERR_STATE, "advance before upload");
   if (nsteps < 0) return failrd(h, FVWB_ERR_ARG, "nsteps must be >= 0");
   const long long n = 4ll * h->p.nx * h->p.ny;
-  const double dx = h->g.dx, dy = h->g.dy;
+  const double area = h->g.dx * h->g.dy;
   const dim3 eb(256), eg((unsigned)((n + 255) / 256));
-  for (int32_t s = 0; s < nsteps && h->t < t_end; ++s) {
-    double cmax = 0.0;
-    int rc = max_speed(h, h->d_u, &cmax);
-    if (rc) return rc;
-    if (!(cmax > 0.0) || !std::isfinite(cmax))
-      return failrd(h, FVWB_ERR_NONPHYSICAL, "wave-speed maximum not finite and positive");
-    const double dt = std::fmin(h->p.cfl * dy * dx / cmax / (2.0 * 1.0 + 1.0), t_end - h->t);
-    HIPR(h, hipMemsetAsync(h->d_diff, 0, sizeof(double) * (size_t)n, h->stream));
-    if ((rc = update(h, h->d_u, h->d_diff, dt, h->d_dudt))) return rc;
-    hipLaunchKernelGGL(rd_pass1_kernel, eg, eb, 0, h->stream, n, dx * dy, h->d_u, h->d_dudt, h->d_up, h->d_diff);
-    if ((rc = update(h, h->d_u, h->d_diff, dt, h->d_dudt))) return rc;
-    hipLaunchKernelGGL(rd_pass2_kernel, eg, eb, 0, h->stream, n, dx * dy, h->d_u, h->d_up, h->d_dudt);
-    HIPR(h, hipGetLastError());
-    h->t = h->t + dt;
-    h->dt_last = dt;
-    h->steps += 1;
+  HIPR(h, hipMemsetAsync(h->d_smax, 0, sizeof(unsigned long long) * SLOTS * SLOT_STRIDE, h->stream));
+  Clock c{};
+  int rc = read_clock(h, &c);
+  if (rc) return rc;
+  int32_t done = 0;
+  while (done < nsteps && !c.err && c.t < t_end) {
+    const int32_t k = std::min(kGroup, nsteps - done);
+    for (int32_t s = 0; s < k; ++s) {
+      launch_speed(h, h->d_u);
+      hipLaunchKernelGGL(rd_dt_kernel, dim3(1), dim3(64), 0, h->stream, h->g, h->p.cfl, t_end, h->d_smax, h->d_clk);
+      HIPR(h, hipMemsetAsync(h->d_diff, 0, sizeof(double) * (size_t)n, h->stream));
+      if ((rc = update(h, h->d_u, h->d_diff, 0.0, h->d_clk, h->d_dudt))) return rc;
+      hipLaunchKernelGGL(rd_pass1_kernel, eg, eb, 0, h->stream, n, area, h->d_u, h->d_dudt, h->d_up, h->d_diff,
+                         h->d_clk);
+      if ((rc = update(h, h->d_u, h->d_diff, 0.0, h->d_clk, h->d_dudt))) return rc;
+      hipLaunchKernelGGL(rd_pass2_kernel, eg, eb, 0, h->stream, n, area, h->d_u, h->d_up, h->d_dudt, h->d_clk);
+      hipLaunchKernelGGL(rd_tick_kernel, dim3(1), dim3(64), 0, h->stream, h->d_clk);
+      HIPR(h, hipGetLastError());
+    }
+    done += k;
+    if ((rc = read_clock(h, &c))) return rc;
   }
-  HIPR(h, hipStreamSynchronize(h->stream));
+  if (c.err) return failrd(h, FVWB_ERR_NONPHYSICAL, "wave-speed maximum not finite and positive");
   return FVWB_OK;
 }
 
@@ -395,8 +461,13 @@ extern "C" int fvwb_rd_download(fvwb_rd_handle h, double* u) {
 
 extern "C" int fvwb_rd_get_time(fvwb_rd_handle h, double* t, double* dt_last, int64_t* steps) {
   if (!h) return failrd(h, FVWB_ERR_ARG, "NULL handle");
-  if (t) *t = h->t;
-  if (dt_last) *dt_last = h->dt_last;
-  if (steps) *steps = h->steps;
+  Clock c{};
+  if (h->loaded) {
+    int rc = read_clock(h, &c);
+    if (rc) return rc;
+  }
+  if (t) *t = c.t;
+  if (dt_last) *dt_last = c.dt_last;
+  if (steps) *steps = c.steps;
   return FVWB_OK;
 }
