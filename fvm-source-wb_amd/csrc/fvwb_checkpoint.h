@@ -27,7 +27,7 @@
 
 namespace fvwb_ckpt {
 
-enum Kind : uint32_t { FV2D = 1, FV1D = 2, DG2D = 3, DG1D = 4 };
+enum Kind : uint32_t { FV2D = 1, FV1D = 2, DG2D = 3, DG1D = 4, RD = 5 };
 
 struct Segment {
   void* dev;     // device memory

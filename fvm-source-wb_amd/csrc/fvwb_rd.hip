@@ -18,6 +18,7 @@
 #include <string>
 
 #include "../../include/fvwb.h"
+#include "fvwb_checkpoint.h"
 
 #pragma clang fp contract(off)
 
@@ -457,6 +458,30 @@ extern "C" int fvwb_rd_download(fvwb_rd_handle h, double* u) {
   HIPR(h, hipMemcpyAsync(u, h->d_u, nb, hipMemcpyDeviceToHost, h->stream));
   HIPR(h, hipStreamSynchronize(h->stream));
   return FVWB_OK;
+}
+
+// ---- checkpoint / resume: the state and the clock -----------------------------------
+extern "C" int fvwb_rd_checkpoint_save(fvwb_rd_handle h, const char* path) {
+  if (!h || !path) return failrd(h, FVWB_ERR_ARG, "NULL argument");
+  if (!h->loaded) return failrd(h, FVWB_ERR_STATE, "checkpoint before upload");
+  const fvwb_ckpt::Segment seg[2] = {{h->d_u, sizeof(double) * 4 * (size_t)h->p.nx * (size_t)h->p.ny},
+                                     {h->d_clk, sizeof(Clock)}};
+  std::string err;
+  const int rc = fvwb_ckpt::save(path, fvwb_ckpt::RD, &h->p, sizeof h->p, 0, seg, 2, h->stream, err);
+  if (rc) h->err = err;
+  return rc;
+}
+
+extern "C" int fvwb_rd_checkpoint_load(fvwb_rd_handle h, const char* path) {
+  if (!h || !path) return failrd(h, FVWB_ERR_ARG, "NULL argument");
+  const fvwb_ckpt::Segment seg[2] = {{h->d_u, sizeof(double) * 4 * (size_t)h->p.nx * (size_t)h->p.ny},
+                                     {h->d_clk, sizeof(Clock)}};
+  std::string err;
+  int32_t parity = 0;
+  const int rc = fvwb_ckpt::load(path, fvwb_ckpt::RD, &h->p, sizeof h->p, &parity, seg, 2, h->stream, err);
+  h->loaded = rc == FVWB_OK;
+  if (rc) h->err = err;
+  return rc;
 }
 
 extern "C" int fvwb_rd_get_time(fvwb_rd_handle h, double* t, double* dt_last, int64_t* steps) {

@@ -163,6 +163,7 @@ def lib():
         "fvwb1d_checkpoint_save": [H, cs], "fvwb1d_checkpoint_load": [H, cs],
         "fvwb_dg_checkpoint_save": [H, cs], "fvwb_dg_checkpoint_load": [H, cs],
         "fvwb_dg1d_checkpoint_save": [H, cs], "fvwb_dg1d_checkpoint_load": [H, cs],
+        "fvwb_rd_checkpoint_save": [H, cs], "fvwb_rd_checkpoint_load": [H, cs],
         "fvwb_params_reference_2d": [pp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
         "fvwb_create": [ctypes.POINTER(H), pp],
         "fvwb_destroy": [H],
@@ -909,6 +910,14 @@ class SolverRD:
         out = np.zeros(self.shape, order="F")
         _checkrd(lib().fvwb_rd_download(self._h, _dp(out)), self._h)
         return out
+
+    def checkpoint_save(self, path):
+        """Write the resident state and clock to a binary checkpoint."""
+        _checkrd(lib().fvwb_rd_checkpoint_save(self._h, os.fsencode(path)), self._h)
+
+    def checkpoint_load(self, path):
+        """Restore a state written by checkpoint_save of a handle with the same parameters."""
+        _checkrd(lib().fvwb_rd_checkpoint_load(self._h, os.fsencode(path)), self._h)
 
     def time(self):
         t, dt = ctypes.c_double(), ctypes.c_double()

@@ -503,6 +503,10 @@ int fvwb_rd_upload(fvwb_rd_handle h, const double* u);
 int fvwb_rd_advance(fvwb_rd_handle h, int32_t nsteps, double t_end);
 int fvwb_rd_download(fvwb_rd_handle h, double* u);
 int fvwb_rd_get_time(fvwb_rd_handle h, double* t, double* dt_last, int64_t* steps_done);
+/* The checkpoint format above for the resident state and clock of this
+ * handle; no counterpart in the reference, whose output is ASCII snapshots. */
+int fvwb_rd_checkpoint_save(fvwb_rd_handle h, const char* path);
+int fvwb_rd_checkpoint_load(fvwb_rd_handle h, const char* path);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,15 @@ def test_dg1d_resume(tmp_path):
                       50, 60)
 
 
+def test_rd_resume(tmp_path):
+    say("case: residual-distribution checkpoint, 65x65 nodes, ninit 8")
+    op = po.make_rd_params(65, 65, bc=po.RD_BC_TRANSMISSIVE)
+    u0 = po.rd_initial(op, 8, 1e-2)
+    resume_equals_run(lambda: fvwb.SolverRD(fvwb.make_rd_params(65, 65)), lambda s: s.upload(u0),
+                      lambda s, n: s.advance(n), lambda s: (s.download(), s.time()), str(tmp_path / "rd.ckpt"),
+                      40, 30)
+
+
 def test_load_rejects_foreign_and_damaged_files(tmp_path):
     say("case: checkpoint load errors")
     p = fvwb.make_params(32, 32, bc=fvwb.BC_REFLECT, precision=fvwb.FP32, rho0=1.0)
