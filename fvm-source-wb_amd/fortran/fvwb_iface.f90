@@ -65,7 +65,7 @@ module fvwb_iface
 
   public :: fvwb_rd_create, fvwb_rd_destroy, fvwb_rd_last_error, fvwb_rd_compute_update
   public :: fvwb_rd_compute_max_speed, fvwb_rd_upload, fvwb_rd_advance, fvwb_rd_download
-  public :: fvwb_rd_get_time, fvwb_rd_check
+  public :: fvwb_rd_get_time, fvwb_rd_check, fvwb_rd_checkpoint_save, fvwb_rd_checkpoint_load
 
   ! struct fvwb1d_params: six int32 then four doubles (24 + 32 bytes)
   type, bind(C), public :: fvwb1d_params
@@ -495,6 +495,18 @@ module fvwb_iface
        type(c_ptr), value :: h
        real(c_double), intent(out) :: u(*)
      end function fvwb_rd_download
+
+     ! binary checkpoint of the resident state and clock (path NUL-terminated)
+     integer(c_int) function fvwb_rd_checkpoint_save(h, path) bind(C, name='fvwb_rd_checkpoint_save')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function fvwb_rd_checkpoint_save
+     integer(c_int) function fvwb_rd_checkpoint_load(h, path) bind(C, name='fvwb_rd_checkpoint_load')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function fvwb_rd_checkpoint_load
 
      integer(c_int) function fvwb_rd_get_time(h, t, dt_last, steps) bind(C, name='fvwb_rd_get_time')
        import :: c_int, c_int64_t, c_ptr, c_double
