@@ -479,6 +479,7 @@ extern "C" int fvwb_checkpoint_load(fvwb_handle h, const char* path) {
   int32_t parity = 0;
   const int rc = fvwb_ckpt::load(path, fvwb_ckpt::FV2D, &h->p, sizeof h->p, &parity, seg, 3, h->stream, err);
   if (rc) {
+    h->uploaded = false;
     h->err = err;
     return rc;
   }

@@ -11,8 +11,10 @@
 //   int32    parity      the handle's step parity
 //   pbytes   the handle's parameter struct, as given to its create call
 //   nseg times: uint64 bytes, then the segment's bytes
-// A load checks magic, version, kind, the parameter block (byte for byte) and
-// every segment size against the handle it is loaded into.
+// A load checks magic, version, kind, the parameter block (byte for byte),
+// every segment size against the handle it is loaded into and the file's
+// length, all before its first copy to the device: a foreign, truncated or
+// padded file is rejected with the device state untouched.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -92,6 +94,21 @@ inline int load(const char* path, Kind kind, const void* params, uint64_t pbytes
   std::vector<char> pblock(pb);
   if (std::fread(pblock.data(), 1, pb, f) != pb) return bad("truncated");
   if (std::memcmp(pblock.data(), params, pb) != 0) return bad("written by a handle with other parameters");
+  // walk the segment headers and the file's end before touching the device
+  const long data_start = std::ftell(f);
+  if (data_start < 0) return bad("cannot position in the file");
+  for (uint32_t i = 0; i < nseg; ++i) {
+    uint64_t n = 0;
+    if (std::fread(&n, 8, 1, f) != 1) return bad("truncated");
+    if (n != seg[i].bytes) return bad("segment size differs");
+    if (std::fseek(f, (long)n, SEEK_CUR) != 0) return bad("truncated");
+  }
+  const long data_end = std::ftell(f);
+  if (data_end < 0 || std::fseek(f, 0, SEEK_END) != 0) return bad("cannot position in the file");
+  const long file_end = std::ftell(f);
+  if (file_end < data_end) return bad("truncated");
+  if (file_end > data_end) return bad("trailing bytes after the last segment");
+  if (std::fseek(f, data_start, SEEK_SET) != 0) return bad("cannot position in the file");
   std::vector<char> buf;
   for (uint32_t i = 0; i < nseg; ++i) {
     uint64_t n = 0;

@@ -73,11 +73,15 @@ __device__ __forceinline__ void cell_phi(const Geo& g, const double* __restrict_
     ff[k][2] = w[0] * w[1] * w[2];
     ff[k][3] = w[1] * uu[k][3] + w[1] * w[3];
     primitive(g.gamma, uu[k], ww[k]);
-    // get_source (:747-767), grad_p = (1, 1)
+    // get_source (:747-767), grad_p = (1, 1): it receives the primitive
+    // vector (:681-684) and converts it once more (:756), so its velocities
+    // are ww[1]/ww[0] and ww[2]/ww[0]
+    double w2[4];
+    primitive(g.gamma, ww[k], w2);
     ss[k][0] = 0.0;
-    ss[k][1] = -ww[k][0] * 1.0;
-    ss[k][2] = -ww[k][0] * 1.0;
-    ss[k][3] = -ww[k][0] * (ww[k][1] * 1.0 + ww[k][2] * 1.0);
+    ss[k][1] = -w2[0] * 1.0;
+    ss[k][2] = -w2[0] * 1.0;
+    ss[k][3] = -w2[0] * (w2[1] * 1.0 + w2[2] * 1.0);
     // the reference hands compute_speed the primitive vector (:692-695)
     sp[k] = speed(g.gamma, ww[k]);
   }

@@ -4,7 +4,14 @@
 !
 !   rd_fvwb stage    [tend] [prefix]   evolve's loop on the host, compute_max_speed
 !                                      and compute_update_rd on the GPU
-!   rd_fvwb resident [tend] [prefix]   the whole loop resident on the device
+!   rd_fvwb resident [tend] [prefix] [checkpoint]
+!                                      the whole loop resident on the device; with a
+!                                      checkpoint path the device state at tend is
+!                                      also saved there (fvwb_rd_checkpoint_save)
+!   rd_fvwb resume   [tend] [prefix] checkpoint
+!                                      the device state and clock are loaded from the
+!                                      checkpoint (fvwb_rd_checkpoint_load) and
+!                                      advanced from their time to tend
 !
 ! Writes prefix.bin (stream): int32 nx, ny, steps, 0; real(8) t; the initial
 ! state u0(4,nx,ny); the final state u(4,nx,ny).
@@ -20,7 +27,7 @@ program rd_fvwb
   integer :: iter, i, j, nargs, unit
   integer(c_int64_t) :: steps
   character(len=32) :: mode, arg
-  character(len=256) :: prefix
+  character(len=256) :: prefix, ckpt
   type(c_ptr) :: h
   external :: compute_update_rd, compute_max_speed
 
@@ -34,6 +41,8 @@ program rd_fvwb
      read(arg, *) t_end
   end if
   if (nargs >= 3) call get_command_argument(3, prefix)
+  ckpt = ''
+  if (nargs >= 4) call get_command_argument(4, ckpt)
 
   ! get_coords (benchmark_rd.f90:24-51): nodes at (i-1) dx, (j-1) dy
   dx = boxlen_x / dble(nx - 1)
@@ -66,16 +75,23 @@ program rd_fvwb
      end do
      steps = iter
      call fvwb_rd_dropin_close()
-  else if (trim(mode) == 'resident') then
+  else if (trim(mode) == 'resident' .or. trim(mode) == 'resume') then
      call fvwb_rd_check(fvwb_rd_create(h, fvwb_rd_dropin_params()), c_null_ptr, 'fvwb_rd_create')
-     call fvwb_rd_check(fvwb_rd_upload(h, u), h, 'fvwb_rd_upload')
+     if (trim(mode) == 'resume') then
+        if (len_trim(ckpt) == 0) error stop 'rd_fvwb: resume needs a checkpoint path'
+        call fvwb_rd_check(fvwb_rd_checkpoint_load(h, trim(ckpt)//c_null_char), h, 'fvwb_rd_checkpoint_load')
+     else
+        call fvwb_rd_check(fvwb_rd_upload(h, u), h, 'fvwb_rd_upload')
+     end if
      call fvwb_rd_check(fvwb_rd_advance(h, huge(1_c_int32_t), t_end), h, 'fvwb_rd_advance')
+     if (trim(mode) == 'resident' .and. len_trim(ckpt) > 0) &
+          call fvwb_rd_check(fvwb_rd_checkpoint_save(h, trim(ckpt)//c_null_char), h, 'fvwb_rd_checkpoint_save')
      call fvwb_rd_check(fvwb_rd_download(h, u), h, 'fvwb_rd_download')
      call fvwb_rd_check(fvwb_rd_get_time(h, t, dt, steps), h, 'fvwb_rd_get_time')
      call fvwb_rd_check(fvwb_rd_destroy(h), c_null_ptr, 'fvwb_rd_destroy')
      write(*, *) 'time=', steps, t, dt
   else
-     write(0, *) 'usage: rd_fvwb stage|resident [tend] [prefix]'
+     write(0, *) 'usage: rd_fvwb stage|resident|resume [tend] [prefix] [checkpoint]'
      error stop 2
   end if
 

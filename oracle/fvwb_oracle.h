@@ -181,7 +181,7 @@ typedef struct {
 void   orc_rd_primitive(double gamma, const double u[4], double w[4]);          /* :521-532 */
 void   orc_rd_conservative(double gamma, const double w[4], double u[4]);       /* :535-547 */
 void   orc_rd_flux(double gamma, const double u[4], double f1[4], double f2[4]); /* :549-571 */
-void   orc_rd_source(const double w[4], double s[4]);                           /* :747-767 */
+void   orc_rd_source(double gamma, const double ww[4], double s[4]);            /* :747-767 */
 double orc_rd_speed(double gamma, const double u[4]);                           /* :478-493 */
 double orc_rd_max_speed(const orc_rd_params* p, const double* u);               /* :448-475 */
 /* corner residuals 11, 12, 21, 22 of cell (i, j) (:667-708) */

@@ -56,9 +56,14 @@ void orc_rd_flux(double gamma, const double u[4], double f1[4], double f2[4])
   f1[3] = w[1] * u[3] + w[1] * w[3];
 }
 
-/* benchmark_rd.f90:747-767 get_source with grad_p = (1, 1) */
-void orc_rd_source(const double w[4], double s[4])
+/* benchmark_rd.f90:747-767 get_source with grad_p = (1, 1).  The caller
+ * (:681-684) hands it the primitive vector ww, and get_source converts its
+ * argument to primitive variables once more (:756), so the energy source is
+ * -rho*(vx/rho + vy/rho); the second conversion's pressure is unused. */
+void orc_rd_source(double gamma, const double ww[4], double s[4])
 {
+  double w[4];
+  orc_rd_primitive(gamma, ww, w);
   s[0] = 0.0;
   s[1] = -w[0] * 1.0;
   s[2] = -w[0] * 1.0;
@@ -109,10 +114,10 @@ void orc_rd_cell(const orc_rd_params* p, const double* u, const double* diff, do
   orc_rd_primitive(p->gamma, uu22, ww22);
   orc_rd_primitive(p->gamma, uu21, ww21);
   double s11[4], s12[4], s21[4], s22[4];
-  orc_rd_source(ww11, s11);
-  orc_rd_source(ww12, s12);
-  orc_rd_source(ww21, s21);
-  orc_rd_source(ww22, s22);
+  orc_rd_source(p->gamma, ww11, s11);
+  orc_rd_source(p->gamma, ww12, s12);
+  orc_rd_source(p->gamma, ww21, s21);
+  orc_rd_source(p->gamma, ww22, s22);
   double totalphi[4];
   for (int v = 0; v < 4; ++v)
     totalphi[v] = -0.5 * (ff11[v] + ff21[v]) + 0.5 * (ff12[v] + ff22[v]) - 0.5 * (g22[v] + g21[v]) +

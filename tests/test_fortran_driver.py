@@ -422,3 +422,34 @@ def test_rd_driver_matches_oracle(tmp_path, mode):
     print(f"{mode}: steps {steps} vs {no}, t {t!r} vs {to!r}", flush=True)
     assert (nx, ny) == (65, 65) and steps == no == 162 and t == to
     np.testing.assert_array_equal(u, uo)
+
+
+def read_rd_bin(prefix):
+    with open(prefix + ".bin", "rb") as f:
+        nx, ny, steps, _ = np.frombuffer(f.read(16), dtype=np.int32)
+        t = np.frombuffer(f.read(8), dtype=np.float64)[0]
+        f.read(8 * 4 * nx * ny)
+        u = np.frombuffer(f.read(8 * 4 * nx * ny), dtype=np.float64).reshape((4, nx, ny), order="F")
+    return int(steps), float(t), u
+
+
+@pytest.mark.gpu
+def test_rd_driver_resume_from_checkpoint_equals_one_run(tmp_path):
+    """The resident residual-distribution driver saves its device state and
+    clock at t = 0.002 through the Fortran binding of fvwb_rd_checkpoint_save;
+    a second run loads that file (fvwb_rd_checkpoint_load) and continues to
+    t = 0.005, ending with the steps, time and state of one run to 0.005, bit
+    for bit."""
+    ck = str(tmp_path / "rd.ckpt")
+    r = run([exe("rd_fvwb"), "resident", "0.002", str(tmp_path / "a"), ck])
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = run([exe("rd_fvwb"), "resume", "0.005", str(tmp_path / "b"), ck])
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = run([exe("rd_fvwb"), "resident", "0.005", str(tmp_path / "c")])
+    assert r.returncode == 0, r.stdout + r.stderr
+    sa, ta, _ = read_rd_bin(str(tmp_path / "a"))
+    sb, tb, ub = read_rd_bin(str(tmp_path / "b"))
+    sc, tc, uc = read_rd_bin(str(tmp_path / "c"))
+    print(f"rd checkpoint at step {sa} (t {ta}); resumed to {sb} steps, t {tb}; one run {sc}, t {tc}", flush=True)
+    assert 0 < sa < sb and sb == sc == 162 and tb == tc
+    np.testing.assert_array_equal(ub, uc)

@@ -120,3 +120,53 @@ def test_load_rejects_foreign_and_damaged_files(tmp_path):
             s.checkpoint_load(cut)
         with pytest.raises(fvwb.FvwbError, match="cannot open"):
             s.checkpoint_load(str(tmp_path / "missing.ckpt"))
+
+
+def test_failed_load_into_a_live_handle(tmp_path):
+    """A handle that holds state and then fails to load a truncated or padded
+    file is left without state (its next advance is refused), and the file is
+    rejected before any of it reaches the device: a good load afterwards
+    resumes bit for bit."""
+    say("case: failed checkpoint load into a handle with state")
+    p = fvwb.make_params(48, 40, bc=fvwb.BC_REFLECT, recon=fvwb.RECON_MUSCL_MINMOD, flux=fvwb.FLUX_HLL,
+                         integrator=fvwb.SSPRK3, precision=fvwb.FP32, rho0=1.0)
+    path = str(tmp_path / "good.ckpt")
+    with fvwb.Solver2D(p) as a:
+        a.init_gaussian(1e-2, 0.4, 0.6, 80.0)
+        a.advance(6)
+        a.checkpoint_save(path)
+        a.advance(5)
+        want = a.download_deviation()
+    data = open(path, "rb").read()
+    cut, padded = str(tmp_path / "cut.ckpt"), str(tmp_path / "padded.ckpt")
+    with open(cut, "wb") as f:
+        f.write(data[:-100])                       # inside the last segment
+    with open(padded, "wb") as f:
+        f.write(data + b"\0" * 8)
+    for bad, why in ((cut, "truncated"), (padded, "trailing bytes")):
+        with fvwb.Solver2D(p) as b:
+            b.init_gaussian(1e-3, 0.5, 0.5, 100.0)
+            b.advance(2)
+            with pytest.raises(fvwb.FvwbError, match=why):
+                b.checkpoint_load(bad)
+            with pytest.raises(fvwb.FvwbError, match="upload"):
+                b.advance(1)
+            b.checkpoint_load(path)
+            b.advance(5)
+            assert np.array_equal(b.download_deviation(), want)
+    for make, bad in ((lambda: fvwb.SolverRD(fvwb.make_rd_params(33, 33)), cut),):
+        op = po.make_rd_params(33, 33, bc=po.RD_BC_TRANSMISSIVE)
+        rpath = str(tmp_path / "rd.ckpt")
+        with make() as r:
+            r.upload(po.rd_initial(op, 8, 1e-2))
+            r.advance(3)
+            r.checkpoint_save(rpath)
+        rdata = open(rpath, "rb").read()
+        with open(bad, "wb") as f:
+            f.write(rdata[:-8])
+        with make() as r:
+            r.upload(po.rd_initial(op, 8, 1e-2))
+            with pytest.raises(fvwb.FvwbError, match="truncated"):
+                r.checkpoint_load(bad)
+            with pytest.raises(fvwb.FvwbError):
+                r.advance(1)

@@ -20,6 +20,15 @@ import pyoracle as po
 GAMMA = po.GAMMA_REF
 
 
+def energy_source(rho, vx, vy):
+    """Minus the energy source of get_source (benchmark_rd.f90:747-767) as the
+    cell residual calls it: compute_update_rd hands it the primitive vector
+    (rho, vx, vy, p) (:681-684) and get_source converts that to primitive
+    variables again (:756), so its velocities are vx/rho and vy/rho and the
+    energy source is -rho (vx/rho + vy/rho), not -rho (vx + vy)."""
+    return rho * (vx / rho + vy / rho)
+
+
 def uniform(nx, ny, rho, a, b, p):
     u1 = po.rd_conservative(GAMMA, [rho, a, b, p])
     return np.asfortranarray(np.broadcast_to(u1[:, None, None], (4, nx, ny)).copy())
@@ -39,17 +48,21 @@ def random_state(nx, ny, seed):
     return u
 
 
+@pytest.mark.parametrize("state", [(1.3, 0.4, -0.25, 0.8), (2.0, 1.0, 1.0, 1.0)])
 @pytest.mark.parametrize("bc", [po.RD_BC_NONE, po.RD_BC_TRANSMISSIVE, po.RD_BC_PERIODIC])
-def test_uniform_state_closed_form(bc):
+def test_uniform_state_closed_form(bc, state):
     """For a uniform state every flux difference vanishes exactly and u_k =
     ubar, so a cell's corners each receive dt*(-s dx dy)/4: interior nodes get
-    dt dx dy (0, rho, rho, rho (a + b)), edges half, corners a quarter, before
-    the boundary copies (benchmark_rd.f90:722-737) are applied."""
-    nx, ny, rho, a, b, pr = 9, 7, 1.3, 0.4, -0.25, 0.8
+    dt dx dy (0, rho, rho, rho (a/rho + b/rho)), edges half, corners a quarter,
+    before the boundary copies (benchmark_rd.f90:722-737) are applied.  At
+    rho = 2, v = (1, 1) the energy entry is 2 dt dx dy; a source that skips
+    get_source's second primitive conversion gives 4."""
+    nx, ny = 9, 7
+    rho, a, b, pr = state
     p = po.make_rd_params(nx, ny, bc=bc, boxlen_x=1.0, boxlen_y=0.75)
     dx, dy, dt = 1.0 / (nx - 1), 0.75 / (ny - 1), 1e-3
     d = po.rd_update(p, uniform(nx, ny, rho, a, b, pr), np.zeros((4, nx, ny), order="F"), dt)
-    full = dt * dx * dy * np.array([0.0, rho, rho, rho * (a + b)])
+    full = dt * dx * dy * np.array([0.0, rho, rho, energy_source(rho, a, b)])
     count = np.full((nx, ny), 4.0)
     count[0, :] /= 2; count[-1, :] /= 2; count[:, 0] /= 2; count[:, -1] /= 2
     want = full[:, None, None] * count[None] / 4.0
@@ -79,7 +92,7 @@ def test_distributed_residual_is_conservative():
     pr = (GAMMA - 1.0) * (u[3] - 0.5 * rho * (vx ** 2 + vy ** 2))
     f = np.stack([rho * vx, rho * vx * vx + pr, rho * vx * vy, vx * (u[3] + pr)])
     g = np.stack([rho * vy, rho * vx * vy, rho * vy * vy + pr, vy * (u[3] + pr)])
-    s = np.stack([0 * rho, -rho, -rho, -rho * (vx + vy)])
+    s = np.stack([0 * rho, -rho, -rho, -energy_source(rho, vx, vy)])
     tw = np.ones(n); tw[0] = tw[-1] = 0.5                     # trapezoid weights
     fx = np.einsum("vj,j->v", f[:, -1, :], tw) - np.einsum("vj,j->v", f[:, 0, :], tw)
     gy = np.einsum("vi,i->v", g[:, :, -1], tw) - np.einsum("vi,i->v", g[:, :, 0], tw)
@@ -154,7 +167,7 @@ def test_linear_pressure_at_rest_closed_form():
 def test_max_speed_and_step_closed_forms():
     """compute_max_speed of a uniform state is |v| + sqrt(gamma p/rho); one
     'RD1' step takes dt = cfl dx dy/cmax/3 and, with transmissive copies, moves
-    the uniform state by -dt (0, rho, rho, rho (a + b))."""
+    the uniform state by -dt (0, rho, rho, rho (a/rho + b/rho))."""
     nx, ny, rho, a, b, pr = 7, 9, 0.9, 0.3, 0.5, 1.1
     p = po.make_rd_params(nx, ny, bc=po.RD_BC_TRANSMISSIVE, cfl=0.4)
     dx, dy = 1.0 / (nx - 1), 1.0 / (ny - 1)
@@ -164,7 +177,7 @@ def test_max_speed_and_step_closed_forms():
     u0 = u.copy(order="F")
     dt = po.rd_step(p, u)
     assert dt == pytest.approx(0.4 * dx * dy / c / 3.0, rel=1e-15)
-    want = u0 - dt * np.array([0.0, rho, rho, rho * (a + b)])[:, None, None]
+    want = u0 - dt * np.array([0.0, rho, rho, energy_source(rho, a, b)])[:, None, None]
     np.testing.assert_allclose(u, want, rtol=1e-13, atol=1e-16)
 
 
