@@ -4,10 +4,11 @@
 !
 !   rd_fvwb stage    [tend] [prefix]   evolve's loop on the host, compute_max_speed
 !                                      and compute_update_rd on the GPU
-!   rd_fvwb resident [tend] [prefix] [checkpoint]
-!                                      the whole loop resident on the device; with a
-!                                      checkpoint path the device state at tend is
-!                                      also saved there (fvwb_rd_checkpoint_save)
+!   rd_fvwb resident [tend] [prefix] [checkpoint] [nsteps]
+!                                      the whole loop resident on the device, at most
+!                                      nsteps steps; with a checkpoint path the device
+!                                      state at the end is also saved there
+!                                      (fvwb_rd_checkpoint_save)
 !   rd_fvwb resume   [tend] [prefix] checkpoint
 !                                      the device state and clock are loaded from the
 !                                      checkpoint (fvwb_rd_checkpoint_load) and
@@ -25,6 +26,7 @@ program rd_fvwb
   real(kind=8), dimension(1:nvar, 1:nx, 1:ny, 1:mx, 1:my) :: u, u0, diff, dudt, up
   real(kind=8) :: t, dt, cmax, dx, dy, t_end
   integer :: iter, i, j, nargs, unit
+  integer(c_int32_t) :: max_steps
   integer(c_int64_t) :: steps
   character(len=32) :: mode, arg
   character(len=256) :: prefix, ckpt
@@ -43,6 +45,11 @@ program rd_fvwb
   if (nargs >= 3) call get_command_argument(3, prefix)
   ckpt = ''
   if (nargs >= 4) call get_command_argument(4, ckpt)
+  max_steps = huge(1_c_int32_t)
+  if (nargs >= 5) then
+     call get_command_argument(5, arg)
+     read(arg, *) max_steps
+  end if
 
   ! get_coords (benchmark_rd.f90:24-51): nodes at (i-1) dx, (j-1) dy
   dx = boxlen_x / dble(nx - 1)
@@ -83,7 +90,7 @@ program rd_fvwb
      else
         call fvwb_rd_check(fvwb_rd_upload(h, u), h, 'fvwb_rd_upload')
      end if
-     call fvwb_rd_check(fvwb_rd_advance(h, huge(1_c_int32_t), t_end), h, 'fvwb_rd_advance')
+     call fvwb_rd_check(fvwb_rd_advance(h, max_steps, t_end), h, 'fvwb_rd_advance')
      if (trim(mode) == 'resident' .and. len_trim(ckpt) > 0) &
           call fvwb_rd_check(fvwb_rd_checkpoint_save(h, trim(ckpt)//c_null_char), h, 'fvwb_rd_checkpoint_save')
      call fvwb_rd_check(fvwb_rd_download(h, u), h, 'fvwb_rd_download')
@@ -91,7 +98,7 @@ program rd_fvwb
      call fvwb_rd_check(fvwb_rd_destroy(h), c_null_ptr, 'fvwb_rd_destroy')
      write(*, *) 'time=', steps, t, dt
   else
-     write(0, *) 'usage: rd_fvwb stage|resident|resume [tend] [prefix] [checkpoint]'
+     write(0, *) 'usage: rd_fvwb stage|resident|resume [tend] [prefix] [checkpoint] [nsteps]'
      error stop 2
   end if
 

@@ -449,6 +449,89 @@ int fvwb_dg1d_advance(fvwb_dg1d_handle h, int32_t nsteps, double t_end);
 int fvwb_dg1d_download(fvwb_dg1d_handle h, double* delta_u, double* u_init);
 int fvwb_dg1d_get_time(fvwb_dg1d_handle h, double* t, double* dt_last, int32_t* steps_done);
 
+/* ---- the reference's own 2D DG program (2d/benchmark_2d_dg.f90) ----------------
+ * Tensor-product (Q) Legendre basis with mx = my = order modes per direction,
+ * FP64, the reference's layouts: modes and nodal values
+ * delta_u(1:4,1:nx,1:ny,1:order,1:order) (element (v,i,j,a,b) at
+ * v + 4*(i + nx*(j + ny*(a + order*b))), mode / node indices slowest), node
+ * coordinates x, y(1:nx,1:ny,1:order,1:order) as get_coords (:91-119) builds
+ * them.  The switches are the compile-time constants of 2d/parameters_dg_2d.f90.
+ * The reference's residual indexes the x faces with the y bounds (:1338-1339)
+ * and sizes get_source's work array nx x nx (:1563), so it is defined for
+ * nx == ny only; create refuses other sizes.  Results equal the reference's
+ * expressions bit for bit (the basis and Gauss rule are built on the host with
+ * the reference's own expressions; the device does +, -, *, / and sqrt in the
+ * reference's order, without contraction). */
+#define FVWB_DGQ_FLUX_NONE 0   /* a flux_type matching no branch of compute_num_flux
+                                  (:991-1006), e.g. the shipped 'llf': F = G = 0 */
+#define FVWB_DGQ_FLUX_LLF1 1   /* 'llf1': compute_llflux, signed normal velocity + c */
+#define FVWB_DGQ_FLUX_HLL2 2   /* 'hll2': compute_hllflux (:1008-1026)                */
+#define FVWB_DGQ_FLUX_HLLC 3   /* 'hllc': compute_hllcflux (:1030-1134) as written   */
+#define FVWB_DGQ_SOLVER_RK4 0  /* 'RK4' (shipped): five-stage SSP, :683-708          */
+#define FVWB_DGQ_SOLVER_SS4 1  /* 'SS4': the same scheme, other literals, :709-731  */
+#define FVWB_DGQ_SOLVER_EQL 2  /* 'EQL': Heun, :672-681                              */
+#define FVWB_DGQ_SOLVER_DEB 3  /* 'DEB': one forward Euler stage, :733-742            */
+
+typedef struct fvwb_dgq_params {
+  int32_t nx, ny;          /* cells, nx == ny                                        */
+  int32_t order;           /* mx = my: 2 or 3 (the reference does not compile with 1:
+                              2d/limiters.f90 indexes the second mode)               */
+  int32_t flux_type;       /* FVWB_DGQ_FLUX_*                                        */
+  int32_t source;          /* 1: none; 2: get_source with grad_phi; 3: get_adv_source */
+  int32_t grad_phi_case;   /* 1: grad_p = (x, y); 2: point mass at (3, 3)            */
+  int32_t bc;              /* 1: periodic; 2, 3: transmissive (get_boundary_conditions) */
+  int32_t ninit;           /* 12 freezes dudt outside r = 2 (special_boundary_conditions);
+                              other values leave compute_update alone                */
+  int32_t solver;          /* FVWB_DGQ_SOLVER_*                                      */
+  int32_t use_limiter;     /* 1: apply_limiter with limiter_type 'ONP', i.e.
+                              compute_positivity (2d/limiters.f90:478-654)           */
+  double gamma, boxlen_x, boxlen_y, cfl;
+  double eps;              /* compute_positivity's threshold                         */
+} fvwb_dgq_params;
+
+typedef struct fvwb_dgq* fvwb_dgq_handle;
+
+/* The shipped set-up of 2d/parameters_dg_2d.f90 (nx = ny = 8, mx = my = 2,
+ * 'llf', source 1, grad_phi_case 2, bc 1, ninit 1, 'RK4', 'ONP', gamma 1.4 and
+ * eps 1e-10 as default-real literals, cfl 0.2, unit box). */
+int fvwb_dgq_params_reference(fvwb_dgq_params* p);
+int fvwb_dgq_create(fvwb_dgq_handle* out, const fvwb_dgq_params* p);
+int fvwb_dgq_destroy(fvwb_dgq_handle h);
+const char* fvwb_dgq_last_error(fvwb_dgq_handle h);
+/* 2d/benchmark_2d_dg.f90:1137-1479 compute_update(delta_u, x, y, u_eq, dudt);
+ * u_eq is unused by the reference and may be NULL. */
+int fvwb_dgq_compute_update(fvwb_dgq_handle h, const double* delta_u, const double* x, const double* y,
+                            const double* u_eq, double* dudt);
+/* :826-857 compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max) on
+ * u(1:4,1:nx,1:ny), with the reference's bookkeeping: v_xmax, v_ymax from the
+ * last cell (x outer, y inner) whose speed reaches the running maximum, cs_max
+ * the smallest sound speed from that cell on. */
+int fvwb_dgq_compute_max_speed(fvwb_dgq_handle h, const double* u, double* cs_max, double* v_xmax,
+                               double* v_ymax, double* speed_max);
+/* :1512-1554 apply_limiter(u) in place (compute_positivity when use_limiter) */
+int fvwb_dgq_apply_limiter(fvwb_dgq_handle h, double* u);
+/* :497-592 get_modes_from_nodes(nodes, u) and get_nodes_from_modes(u, nodes) */
+int fvwb_dgq_modes_from_nodes(fvwb_dgq_handle h, const double* nodes, double* modes);
+int fvwb_dgq_nodes_from_modes(fvwb_dgq_handle h, const double* modes, double* nodes);
+/* :624-775 evolve, resident on the device: upload takes the limited modes and
+ * the node coordinates and resets the clock; advance takes at most nsteps
+ * steps while t < t_end, each with the dt of :671 from the device's
+ * compute_max_speed of the means, the configured solver and apply_limiter
+ * after every stage; FVWB_ERR_NONPHYSICAL stops the clock if dt stops being
+ * finite and positive. */
+int fvwb_dgq_upload(fvwb_dgq_handle h, const double* delta_u, const double* x, const double* y);
+int fvwb_dgq_advance(fvwb_dgq_handle h, int32_t nsteps, double t_end);
+int fvwb_dgq_download(fvwb_dgq_handle h, double* delta_u);
+int fvwb_dgq_get_time(fvwb_dgq_handle h, double* t, double* dt_last, int64_t* steps_done);
+/* :23-89 compute_error: per variable the maximum |u - u_init| (linf), the
+ * Gauss-weighted L1 norm sum_cells (sum_ab |u - u_init| w_a w_b)(dx dy)/4 and
+ * the L2 norm, with dx = 1/nx, dy = 1/ny as the reference takes them; a device
+ * reduction (its summation order differs from the reference's serial loop, so
+ * the norms agree to rounding).  nodes may be NULL: then the nodal values of
+ * the resident state are used. */
+int fvwb_dgq_compute_error(fvwb_dgq_handle h, const double* nodes, const double* u_init, double* linf,
+                           double* l1, double* l2);
+
 /* ---- checkpoint / resume ----------------------------------------------------
  * No counterpart in the reference, whose snapshots are six-digit ASCII and not
  * restartable (benchmark_2d.f90:115-143).  save writes the device state a time

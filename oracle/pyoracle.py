@@ -122,6 +122,88 @@ class RDParams(ctypes.Structure):
 RD_BC_NONE, RD_BC_PERIODIC, RD_BC_TRANSMISSIVE = 0, 2, 3
 
 
+class DGQParams(ctypes.Structure):
+    """Mirror of ``fvwb_dgq_params`` (include/fvwb.h): the compile-time switches
+    of 2d/parameters_dg_2d.f90 for the reference's own 2D DG program."""
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("flux_type", ctypes.c_int32), ("source", ctypes.c_int32), ("grad_phi_case", ctypes.c_int32),
+                ("bc", ctypes.c_int32), ("ninit", ctypes.c_int32), ("solver", ctypes.c_int32),
+                ("use_limiter", ctypes.c_int32), ("gamma", ctypes.c_double), ("boxlen_x", ctypes.c_double),
+                ("boxlen_y", ctypes.c_double), ("cfl", ctypes.c_double), ("eps", ctypes.c_double)]
+
+
+DGQ_FLUX_NONE, DGQ_FLUX_LLF1, DGQ_FLUX_HLL2, DGQ_FLUX_HLLC = 0, 1, 2, 3
+DGQ_SOLVER_RK4, DGQ_SOLVER_SS4, DGQ_SOLVER_EQL, DGQ_SOLVER_DEB = 0, 1, 2, 3
+EPS_REF_DG = float(np.float32(1e-10))    # 2d/parameters_dg_2d.f90:30, default-real literal
+
+
+def make_dgq_params(n=8, *, order=2, flux_type=DGQ_FLUX_NONE, source=1, grad_phi_case=2, bc=1, ninit=1,
+                    solver=DGQ_SOLVER_RK4, use_limiter=1, gamma=GAMMA_REF, boxlen_x=1.0, boxlen_y=1.0,
+                    cfl=float(np.float32(0.2)), eps=EPS_REF_DG) -> DGQParams:
+    """Defaults: the shipped 2d/parameters_dg_2d.f90 (8x8, mx = my = 2, 'llf',
+    source 1, grad_phi_case 2, bc 1, ninit 1, 'RK4', 'ONP', cfl 0.2 as a default-real
+    literal)."""
+    return DGQParams(n, n, order, flux_type, source, grad_phi_case, bc, ninit, solver, use_limiter, gamma,
+                     boxlen_x, boxlen_y, cfl, eps)
+
+
+def dgq_coords(p: DGQParams):
+    """get_coords (2d/benchmark_2d_dg.f90:91-119): x(i,j,a,b) = (i-0.5) dx +
+    dx/2 x_quad(a), with (i-0.5) in default REAL as the reference writes it."""
+    nx, ny, o = p.nx, p.ny, p.order
+    xq, _ = dg_gl_quadrature(o)
+    dx, dy = p.boxlen_x / nx, p.boxlen_y / ny
+    ic = (np.arange(1, nx + 1, dtype=np.float32) - np.float32(0.5)).astype(np.float64)
+    jc = (np.arange(1, ny + 1, dtype=np.float32) - np.float32(0.5)).astype(np.float64)
+    x = np.zeros((nx, ny, o, o), order="F")
+    y = np.zeros((nx, ny, o, o), order="F")
+    for a in range(o):
+        for b in range(o):
+            x[:, :, a, b] = (ic * dx + dx / 2.0 * xq[a])[:, None]
+            y[:, :, a, b] = (jc * dy + dy / 2.0 * xq[b])[None, :]
+    return x, y
+
+
+def refdg_update(p: DGQParams, delta_u, x, y):
+    """compute_update (2d/benchmark_2d_dg.f90:1137-1479) as shipped."""
+    assert lib().orc_refdg_valid(ctypes.byref(p))
+    du, x, y = fortran(delta_u), fortran(x), fortran(y)
+    out = np.zeros_like(du, order="F")
+    lib().orc_refdg_update(ctypes.byref(p), _p(du), _p(x), _p(y), _p(out))
+    return out
+
+
+def refdg_max_speed(p: DGQParams, u):
+    """compute_max_speed (:826-857) on u(1:4,1:nx,1:ny): (cs_max, v_xmax, v_ymax, speed_max)."""
+    out = np.zeros(4)
+    lib().orc_refdg_max_speed(ctypes.byref(p), _p(fortran(u)), _p(out))
+    return tuple(out)
+
+
+def refdg_limiter(p: DGQParams, u):
+    u = fortran(u).copy(order="F")
+    lib().orc_refdg_limiter(ctypes.byref(p), _p(u))
+    return u
+
+
+def refdg_evolve(p: DGQParams, delta_u, x, y, t_end, max_steps=1 << 30, t0=0.0):
+    """evolve's loop (:672-759) from a limited state: (modes, t, steps, dts)."""
+    du = fortran(delta_u).copy(order="F")
+    t = ctypes.c_double(t0)
+    nmax = int(min(max_steps, 1 << 20))
+    dts = np.zeros(nmax)
+    n = lib().orc_refdg_evolve(ctypes.byref(p), _p(du), _p(fortran(x)), _p(fortran(y)), float(t_end), nmax,
+                               ctypes.byref(t), _p(dts))
+    return du, t.value, n, dts[:n]
+
+
+def refdg_error(p: DGQParams, u, u_init):
+    """compute_error (:23-89): (linf[4], l1[4], l2[4])."""
+    out = np.zeros(12)
+    lib().orc_refdg_error(p.nx, p.ny, p.order, _p(fortran(u)), _p(fortran(u_init)), _p(out))
+    return out[:4], out[4:8], out[8:]
+
+
 def make_rd_params(nx, ny, *, bc=RD_BC_TRANSMISSIVE, gamma=GAMMA_REF, boxlen_x=1.0, boxlen_y=1.0,
                    cfl=0.5) -> RDParams:
     return RDParams(nx, ny, bc, 0, gamma, boxlen_x, boxlen_y, cfl)
@@ -228,6 +310,17 @@ def lib():
         L.orc_rd_initial.argtypes = [pr, ctypes.c_int, ctypes.c_double, dp]
         L.orc_rd_initial.restype = ctypes.c_int
         L.orc_rd_conservative.argtypes = [ctypes.c_double, dp, dp]
+        pq = ctypes.POINTER(DGQParams)
+        L.orc_refdg_valid.argtypes = [pq]
+        L.orc_refdg_valid.restype = ctypes.c_int
+        L.orc_refdg_update.argtypes = [pq, dp, dp, dp, dp]
+        L.orc_refdg_max_speed.argtypes = [pq, dp, dp]
+        L.orc_refdg_dt.argtypes = [pq, dp, ctypes.c_double, ctypes.c_double, dp]
+        L.orc_refdg_dt.restype = ctypes.c_double
+        L.orc_refdg_limiter.argtypes = [pq, dp]
+        L.orc_refdg_evolve.argtypes = [pq, dp, dp, dp, ctypes.c_double, ctypes.c_int, dp, dp]
+        L.orc_refdg_evolve.restype = ctypes.c_int
+        L.orc_refdg_error.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp]
         _lib = L
     return _lib
 

@@ -236,6 +236,71 @@ def plain_fixture():
     return dict(summarize(d), salt=3, amp_bump=1e-2, amp_noise=1e-3)
 
 
+# the reference's 2D DG builds: the shipped set-up and three edits of its
+# compile-time switches (oracle/Makefile); the oracle's parameters for each
+DGQ_BUILDS = {
+    "dg2d": {},
+    "dgq_hll2": dict(order=3, flux_type=po.DGQ_FLUX_HLL2, source=2, solver=po.DGQ_SOLVER_DEB),
+    "dgq_llf1": dict(flux_type=po.DGQ_FLUX_LLF1, source=2, grad_phi_case=1, bc=2, ninit=2,
+                     solver=po.DGQ_SOLVER_SS4),
+    "dgq_hllc": dict(order=3, flux_type=po.DGQ_FLUX_HLLC, source=3, bc=3, ninit=12, solver=po.DGQ_SOLVER_EQL),
+}
+DGQ_TEND = {"dg2d": 0.05, "dgq_hll2": 0.02, "dgq_llf1": 0.05, "dgq_hllc": 0.05}
+
+
+def dgq_perturbed_modes(lib, u0, o, seed=5):
+    """modes of the build's own initial nodal state with noise on every
+    variable (projected by the reference's get_modes_from_nodes)"""
+    rng = np.random.default_rng(seed)
+    nodes = u0.copy(order="F")
+    nodes[0] *= 1 + 0.05 * rng.standard_normal(nodes[0].shape)
+    nodes[1] += 0.1 * rng.standard_normal(nodes[0].shape)
+    nodes[2] += 0.1 * rng.standard_normal(nodes[0].shape)
+    nodes[3] *= 1 + 0.05 * rng.random(nodes[0].shape)
+    nodes = np.asfortranarray(nodes.astype(np.float32).astype(np.float64))
+    return rr.run("dg2d_modes_from_nodes", a=nodes)["r"] if lib == "dg2d" else _modes(nodes, o)
+
+
+def _modes(nodes, o):
+    m = np.zeros_like(nodes, order="F")
+    po.lib().orc_ref_dg2d_modes_from_nodes(nodes.shape[1], nodes.shape[2], o, po._p(nodes), po._p(m))
+    return m
+
+
+def dgq_fixtures():
+    """The reference's own 2D DG program, per build: its set-up (get_coords,
+    get_initial_conditions, get_equilibrium_solution), compute_update on a
+    perturbed state, compute_max_speed of its means, apply_limiter, and a whole
+    evolve run from its initial condition (every dt, the final nodal state,
+    compute_error's printed norms, and the snapshot files output_file wrote)."""
+    out = {}
+    for lib, kw in DGQ_BUILDS.items():
+        p = po.make_dgq_params(8, **kw)
+        o = p.order
+        r = rr.run("dgq_setup", lib=np.array(lib), n=np.array(8), o=np.array(o))
+        x, y, u0 = r["x"], r["y"], r["u"]
+        du = dgq_perturbed_modes(lib, u0, o)
+        d = rr.run("dgq_update", lib=np.array(lib), delta_u=du, x=x, y=y)["dudt"]
+        f = {"x": x, "y": y, "u0": u0, "u_eq": r["u_eq"], "delta_u": du, "dudt": d}
+        if p.ninit == 12:       # coordinates scaled so that special_boundary_conditions freezes cells
+            f["dudt_x6"] = rr.run("dgq_update", lib=np.array(lib), delta_u=du, x=6 * x, y=6 * y)["dudt"]
+        means = np.asfortranarray(du[:, :, :, 0, 0])
+        f["max_speed"] = rr.run("dgq_max_speed", lib=np.array(lib), u=means)["out"]
+        lim_in = du.copy(order="F")
+        lim_in[:, :, :, 1:, :] *= 40.0        # steep enough that density and pressure are limited
+        lim_in[:, :, :, 0, 1:] *= 40.0
+        f["limiter_in"] = lim_in
+        f["limiter_out"] = rr.run("dgq_limiter", lib=np.array(lib), u=lim_in)["u"]
+        ev = rr.run_dg2d_evolve(lib, u0, x, y, DGQ_TEND[lib])
+        f.update(tend=np.array(DGQ_TEND[lib]), evolve_u=ev["u"], evolve_t=ev["t"], evolve_dt=ev["dt"],
+                 evolve_iters=ev["iters"], norms=ev["norms"], norm_keys=ev["norm_keys"], files=ev["files"])
+        for k in ev:
+            if k.startswith("file_"):
+                f[k] = ev[k]
+        out["dgq_" + ("shipped" if lib == "dg2d" else lib[4:])] = f
+    return out
+
+
 def main(only=None):
     assert rr.available("bench2d") and rr.available("dg2d"), "build oracle/_ref first"
     out = {}
@@ -247,6 +312,8 @@ def main(only=None):
         out.update(fvm1d_fixtures())
     elif only == "dg1d":
         out.update(dg1d_fixtures())
+    elif only == "dgq":
+        out.update(dgq_fixtures())
     if out:
         for name, arrs in out.items():
             path = os.path.join(HERE, name + ".npz")
@@ -346,6 +413,9 @@ def main(only=None):
     a = np.asfortranarray(np.random.default_rng(17).normal(size=(4, 8, 8, 2, 2)))
     out["dg2d_projections"] = {"a": a, "modes_from_nodes": rr.run("dg2d_modes_from_nodes", a=a)["r"],
                                "nodes_from_modes": rr.run("dg2d_nodes_from_modes", a=a)["r"]}
+
+    # the reference's own 2D DG program, shipped and with edited switches
+    out.update(dgq_fixtures())
 
     for name, arrs in out.items():
         path = os.path.join(HERE, name + ".npz")

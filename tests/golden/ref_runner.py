@@ -33,6 +33,10 @@ LIBS = {
     "dg2d": "libref_dg2d.so",
     "dg1d": "libref_dg1d.so",
     "fvm": "libref_fvm.so",
+    # variants of the 2D DG set-up's compile-time switches (oracle/Makefile)
+    "dgq_hll2": "libref_dgq_hll2.so",
+    "dgq_llf1": "libref_dgq_llf1.so",
+    "dgq_hllc": "libref_dgq_hllc.so",
 }
 
 
@@ -52,6 +56,45 @@ def run(op: str, **arrays):
                            stdout=dn, check=True)
         with np.load(fout) as z:
             return {k: z[k] for k in z.files}
+
+
+def run_dg2d_evolve(lib: str, u, x, y, tend):
+    """Call evolve (2d/benchmark_2d_dg.f90:624-775) of the 2D DG build `lib`
+    ("dg2d" for the shipped set-up, or a variant) on the nodal state u with
+    tend set in its module, in a child process whose working directory holds
+    the longrotation2_o1/ folder that output_file writes into and whose stdin
+    is not a terminal (so the two `pause` statements do not wait).  Returns
+    the final nodal state, the (iter, t, dt) of every step from its
+    'time=' lines (:746), compute_error's printed norms (:38-88), and the
+    snapshot files it wrote with their names."""
+    import re
+    with tempfile.TemporaryDirectory() as td:
+        os.makedirs(os.path.join(td, "longrotation2_o1"))
+        fin, fout = os.path.join(td, "in.npz"), os.path.join(td, "out.npz")
+        np.savez(fin, u=np.asarray(u), x=np.asarray(x), y=np.asarray(y), tend=np.array(float(tend)),
+                 lib=np.array(lib))
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "dgq_evolve", fin, fout], cwd=td,
+                           stdin=subprocess.DEVNULL, capture_output=True, text=True, check=True, timeout=1800)
+        with np.load(fout) as z:
+            out = {k: z[k] for k in z.files}
+        rows = re.findall(r"time=\s+(\d+)\s+(\S+)\s+(\S+)", r.stdout)
+        out["iters"] = np.array([int(a) for a, _, _ in rows])
+        out["t"] = np.array([float(b) for _, b, _ in rows])
+        out["dt"] = np.array([float(c) for _, _, c in rows])
+        norms = {}
+        for key in ("maxerror rho", "maxerror velx", "maxerror vely", "maxerror energy", "l1error rho",
+                    "l1error velx", "l1error vely", "l1error energy", "l2error rho", "l2error velx",
+                    "l2error vely", "l2error energy"):
+            m = re.findall(re.escape(key) + r"\s+(\S+)", r.stdout)
+            norms[key] = float(m[-1])
+        out["norms"] = np.array([norms[k] for k in sorted(norms)])
+        out["norm_keys"] = np.array(sorted(norms))
+        files = sorted(os.listdir(os.path.join(td, "longrotation2_o1")))
+        out["files"] = np.array(files)
+        for f in files:
+            with open(os.path.join(td, "longrotation2_o1", f)) as fh:
+                out["file_" + f] = np.array(fh.read())
+    return out
 
 
 def run_dg1d_program():
@@ -196,6 +239,44 @@ def _child(op, fin, fout):
                 L.riemann_hllc_(_dp(a), _dp(b), _dp(o), ctypes.byref(gam), ctypes.byref(nv))
                 res[q] = o
             out["f"] = res
+        else:
+            raise ValueError(op)
+    elif op.startswith("dgq_"):
+        L = ctypes.CDLL(os.path.join(REF_DIR, LIBS[str(z["lib"])]))
+        mod = "_QMparameters_dg_2dE"
+        if op == "dgq_update":
+            du = np.asfortranarray(z["delta_u"]); x = np.asfortranarray(z["x"]); y = np.asfortranarray(z["y"])
+            ueq = np.zeros_like(du, order="F")
+            d = np.zeros_like(du, order="F")
+            L.compute_update_(_dp(du), _dp(x), _dp(y), _dp(ueq), _dp(d))
+            out["dudt"] = d
+        elif op == "dgq_max_speed":
+            u = np.asfortranarray(z["u"])
+            r = [ctypes.c_double() for _ in range(4)]
+            L.compute_max_speed_(_dp(u), *[ctypes.byref(v) for v in r])
+            out["out"] = np.array([v.value for v in r])
+        elif op == "dgq_limiter":
+            u = np.asfortranarray(z["u"]).copy(order="F")
+            L.apply_limiter_(_dp(u))
+            out["u"] = u
+        elif op == "dgq_setup":
+            n, o = int(z["n"]), int(z["o"])
+            x = np.zeros((n, n, o, o), order="F"); y = np.zeros((n, n, o, o), order="F")
+            u = np.zeros((4, n, n, o, o), order="F"); ueq = np.zeros((4, n, n, o, o), order="F")
+            sx, sy, ox, oy = (ctypes.c_int(v) for v in (n, n, o, o))
+            args = (ctypes.byref(sx), ctypes.byref(sy), ctypes.byref(ox), ctypes.byref(oy))
+            L.get_coords_(_dp(x), _dp(y), *args)
+            L.get_initial_conditions_(_dp(x), _dp(y), _dp(u), *args)
+            L.get_equilibrium_solution_(_dp(x), _dp(y), _dp(ueq), *args)
+            out.update(x=x, y=y, u=u, u_eq=ueq)
+        elif op == "dgq_evolve":
+            u = np.asfortranarray(z["u"]).copy(order="F")
+            x = np.asfortranarray(z["x"]); y = np.asfortranarray(z["y"])
+            ueq = np.zeros_like(u, order="F")
+            ctypes.c_double.in_dll(L, mod + "tend").value = float(z["tend"])
+            L.evolve_(_dp(u), _dp(x), _dp(y), _dp(ueq))
+            sys.stdout.flush()
+            out["u"] = u
         else:
             raise ValueError(op)
     elif op.startswith("dg2d_"):

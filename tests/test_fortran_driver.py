@@ -435,13 +435,14 @@ def read_rd_bin(prefix):
 
 @pytest.mark.gpu
 def test_rd_driver_resume_from_checkpoint_equals_one_run(tmp_path):
-    """The resident residual-distribution driver saves its device state and
-    clock at t = 0.002 through the Fortran binding of fvwb_rd_checkpoint_save;
-    a second run loads that file (fvwb_rd_checkpoint_load) and continues to
-    t = 0.005, ending with the steps, time and state of one run to 0.005, bit
-    for bit."""
+    """The resident residual-distribution driver stops after 58 steps of a
+    run to t = 0.005 (a step boundary of the uninterrupted run, so no step is
+    clipped) and saves its device state and clock through the Fortran binding
+    of fvwb_rd_checkpoint_save; a second run loads that file
+    (fvwb_rd_checkpoint_load) and continues to t = 0.005, ending with the
+    steps, time and state of one run to 0.005, bit for bit."""
     ck = str(tmp_path / "rd.ckpt")
-    r = run([exe("rd_fvwb"), "resident", "0.002", str(tmp_path / "a"), ck])
+    r = run([exe("rd_fvwb"), "resident", "0.005", str(tmp_path / "a"), ck, "58"])
     assert r.returncode == 0, r.stdout + r.stderr
     r = run([exe("rd_fvwb"), "resume", "0.005", str(tmp_path / "b"), ck])
     assert r.returncode == 0, r.stdout + r.stderr
@@ -451,5 +452,5 @@ def test_rd_driver_resume_from_checkpoint_equals_one_run(tmp_path):
     sb, tb, ub = read_rd_bin(str(tmp_path / "b"))
     sc, tc, uc = read_rd_bin(str(tmp_path / "c"))
     print(f"rd checkpoint at step {sa} (t {ta}); resumed to {sb} steps, t {tb}; one run {sc}, t {tc}", flush=True)
-    assert 0 < sa < sb and sb == sc == 162 and tb == tc
+    assert sa == 58 and 0 < ta < tb and sb == sc == 162 and tb == tc
     np.testing.assert_array_equal(ub, uc)
