@@ -122,6 +122,35 @@ class RDParams(ctypes.Structure):
 RD_BC_PERIODIC, RD_BC_TRANSMISSIVE = 2, 3
 
 
+class DGQParams(ctypes.Structure):
+    """``fvwb_dgq_params`` of include/fvwb.h: the compile-time switches of
+    2d/parameters_dg_2d.f90 for the reference's own 2D DG program."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("order", ctypes.c_int32),
+        ("flux_type", ctypes.c_int32), ("source", ctypes.c_int32), ("grad_phi_case", ctypes.c_int32),
+        ("bc", ctypes.c_int32), ("ninit", ctypes.c_int32), ("solver", ctypes.c_int32),
+        ("use_limiter", ctypes.c_int32), ("gamma", ctypes.c_double), ("boxlen_x", ctypes.c_double),
+        ("boxlen_y", ctypes.c_double), ("cfl", ctypes.c_double), ("eps", ctypes.c_double),
+    ]
+
+
+DGQ_FLUX_NONE, DGQ_FLUX_LLF1, DGQ_FLUX_HLL2, DGQ_FLUX_HLLC = 0, 1, 2, 3
+DGQ_SOLVER_RK4, DGQ_SOLVER_SS4, DGQ_SOLVER_EQL, DGQ_SOLVER_DEB = 0, 1, 2, 3
+
+
+def reference_params_dgq(n=8, **changes) -> DGQParams:
+    """The shipped 2d/parameters_dg_2d.f90 set-up (8x8, mx = my = 2, 'llf',
+    source 1, grad_phi_case 2, bc 1, ninit 1, 'RK4', 'ONP'; gamma, cfl and eps
+    as default-real literals) at n x n cells, with `changes` applied."""
+    p = DGQParams()
+    _checkdgq(lib().fvwb_dgq_params_reference(ctypes.byref(p)))
+    p.nx = p.ny = n
+    for k, v in changes.items():
+        setattr(p, k, v)
+    return p
+
+
 def make_rd_params(nx, ny, *, bc=RD_BC_TRANSMISSIVE, gamma=float(np.float32(1.4)), boxlen_x=1.0, boxlen_y=1.0,
                    cfl=0.5) -> RDParams:
     """gamma defaults to the reference's default-real 1.4, as its other parameter modules set it."""
@@ -233,6 +262,22 @@ def lib():
         "fvwb_rd_download": [H, dp],
         "fvwb_rd_get_time": [H, dp, dp, ctypes.POINTER(ctypes.c_int64)],
     })
+    pq = ctypes.POINTER(DGQParams)
+    sig.update({
+        "fvwb_dgq_params_reference": [pq],
+        "fvwb_dgq_create": [ctypes.POINTER(H), pq],
+        "fvwb_dgq_destroy": [H],
+        "fvwb_dgq_compute_update": [H, dp, dp, dp, dp, dp],
+        "fvwb_dgq_compute_max_speed": [H, dp, dp, dp, dp, dp],
+        "fvwb_dgq_apply_limiter": [H, dp],
+        "fvwb_dgq_modes_from_nodes": [H, dp, dp],
+        "fvwb_dgq_nodes_from_modes": [H, dp, dp],
+        "fvwb_dgq_upload": [H, dp, dp, dp],
+        "fvwb_dgq_advance": [H, ctypes.c_int32, ctypes.c_double],
+        "fvwb_dgq_download": [H, dp],
+        "fvwb_dgq_get_time": [H, dp, dp, ctypes.POINTER(ctypes.c_int64)],
+        "fvwb_dgq_compute_error": [H, dp, dp, dp, dp, dp],
+    })
     p1 = ctypes.POINTER(Params1D)
     sig.update({
         "fvwb1d_params_reference": [p1, ctypes.c_int32, ctypes.c_int32],
@@ -267,6 +312,8 @@ def lib():
     L.fvwb_dg1d_last_error.restype = ctypes.c_char_p
     L.fvwb_rd_last_error.argtypes = [H]
     L.fvwb_rd_last_error.restype = ctypes.c_char_p
+    L.fvwb_dgq_last_error.argtypes = [H]
+    L.fvwb_dgq_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -924,3 +971,109 @@ class SolverRD:
         n = ctypes.c_int64()
         _checkrd(lib().fvwb_rd_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), self._h)
         return t.value, dt.value, n.value
+
+
+def _checkdgq(rc, h=None):
+    if rc != OK:
+        msg = lib().fvwb_dgq_last_error(h)
+        raise FvwbError(rc, msg.decode() if msg else "")
+
+
+class SolverDGQ:
+    """The reference's own 2D DG program (2d/benchmark_2d_dg.f90: tensor
+    Legendre basis, mx = my = order, its flux dispatch, sources, positivity
+    limiter and solvers) on the GPU in FP64, equal to the reference bit for bit.
+    Modes and nodal values are float64 (4, nx, ny, order, order), order="F"
+    (the reference's delta_u(1:4,1:nx,1:ny,1:mx,1:my)); coordinates
+    (nx, ny, order, order)."""
+
+    def __init__(self, params: DGQParams):
+        self.params = params
+        o = params.order
+        self.shape = (4, params.nx, params.ny, o, o)
+        self.cshape = (params.nx, params.ny, o, o)
+        self._h = ctypes.c_void_p()
+        _checkdgq(lib().fvwb_dgq_create(ctypes.byref(self._h), ctypes.byref(params)))
+
+    def close(self):
+        if self._h:
+            lib().fvwb_dgq_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _arr(self, a, shape=None):
+        shape = shape or self.shape
+        a = np.asfortranarray(a, dtype=np.float64)
+        if a.shape != shape:
+            raise FvwbError(ERR_ARG, f"expected shape {shape}, got {a.shape}")
+        return a
+
+    def compute_update(self, delta_u, x, y, u_eq=None):
+        """compute_update(delta_u, x, y, u_eq, dudt) (2d/benchmark_2d_dg.f90:1137-1479)."""
+        a, xx, yy = self._arr(delta_u), self._arr(x, self.cshape), self._arr(y, self.cshape)
+        out = np.zeros(self.shape, order="F")
+        _checkdgq(lib().fvwb_dgq_compute_update(self._h, _dp(a), _dp(xx), _dp(yy), None, _dp(out)), self._h)
+        return out
+
+    def compute_max_speed(self, u):
+        """compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max) (:826-857) of u(4,nx,ny)."""
+        a = self._arr(u, (4, self.params.nx, self.params.ny))
+        r = [ctypes.c_double() for _ in range(4)]
+        _checkdgq(lib().fvwb_dgq_compute_max_speed(self._h, _dp(a), *[ctypes.byref(v) for v in r]), self._h)
+        return tuple(v.value for v in r)
+
+    def apply_limiter(self, u):
+        """apply_limiter(u) (:1512-1554), 'ONP': returns the limited modes."""
+        a = self._arr(u).copy(order="F")
+        _checkdgq(lib().fvwb_dgq_apply_limiter(self._h, _dp(a)), self._h)
+        return a
+
+    def modes_from_nodes(self, nodes):
+        out = np.zeros(self.shape, order="F")
+        _checkdgq(lib().fvwb_dgq_modes_from_nodes(self._h, _dp(self._arr(nodes)), _dp(out)), self._h)
+        return out
+
+    def nodes_from_modes(self, modes):
+        out = np.zeros(self.shape, order="F")
+        _checkdgq(lib().fvwb_dgq_nodes_from_modes(self._h, _dp(self._arr(modes)), _dp(out)), self._h)
+        return out
+
+    def upload(self, delta_u, x, y):
+        _checkdgq(lib().fvwb_dgq_upload(self._h, _dp(self._arr(delta_u)), _dp(self._arr(x, self.cshape)),
+                                        _dp(self._arr(y, self.cshape))), self._h)
+
+    def advance(self, nsteps=(1 << 31) - 1, t_end=math.inf):
+        """evolve's loop (:665-748) while t < t_end, at most nsteps steps."""
+        _checkdgq(lib().fvwb_dgq_advance(self._h, int(nsteps), float(t_end)), self._h)
+
+    def download(self):
+        out = np.zeros(self.shape, order="F")
+        _checkdgq(lib().fvwb_dgq_download(self._h, _dp(out)), self._h)
+        return out
+
+    def time(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int64()
+        _checkdgq(lib().fvwb_dgq_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), self._h)
+        return t.value, dt.value, n.value
+
+    def compute_error(self, u_init, nodes=None):
+        """compute_error (:23-89): (linf[4], l1[4], l2[4]) of nodes (default: the
+        resident state's nodal values) against u_init."""
+        ui = self._arr(u_init)
+        nd = None if nodes is None else self._arr(nodes)
+        linf, l1, l2 = np.zeros(4), np.zeros(4), np.zeros(4)
+        _checkdgq(lib().fvwb_dgq_compute_error(self._h, None if nd is None else _dp(nd), _dp(ui), _dp(linf),
+                                               _dp(l1), _dp(l2)), self._h)
+        return linf, l1, l2

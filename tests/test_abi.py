@@ -73,3 +73,31 @@ def test_checkpoint_null_arguments_are_argument_errors():
                  "fvwb_dg_checkpoint_save", "fvwb_dg_checkpoint_load", "fvwb_dg1d_checkpoint_save",
                  "fvwb_dg1d_checkpoint_load"):
         assert getattr(L, name)(None, b"/tmp/x") == fvwb.ERR_ARG, name
+
+
+def test_dgq_reference_params_match_parameters_dg_2d():
+    p = fvwb.reference_params_dgq()
+    assert (p.nx, p.ny, p.order, p.flux_type, p.source, p.grad_phi_case, p.bc, p.ninit, p.solver,
+            p.use_limiter) == (8, 8, 2, fvwb.DGQ_FLUX_NONE, 1, 2, 1, 1, fvwb.DGQ_SOLVER_RK4, 1)
+    assert p.gamma == 1.399999976158142 and p.cfl == 0.20000000298023224 and p.eps == 1.0000000133514320e-10
+    assert p.boxlen_x == 1.0 and p.boxlen_y == 1.0
+
+
+@pytest.mark.parametrize("field,value", [("ny", 9), ("nx", 0), ("order", 1), ("order", 4), ("flux_type", 4),
+                                         ("source", 0), ("source", 4), ("grad_phi_case", 3), ("bc", 0),
+                                         ("solver", 4), ("gamma", 1.0), ("cfl", 0.0), ("boxlen_y", -1.0)])
+def test_dgq_create_rejects_bad_params_before_any_device_call(field, value):
+    p = fvwb.reference_params_dgq()
+    setattr(p, field, value)
+    h = ctypes.c_void_p()
+    rc = fvwb.lib().fvwb_dgq_create(ctypes.byref(h), ctypes.byref(p))
+    assert rc == fvwb.ERR_ARG and not h.value
+    assert fvwb.lib().fvwb_dgq_last_error(None)
+
+
+def test_dgq_null_handle_calls_are_argument_errors():
+    L = fvwb.lib()
+    assert L.fvwb_dgq_compute_update(None, None, None, None, None, None) == fvwb.ERR_ARG
+    assert L.fvwb_dgq_advance(None, 1, 1.0) == fvwb.ERR_ARG
+    assert L.fvwb_dgq_compute_error(None, None, None, None, None, None) == fvwb.ERR_ARG
+    assert L.fvwb_dgq_destroy(None) == fvwb.OK
