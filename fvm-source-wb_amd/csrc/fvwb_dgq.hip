@@ -28,9 +28,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -856,9 +858,13 @@ struct fvwb_dgq {
 };
 
 static thread_local std::string g_errq;
+// the message of the allocation-free entry point's last failure (a literal,
+// so reporting it allocates nothing); cleared by every other failure
+static thread_local const char* g_errq_static = nullptr;
 
 static int failq(fvwb_dgq* h, int code, const std::string& msg) {
   (h ? h->err : g_errq) = msg;
+  if (!h) g_errq_static = nullptr;
   return code;
 }
 
@@ -868,7 +874,10 @@ static int failq(fvwb_dgq* h, int code, const std::string& msg) {
     if (e_ != hipSuccess) return failq(h, FVWB_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-extern "C" const char* fvwb_dgq_last_error(fvwb_dgq_handle h) { return h ? h->err.c_str() : g_errq.c_str(); }
+extern "C" const char* fvwb_dgq_last_error(fvwb_dgq_handle h) {
+  if (h) return h->err.c_str();
+  return g_errq_static ? g_errq_static : g_errq.c_str();
+}
 
 extern "C" int fvwb_dgq_params_reference(fvwb_dgq_params* p) {
   if (!p) return failq(nullptr, FVWB_ERR_ARG, "NULL argument");
@@ -1208,5 +1217,69 @@ extern "C" int fvwb_dgq_compute_error(fvwb_dgq_handle h, const double* nodes, co
     l1[v] = r[4 + v];
     l2[v] = r[8 + v];
   }
+  return FVWB_OK;
+}
+
+// ---- compute_error without allocation ------------------------------------------
+// The block partials of dgq_error_kernel live in this file-scope array instead
+// of a handle's allocation.  Its host-side name is only a stand-in: the device
+// address comes from hipGetSymbolAddress, once per device, and only that
+// address is handed to kernels.  Every call on a GPU shares that GPU's array;
+// each block overwrites its own 12 partials and the final pass reads only the
+// blocks of the current call, so nothing needs clearing, but calls that
+// overlap on one GPU race.
+static __device__ __align__(256) double scratch[12 * (FVWB_DGQ_NOALLOC_MAX_CELLS / ERR_THREADS)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::atomic<double*> g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device (nullptr on failure)
+double* scratch_address(int dev) {
+  double* ptr = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (ptr) return ptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  ptr = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+  if (ptr) return ptr;
+  void* a = nullptr;
+  if (hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)) != hipSuccess || !a) return nullptr;
+  ptr = static_cast<double*>(a);
+  g_scratch_ptr[dev].store(ptr, std::memory_order_release);
+  return ptr;
+}
+
+int fail_static(int code, const char* msg) {
+  g_errq_static = msg;
+  return code;
+}
+}  // namespace
+
+extern "C" int fvwb_dgq_compute_error_noalloc(int32_t nx, int32_t ny, int32_t order, const double* nodes,
+                                              const double* u_init, double* out, void* stream) {
+  if (!nodes || !u_init || !out) return fail_static(FVWB_ERR_ARG, "NULL argument");
+  if (nx < 1 || ny < 1) return fail_static(FVWB_ERR_ARG, "nx, ny >= 1 required");
+  if (order < 2 || order > 3) return fail_static(FVWB_ERR_ARG, "order must be 2 or 3");
+  const long long nc = (long long)nx * ny;
+  if (nc > FVWB_DGQ_NOALLOC_MAX_CELLS)
+    return fail_static(FVWB_ERR_ARG, "more cells than FVWB_DGQ_NOALLOC_MAX_CELLS (2^20): the scratch array "
+                                     "holds the partial sums of that many cells");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail_static(FVWB_ERR_HIP, "hipGetDevice failed or device index out of range");
+  double* part = scratch_address(dev);
+  if (!part) return fail_static(FVWB_ERR_HIP, "hipGetSymbolAddress of the scratch array failed");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Geo g{};
+  g.nx = nx;
+  g.ny = ny;
+  const Tab T = make_tab(order);
+  const long long nb = err_blocks(nc);
+  if (order == 2)
+    hipLaunchKernelGGL(dgq_error_kernel<2>, dim3((unsigned)nb), dim3(ERR_THREADS), 0, st, g, T, nodes, u_init, part);
+  else
+    hipLaunchKernelGGL(dgq_error_kernel<3>, dim3((unsigned)nb), dim3(ERR_THREADS), 0, st, g, T, nodes, u_init, part);
+  hipLaunchKernelGGL(dgq_error_final_kernel, dim3(1), dim3(64), 0, st, (const double*)part, (int)nb, out);
+  if (hipGetLastError() != hipSuccess) return fail_static(FVWB_ERR_HIP, "kernel launch failed");
   return FVWB_OK;
 }

@@ -277,6 +277,8 @@ def lib():
         "fvwb_dgq_download": [H, dp],
         "fvwb_dgq_get_time": [H, dp, dp, ctypes.POINTER(ctypes.c_int64)],
         "fvwb_dgq_compute_error": [H, dp, dp, dp, dp, dp],
+        "fvwb_dgq_compute_error_noalloc": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     })
     p1 = ctypes.POINTER(Params1D)
     sig.update({

@@ -531,6 +531,19 @@ int fvwb_dgq_get_time(fvwb_dgq_handle h, double* t, double* dt_last, int64_t* st
  * the resident state are used. */
 int fvwb_dgq_compute_error(fvwb_dgq_handle h, const double* nodes, const double* u_init, double* linf,
                            double* l1, double* l2);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops): compute_error of
+ * device arrays with no handle and no allocation.  nodes and u_init are device
+ * arrays (4, nx, ny, order, order) as above, out a device array of 12
+ * doubles (linf[4], l1[4], l2[4]), equal bit for bit to fvwb_dgq_compute_error;
+ * stream is a hipStream_t (NULL: the default stream) and the call returns
+ * once the work is queued on it.  The block partial sums live in a static
+ * device array of 384 KiB per GPU, so nx*ny is limited to
+ * FVWB_DGQ_NOALLOC_MAX_CELLS (FVWB_ERR_ARG beyond it); calls that overlap on
+ * one GPU share that array and race. */
+#define FVWB_DGQ_NOALLOC_MAX_CELLS (1 << 20)
+int fvwb_dgq_compute_error_noalloc(int32_t nx, int32_t ny, int32_t order, const double* nodes, const double* u_init,
+                                   double* out, void* stream);
 
 /* ---- checkpoint / resume ----------------------------------------------------
  * No counterpart in the reference, whose snapshots are six-digit ASCII and not
