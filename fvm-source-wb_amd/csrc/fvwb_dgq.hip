@@ -87,6 +87,9 @@ static double h_legendre(double x, int n) {
     case 1: l = x; break;
     case 2: l = 0.5 * (3 * x2 - 1); break;
     case 3: l = 0.5 * (5.0 * (x2 * x) - 3.0 * x); break;
+    case 4: l = 0.125 * (35.0 * (x2 * x2) - 30.0 * x2 + 3.0); break;
+    case 5: l = 0.125 * (63.0 * (x2 * x2 * x) - 70.0 * (x2 * x) + 15.0 * x); break;
+    case 6: l = 1.0 / 16.0 * (231.0 * (x2 * x2 * x2) - 315.0 * (x2 * x2) + 105.0 * x2 - 5.0); break;
     default: l = 0.0;
   }
   return std::sqrt(2.0 * (double)n + 1.0) * l;
@@ -100,6 +103,9 @@ static double h_legendre_prime(double x, int n) {
     case 1: l = 1.0; break;
     case 2: l = 3.0 * x; break;
     case 3: l = 0.5 * (15.0 * x2 - 3.0); break;
+    case 4: l = 0.125 * (140.0 * (x2 * x) - 60.0 * x); break;
+    case 5: l = 0.125 * (315.0 * (x2 * x2) - 210.0 * x2 + 15.0); break;
+    case 6: l = 1.0 / 16.0 * (1386.0 * (x2 * x2 * x) - 1260.0 * (x2 * x) + 210.0 * x); break;
     default: l = 0.0;
   }
   return std::sqrt(2.0 * (double)n + 1.0) * l;
@@ -1013,6 +1019,13 @@ static int project(fvwb_dgq* h, const double* src, double* dst, bool to_nodes) {
 
 static size_t state_bytes(const fvwb_dgq* h) { return sizeof(double) * (size_t)h->ns; }
 static size_t coord_bytes(const fvwb_dgq* h) { return sizeof(double) * (size_t)(h->nc * h->p.order * h->p.order); }
+
+extern "C" int fvwb_dgq_gl_quadrature(int32_t n, double* x_quad, double* w_quad) {
+  if (!x_quad || !w_quad) return failq(nullptr, FVWB_ERR_ARG, "NULL argument");
+  if (n < 1 || n > 6) return failq(nullptr, FVWB_ERR_ARG, "n must be 1..6 (legendre is defined up to 6)");
+  h_gl_quadrature(n, x_quad, w_quad);
+  return FVWB_OK;
+}
 
 extern "C" int fvwb_dgq_compute_update(fvwb_dgq_handle h, const double* delta_u, const double* x, const double* y,
                                        const double* u_eq, double* dudt) {

@@ -22,6 +22,9 @@
 !             tend, the whole run in one launch (fvwb1d_advance)
 !   prefix    when given: <prefix>_init.dat, <prefix>_final.dat and <prefix>.bin
 !             (int32 nx, nvar, steps; real(8) t; u0, w_eq, u)
+! With make_movie set, the host loops also write simul/SIMnnn.dat every 200
+! steps (:252-256; the folder must exist), x and p - p_eq per cell as
+! output_file (:69-100) writes them.
 program benchmark_1d_fvwb
   use iso_c_binding
   use parameters
@@ -136,9 +139,12 @@ contains
     character(len=3), intent(in) :: which
     real(kind=8), allocatable :: dudt(:,:), w1(:,:)
     real(kind=8) :: dt, cmax
+    integer :: snap_counter
+    character(len=16) :: filename
     allocate(dudt(nvar, nx), w1(nvar, nx))
     t = 0
     iter = 0
+    snap_counter = 0
     do while (t < tend)
        call compute_max_speed(u, cmax)
        dt = 0.8 * dx / cmax / (2.0 * dble(1) + 1.0)
@@ -158,6 +164,11 @@ contains
        u = 0.5 * u + 0.5 * w1 + 0.5 * dt * dudt
        t = t + dt
        iter = iter + 1
+       if (make_movie .and. modulo(iter, 200) == 0) then
+          snap_counter = snap_counter + 1
+          write(filename, '(a, i3.3)') 'SIM', snap_counter
+          call write_snapshot(x, u, w_eq, 'simul/'//trim(filename)//'.dat')
+       end if
     end do
   end subroutine evolve_stages
 

@@ -67,6 +67,39 @@ module fvwb_iface
   public :: fvwb_rd_compute_max_speed, fvwb_rd_upload, fvwb_rd_advance, fvwb_rd_download
   public :: fvwb_rd_get_time, fvwb_rd_check, fvwb_rd_checkpoint_save, fvwb_rd_checkpoint_load
 
+  ! ---- 2D modal DG in deviation form (the configured P2 scheme, FP32 device path) ----
+  integer(c_int32_t), parameter, public :: FVWB_FLUX_HLLC = 2
+  integer(c_int32_t), parameter, public :: FVWB_DG_LIMITER_NONE = 0, FVWB_DG_LIMITER_TVB = 1, &
+       FVWB_DG_LIMITER_POSITIVITY = 2, FVWB_DG_LIMITER_TVB_POSITIVITY = 3, FVWB_DG_LIMITER_CHARACTERISTIC = 4, &
+       FVWB_DG_LIMITER_CHARACTERISTIC_POSITIVITY = 5
+  integer(c_int32_t), parameter, public :: FVWB_DG_SSPRK3 = 0, FVWB_DG_SSPRK54 = 1
+
+  ! struct fvwb_dg_params: eight int32 then eight doubles (32 + 64 bytes)
+  type, bind(C), public :: fvwb_dg_params
+     integer(c_int32_t) :: nvar, nx, ny, degree, flux, limiter, bc, integrator
+     real(c_double) :: gamma, boxlen_x, boxlen_y, cfl, rho0, p0, g, tvb_m
+  end type fvwb_dg_params
+
+  public :: fvwb_dg_create, fvwb_dg_destroy, fvwb_dg_last_error, fvwb_dg_compute_residual, fvwb_dg_upload
+  public :: fvwb_dg_init_fourier, fvwb_dg_advance, fvwb_dg_download, fvwb_dg_get_time, fvwb_dg_check
+
+  ! ---- the reference's own 2D DG program (2d/benchmark_2d_dg.f90, FP64) ----
+  integer(c_int32_t), parameter, public :: FVWB_DGQ_FLUX_NONE = 0, FVWB_DGQ_FLUX_LLF1 = 1, &
+       FVWB_DGQ_FLUX_HLL2 = 2, FVWB_DGQ_FLUX_HLLC = 3
+  integer(c_int32_t), parameter, public :: FVWB_DGQ_SOLVER_RK4 = 0, FVWB_DGQ_SOLVER_SS4 = 1, &
+       FVWB_DGQ_SOLVER_EQL = 2, FVWB_DGQ_SOLVER_DEB = 3
+
+  ! struct fvwb_dgq_params: ten int32 then five doubles (40 + 40 bytes)
+  type, bind(C), public :: fvwb_dgq_params
+     integer(c_int32_t) :: nx, ny, order, flux_type, source, grad_phi_case, bc, ninit, solver, use_limiter
+     real(c_double) :: gamma, boxlen_x, boxlen_y, cfl, eps
+  end type fvwb_dgq_params
+
+  public :: fvwb_dgq_params_reference, fvwb_dgq_gl_quadrature, fvwb_dgq_create, fvwb_dgq_destroy
+  public :: fvwb_dgq_last_error, fvwb_dgq_compute_update, fvwb_dgq_compute_max_speed, fvwb_dgq_apply_limiter
+  public :: fvwb_dgq_modes_from_nodes, fvwb_dgq_nodes_from_modes, fvwb_dgq_upload, fvwb_dgq_advance
+  public :: fvwb_dgq_download, fvwb_dgq_get_time, fvwb_dgq_compute_error, fvwb_dgq_check
+
   ! struct fvwb1d_params: six int32 then four doubles (24 + 32 bytes)
   type, bind(C), public :: fvwb1d_params
      integer(c_int32_t) :: nvar, nx, bc, recon, integrator, nequilibrium
@@ -514,6 +547,147 @@ module fvwb_iface
        real(c_double), intent(out) :: t, dt_last
        integer(c_int64_t), intent(out) :: steps
      end function fvwb_rd_get_time
+     ! ---- 2D modal DG, configured P2 deviation scheme (include/fvwb.h) ----
+     integer(c_int) function fvwb_dg_create(h, p) bind(C, name='fvwb_dg_create')
+       import :: c_int, c_ptr, fvwb_dg_params
+       type(c_ptr), intent(out) :: h
+       type(fvwb_dg_params), intent(in) :: p
+     end function fvwb_dg_create
+     integer(c_int) function fvwb_dg_destroy(h) bind(C, name='fvwb_dg_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_dg_destroy
+     type(c_ptr) function fvwb_dg_last_error(h) bind(C, name='fvwb_dg_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_dg_last_error
+     ! the deviation form of compute_update (2d/benchmark_2d_dg.f90:1137-1479), modes (4,nx,ny,6)
+     integer(c_int) function fvwb_dg_compute_residual(h, modes, dudt) bind(C, name='fvwb_dg_compute_residual')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: modes(*)
+       real(c_double), intent(out) :: dudt(*)
+     end function fvwb_dg_compute_residual
+     integer(c_int) function fvwb_dg_upload(h, modes) bind(C, name='fvwb_dg_upload')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: modes(*)
+     end function fvwb_dg_upload
+     integer(c_int) function fvwb_dg_init_fourier(h, nterms, a, mk, nk, theta, amp) bind(C, name='fvwb_dg_init_fourier')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nterms
+       real(c_double), intent(in) :: a(*), mk(*), nk(*), theta(*)
+       real(c_double), value :: amp
+     end function fvwb_dg_init_fourier
+     integer(c_int) function fvwb_dg_advance(h, nsteps, t_end) bind(C, name='fvwb_dg_advance')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nsteps
+       real(c_double), value :: t_end
+     end function fvwb_dg_advance
+     integer(c_int) function fvwb_dg_download(h, modes) bind(C, name='fvwb_dg_download')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: modes(*)
+     end function fvwb_dg_download
+     integer(c_int) function fvwb_dg_get_time(h, t, dt_last, steps_done) bind(C, name='fvwb_dg_get_time')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: t, dt_last
+       integer(c_int32_t), intent(out) :: steps_done
+     end function fvwb_dg_get_time
+
+     ! ---- the reference's own 2D DG program (2d/benchmark_2d_dg.f90) ----
+     integer(c_int) function fvwb_dgq_params_reference(p) bind(C, name='fvwb_dgq_params_reference')
+       import :: c_int, fvwb_dgq_params
+       type(fvwb_dgq_params), intent(out) :: p
+     end function fvwb_dgq_params_reference
+     ! 2d/legendre.f90:77-108 gl_quadrature(x_quad, w_quad, n)
+     integer(c_int) function fvwb_dgq_gl_quadrature(n, x_quad, w_quad) bind(C, name='fvwb_dgq_gl_quadrature')
+       import :: c_int, c_int32_t, c_double
+       integer(c_int32_t), value :: n
+       real(c_double), intent(out) :: x_quad(*), w_quad(*)
+     end function fvwb_dgq_gl_quadrature
+     integer(c_int) function fvwb_dgq_create(h, p) bind(C, name='fvwb_dgq_create')
+       import :: c_int, c_ptr, fvwb_dgq_params
+       type(c_ptr), intent(out) :: h
+       type(fvwb_dgq_params), intent(in) :: p
+     end function fvwb_dgq_create
+     integer(c_int) function fvwb_dgq_destroy(h) bind(C, name='fvwb_dgq_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_dgq_destroy
+     type(c_ptr) function fvwb_dgq_last_error(h) bind(C, name='fvwb_dgq_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end function fvwb_dgq_last_error
+     ! :1137-1479 compute_update(delta_u, x, y, u_eq, dudt)
+     integer(c_int) function fvwb_dgq_compute_update(h, delta_u, x, y, u_eq, dudt) &
+          bind(C, name='fvwb_dgq_compute_update')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: delta_u(*), x(*), y(*), u_eq(*)
+       real(c_double), intent(out) :: dudt(*)
+     end function fvwb_dgq_compute_update
+     ! :826-857 compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max)
+     integer(c_int) function fvwb_dgq_compute_max_speed(h, u, cs_max, v_xmax, v_ymax, speed_max) &
+          bind(C, name='fvwb_dgq_compute_max_speed')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: u(*)
+       real(c_double), intent(out) :: cs_max, v_xmax, v_ymax, speed_max
+     end function fvwb_dgq_compute_max_speed
+     ! :1512-1554 apply_limiter(u), in place
+     integer(c_int) function fvwb_dgq_apply_limiter(h, u) bind(C, name='fvwb_dgq_apply_limiter')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(inout) :: u(*)
+     end function fvwb_dgq_apply_limiter
+     ! :497-592 get_modes_from_nodes / get_nodes_from_modes
+     integer(c_int) function fvwb_dgq_modes_from_nodes(h, nodes, modes) bind(C, name='fvwb_dgq_modes_from_nodes')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: nodes(*)
+       real(c_double), intent(out) :: modes(*)
+     end function fvwb_dgq_modes_from_nodes
+     integer(c_int) function fvwb_dgq_nodes_from_modes(h, modes, nodes) bind(C, name='fvwb_dgq_nodes_from_modes')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: modes(*)
+       real(c_double), intent(out) :: nodes(*)
+     end function fvwb_dgq_nodes_from_modes
+     ! :624-775 evolve, resident on the device
+     integer(c_int) function fvwb_dgq_upload(h, delta_u, x, y) bind(C, name='fvwb_dgq_upload')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: delta_u(*), x(*), y(*)
+     end function fvwb_dgq_upload
+     integer(c_int) function fvwb_dgq_advance(h, nsteps, t_end) bind(C, name='fvwb_dgq_advance')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nsteps
+       real(c_double), value :: t_end
+     end function fvwb_dgq_advance
+     integer(c_int) function fvwb_dgq_download(h, delta_u) bind(C, name='fvwb_dgq_download')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: delta_u(*)
+     end function fvwb_dgq_download
+     integer(c_int) function fvwb_dgq_get_time(h, t, dt_last, steps_done) bind(C, name='fvwb_dgq_get_time')
+       import :: c_int, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: t, dt_last
+       integer(c_int64_t), intent(out) :: steps_done
+     end function fvwb_dgq_get_time
+     ! :23-89 compute_error of host nodal arrays
+     integer(c_int) function fvwb_dgq_compute_error(h, nodes, u_init, linf, l1, l2) &
+          bind(C, name='fvwb_dgq_compute_error')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: nodes(*), u_init(*)
+       real(c_double), intent(out) :: linf(4), l1(4), l2(4)
+     end function fvwb_dgq_compute_error
   end interface
 
 contains
@@ -594,5 +768,27 @@ contains
        error stop 1
     end if
   end subroutine fvwb_rd_check
+
+  subroutine fvwb_dg_check(status, h, what)
+    integer(c_int), intent(in) :: status
+    type(c_ptr), intent(in) :: h
+    character(len=*), intent(in) :: what
+    if (status /= FVWB_OK) then
+       write(0, '(a,a,a,i0,a,a)') 'fvwb: ', what, ' failed with status ', status, ': ', &
+            fvwb_message(fvwb_dg_last_error(h))
+       error stop 1
+    end if
+  end subroutine fvwb_dg_check
+
+  subroutine fvwb_dgq_check(status, h, what)
+    integer(c_int), intent(in) :: status
+    type(c_ptr), intent(in) :: h
+    character(len=*), intent(in) :: what
+    if (status /= FVWB_OK) then
+       write(0, '(a,a,a,i0,a,a)') 'fvwb: ', what, ' failed with status ', status, ': ', &
+            fvwb_message(fvwb_dgq_last_error(h))
+       error stop 1
+    end if
+  end subroutine fvwb_dgq_check
 
 end module fvwb_iface

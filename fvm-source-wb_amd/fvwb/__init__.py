@@ -265,6 +265,7 @@ def lib():
     pq = ctypes.POINTER(DGQParams)
     sig.update({
         "fvwb_dgq_params_reference": [pq],
+        "fvwb_dgq_gl_quadrature": [ctypes.c_int32, dp, dp],
         "fvwb_dgq_create": [ctypes.POINTER(H), pq],
         "fvwb_dgq_destroy": [H],
         "fvwb_dgq_compute_update": [H, dp, dp, dp, dp, dp],
@@ -973,6 +974,13 @@ class SolverRD:
         n = ctypes.c_int64()
         _checkrd(lib().fvwb_rd_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), self._h)
         return t.value, dt.value, n.value
+
+
+def dgq_gl_quadrature(n):
+    """gl_quadrature of 2d/legendre.f90:77-108 as the library computes it: (x, w)."""
+    x, w = np.zeros(n), np.zeros(n)
+    _checkdgq(lib().fvwb_dgq_gl_quadrature(int(n), _dp(x), _dp(w)))
+    return x, w
 
 
 def _checkdgq(rc, h=None):

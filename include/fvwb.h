@@ -495,6 +495,10 @@ typedef struct fvwb_dgq* fvwb_dgq_handle;
  * 'llf', source 1, grad_phi_case 2, bc 1, ninit 1, 'RK4', 'ONP', gamma 1.4 and
  * eps 1e-10 as default-real literals, cfl 0.2, unit box). */
 int fvwb_dgq_params_reference(fvwb_dgq_params* p);
+/* 2d/legendre.f90:77-108 gl_quadrature(x_quad, w_quad, n), n = 1..6, on the
+ * host with the reference's expressions (no device work): the nodes and
+ * weights get_coords and every drop-in below use. */
+int fvwb_dgq_gl_quadrature(int32_t n, double* x_quad, double* w_quad);
 int fvwb_dgq_create(fvwb_dgq_handle* out, const fvwb_dgq_params* p);
 int fvwb_dgq_destroy(fvwb_dgq_handle h);
 const char* fvwb_dgq_last_error(fvwb_dgq_handle h);

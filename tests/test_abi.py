@@ -101,3 +101,19 @@ def test_dgq_null_handle_calls_are_argument_errors():
     assert L.fvwb_dgq_advance(None, 1, 1.0) == fvwb.ERR_ARG
     assert L.fvwb_dgq_compute_error(None, None, None, None, None, None) == fvwb.ERR_ARG
     assert L.fvwb_dgq_destroy(None) == fvwb.OK
+
+
+def test_dgq_gl_quadrature_equals_reference_build():
+    """The library's host Gauss-Legendre rule (the nodes get_coords and every
+    reference-DG drop-in use) against gl_quadrature of the reference build,
+    n = 1..6, bit for bit (host code: no GPU)."""
+    import os
+    import numpy as np
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dg2d_gl_quadrature.npz")
+    with np.load(gold) as z:
+        for n in range(1, 7):
+            x, w = fvwb.dgq_gl_quadrature(n)
+            np.testing.assert_array_equal(x, z[f"x{n}"])
+            np.testing.assert_array_equal(w, z[f"w{n}"])
+    with pytest.raises(fvwb.FvwbError):
+        fvwb.dgq_gl_quadrature(7)

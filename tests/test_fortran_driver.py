@@ -69,6 +69,17 @@ def test_fortran_params_struct_matches_c_view():
                                                           c1.nequilibrium]
     d1 = [np.frombuffer(bytes.fromhex(h), dtype=">f8")[0] for h in lines["doubles1d"].split()]
     assert d1 == [c1.gamma, c1.boxlen, c1.cfl, c1.g]
+    assert int(lines["sizeofdg"]) == ctypes.sizeof(fvwb.DGParams) == 96
+    assert int(lines["sizeofdgq"]) == ctypes.sizeof(fvwb.DGQParams) == 80
+    q = fvwb.reference_params_dgq()
+    assert [int(v) for v in lines["intsdgq"].split()] == [q.nx, q.ny, q.order, q.flux_type, q.source,
+                                                          q.grad_phi_case, q.bc, q.ninit, q.solver, q.use_limiter]
+    dq = [np.frombuffer(bytes.fromhex(h), dtype=">f8")[0] for h in lines["doublesdgq"].split()]
+    assert dq == [q.gamma, q.boxlen_x, q.boxlen_y, q.cfl, q.eps]
+    assert int(lines["createdgq_bad"]) == fvwb.ERR_ARG
+    g3 = [np.frombuffer(bytes.fromhex(h), dtype=">f8")[0] for h in lines["glq3"].split()]
+    x3, w3 = fvwb.dgq_gl_quadrature(3)
+    assert g3 == list(x3) + list(w3)
 
 
 def test_driver_binds_dropins_to_library():
@@ -109,6 +120,20 @@ def test_dg1d_driver_binds_dropin_to_library():
     assert "compute_update_exact_delta_" in defined
     for sym in ("fvwb_dg1d_compute_update_exact_delta", "fvwb_dg1d_create", "fvwb_dg1d_upload",
                 "fvwb_dg1d_advance"):
+        assert sym in undefined and sym in fvwb.declared_symbols()
+
+
+def test_dg2d_driver_binds_dropins_to_library():
+    """The reference's 2D DG program binds its external procedures to the GPU
+    drop-ins, which call the fvwb_dgq_* entry points the header declares."""
+    nm = subprocess.run(["nm", exe("benchmark_2d_dg_fvwb")], capture_output=True, text=True, check=True).stdout
+    defined = set(re.findall(r"^[0-9a-f]+ T (\S+)$", nm, re.M))
+    undefined = set(re.findall(r"^\s+U (\S+)$", nm, re.M))
+    assert {"compute_update_", "compute_max_speed_", "apply_limiter_", "get_modes_from_nodes_",
+            "get_nodes_from_modes_", "gl_quadrature_", "compute_error_", "evolve_"} <= defined
+    for sym in ("fvwb_dgq_compute_update", "fvwb_dgq_compute_max_speed", "fvwb_dgq_apply_limiter",
+                "fvwb_dgq_modes_from_nodes", "fvwb_dgq_nodes_from_modes", "fvwb_dgq_gl_quadrature",
+                "fvwb_dgq_upload", "fvwb_dgq_advance", "fvwb_dgq_compute_error"):
         assert sym in undefined and sym in fvwb.declared_symbols()
 
 
@@ -454,3 +479,80 @@ def test_rd_driver_resume_from_checkpoint_equals_one_run(tmp_path):
     print(f"rd checkpoint at step {sa} (t {ta}); resumed to {sb} steps, t {tb}; one run {sc}, t {tc}", flush=True)
     assert sa == 58 and 0 < ta < tb and sb == sc == 162 and tb == tc
     np.testing.assert_array_equal(ub, uc)
+
+
+DG_DRIVERS = {"benchmark_2d_dg_fvwb": "shipped", "benchmark_2d_dg_fvwb_llf1": "llf1",
+              "benchmark_2d_dg_fvwb_hll2": "hll2"}
+
+
+def read_dg_bin(path):
+    with open(path, "rb") as f:
+        nx, ny, mx, steps = (int(v) for v in np.frombuffer(f.read(16), dtype=np.int32))
+        t = float(np.frombuffer(f.read(8), dtype=np.float64)[0])
+        u = np.frombuffer(f.read(8 * 4 * nx * ny * mx * mx), dtype=np.float64).reshape((4, nx, ny, mx, mx), order="F")
+    return steps, t, u
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["stage", "resident"])
+@pytest.mark.parametrize("name", list(DG_DRIVERS))
+def test_dg2d_driver_reproduces_reference_program(tmp_path, name, mode):
+    """The reference's 2D DG program over the GPU drop-ins, run to the fixture's
+    tend with the reference's host loop (stage) or resident on the device:
+    the steps, t and final nodal state equal the reference build's evolve bit
+    for bit; the snapshot files it writes into longrotation2_o1/ (derp.dat,
+    and SIM00001.dat after step 50) equal the reference's byte for byte; the
+    printed compute_error norms equal the reference's to the device
+    reduction's rounding."""
+    with np.load(os.path.join(ROOT, "tests", "golden", f"dgq_{DG_DRIVERS[name]}.npz")) as z:
+        g = {k: z[k] for k in z.files}
+    os.makedirs(tmp_path / "longrotation2_o1")
+    r = run([exe(name), mode, repr(float(g["tend"])), str(tmp_path / "out")], cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps, t, u = read_dg_bin(str(tmp_path / "out.bin"))
+    print(f"{name} {mode}: {steps} steps to t {t!r}; reference {len(g['evolve_dt'])}", flush=True)
+    assert steps == len(g["evolve_dt"]) and t == float(g["tend"])
+    np.testing.assert_array_equal(u, g["evolve_u"])
+    for f in g["files"]:
+        with open(tmp_path / "longrotation2_o1" / str(f)) as fh:
+            assert fh.read() == str(g["file_" + str(f)]), f
+    printed = dict(zip([str(k) for k in g["norm_keys"]], g["norms"]))
+    for key, val in printed.items():
+        m = re.findall(re.escape(key) + r"\s+(\S+)", r.stdout)
+        np.testing.assert_allclose(float(m[-1]), val, rtol=1e-13, atol=1e-300, err_msg=key)
+    if mode == "stage":
+        assert len(re.findall(r"time=", r.stdout)) == steps
+
+
+@pytest.mark.gpu
+def test_1d_driver_snapshot_cadence(tmp_path):
+    """make_movie in the 1D driver: simul/SIMnnn.dat after every 200th step
+    (benchmark_1d.f90:252-256), each the state of that step in output_file's
+    format (x and p - p_eq, '(7(1PE12.5,1X))'); the state comes from the
+    oracle's restatement of the same loop, which the drop-in loop equals bit
+    for bit."""
+    os.makedirs(tmp_path / "simul")
+    tend = 1.5
+    r = run([exe("benchmark_1d_fvwb_64_movie"), "stage", repr(tend), str(tmp_path / "m")], cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps, t, u0, w_eq, u = read_state_1d(str(tmp_path / "m.bin"))
+    files = sorted(os.listdir(tmp_path / "simul"))
+    print(f"{steps} steps, snapshots {files}", flush=True)
+    assert steps >= 400 and files == [f"SIM{k:03d}.dat" for k in range(1, steps // 200 + 1)]
+    p = po.make_params_1d(64)
+    dx = 1.0 / 64
+    x = (np.arange(64) + 0.5) * dx
+    uo, to, n = np.asfortranarray(u0.copy()), 0.0, 0
+    while n < 200:
+        cmax = po.ref1d_max_speed(p, uo)
+        dt = float(np.float32(0.8)) * dx / cmax / (2.0 * 1.0 + 1.0)
+        d = po.ref1d_update(p, uo, w_eq)
+        w1 = uo + dt * d
+        d = po.ref1d_update(p, w1, w_eq)
+        uo = 0.5 * uo + 0.5 * w1 + 0.5 * dt * d
+        n += 1
+    gm = p.gamma
+    pr = (gm - 1.0) * (uo[2] - 0.5 * uo[0] * (uo[1] / uo[0]) ** 2)
+    want = "".join(f"{a:12.5E} {b:12.5E}\n" for a, b in zip(x, pr - w_eq[2]))
+    with open(tmp_path / "simul" / "SIM001.dat") as f:
+        assert f.read() == want
