@@ -89,8 +89,10 @@ struct Blk { Vec4<T> c[C]; };
 
 // One wave computes one strip ("tile") of one instance.  The wave that owns
 // tile 0 of an instance also does that instance's step bookkeeping: the first
-// stage clears the next step's speed slots, the last stage advances t.
-template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL>
+// stage clears the next step's speed slots, the last stage advances t.  LAST
+// (the step's last stage, a.last) is a template parameter so that the other
+// stages carry no wave-speed code at all.
+template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL, bool LAST>
 __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile, const int lane, const int slot) {
   // each lane owns CPL adjacent columns (8 or 16 bytes per variable plane);
   // lanes 0 and 63 are halo, so a wave produces 62*CPL columns
@@ -113,7 +115,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
       // a step past t_end changes nothing and carries the wave-speed maximum to
       // the next parity, so a later call with a larger t_end resumes from it
       if (a.first) cw->cmax[q][lane].v = active ? 0ull : c->cmax[p][lane].v;
-      if (a.last && lane == 0) {
+      if (LAST && lane == 0) {
         cw->t[q] = active ? t + dt : t;
         if (active) { cw->steps_done += 1; cw->dt_last = dt; }
       }
@@ -399,28 +401,36 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
         }
       }
       Gp = Gn;
-      if (own && j >= y0 && j < y1) {
-        T* d = a.out + (long long)j * pitch;
+      // The stores are issued on every row and lane, without a branch: a lane
+      // or row that owns nothing gets an offset past the row's extent, which
+      // the buffer range check discards.  A branch around them would make the
+      // wait for the next row's loads (issued before these stores) also wait
+      // for the stores, since vector loads and stores share one in-order
+      // counter and the two paths would be merged pessimistically.
+      const bool row_ok = j >= y0 && j < y1;
+      {
+        T* d = a.out + (long long)jt * pitch;
+        const unsigned soff = (own && row_ok) ? vboff : 0x80000000u;
         Lanes<T, CPL> o0, o1, o2, o3;
 #pragma unroll
         for (int k = 0; k < CPL; ++k) {
           o0.v[k] = O.c[k].r; o1.v[k] = O.c[k].mx; o2.v[k] = O.c[k].my; o3.v[k] = O.c[k].e;
         }
         // a block may run past nx into the row padding, which is never read as data
-        stv<T, CPL>(d, vboff, rowb, o0);
-        stv<T, CPL>(d + plane, vboff, rowb, o1);
-        stv<T, CPL>(d + 2 * plane, vboff, rowb, o2);
-        stv<T, CPL>(d + 3 * plane, vboff, rowb, o3);
-        if (MODE != MODE_RESIDUAL && a.last) {
+        stv<T, CPL>(d, soff, rowb, o0);
+        stv<T, CPL>(d + plane, soff, rowb, o1);
+        stv<T, CPL>(d + 2 * plane, soff, rowb, o2);
+        stv<T, CPL>(d + 3 * plane, soff, rowb, o3);
+      }
+      if (MODE != MODE_RESIDUAL && LAST && row_ok) {
 #pragma unroll
-          for (int k = 0; k < CPL; ++k) {
-            if (x0 + k < nx) {
-              bool ok;
-              T rct = T(rxc[k] * eyc);
-              T sp = cell_speed(P, rct, rct * eor, O.c[k], ok);
-              smax = tmax(smax, sp);
-              bad |= !ok;
-            }
+        for (int k = 0; k < CPL; ++k) {
+          if (own && x0 + k < nx) {
+            bool ok;
+            T rct = T(rxc[k] * eyc);
+            T sp = cell_speed(P, rct, rct * eor, O.c[k], ok);
+            smax = tmax(smax, sp);
+            bad |= !ok;
           }
         }
       }
@@ -444,7 +454,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
     else sweep(std::integral_constant<int, 1>{}, std::false_type{});
   }
 
-  if (MODE != MODE_RESIDUAL && a.last) {
+  if (MODE != MODE_RESIDUAL && LAST) {
     if (bad) atomicOr(&a.ctrl->err, 1u);
     double sm = (double)smax;
 #pragma unroll
@@ -453,7 +463,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
   }
 }
 
-template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL>
+template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL, bool LAST>
 __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
   const int lane = threadIdx.x & 63;
   // wave index made explicitly uniform so that strip, row and row pointers
@@ -461,13 +471,13 @@ __global__ __launch_bounds__(256) void fv2d_stage_kernel(StageArgs<T> a) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tile = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
   if (tile >= a.ntiles) return;
-  stage_tile<T, RECON, FLUX, BC, FORM, MODE, CPL>(a, tile, lane, blockIdx.x & 7);
+  stage_tile<T, RECON, FLUX, BC, FORM, MODE, CPL, LAST>(a, tile, lane, blockIdx.x & 7);
 }
 
 // Persistent batched stage: every wave walks the global tile list of all
 // instances (static stride, each XCD taking contiguous runs) and finds each
 // tile's instance by binary search over the tile prefix sums.
-template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL>
+template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL, bool LAST>
 __global__ __launch_bounds__(256) void fv2d_batch_stage_kernel(BatchArgs<T> b) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -496,7 +506,7 @@ __global__ __launch_bounds__(256) void fv2d_batch_stage_kernel(BatchArgs<T> b) {
     a.gamma = d.gamma; a.g = d.g; a.p_over_rho = d.p_over_rho; a.inv_gm1 = d.inv_gm1;
     a.odx = d.odx; a.ody = d.ody;
     a.alpha = b.alpha; a.beta = b.beta;
-    stage_tile<T, RECON, FLUX, BC, FORM, MODE, CPL>(a, t - b.tile_start[lo], lane, blockIdx.x & 7);
+    stage_tile<T, RECON, FLUX, BC, FORM, MODE, CPL, LAST>(a, t - b.tile_start[lo], lane, blockIdx.x & 7);
   }
 }
 
@@ -612,7 +622,10 @@ __global__ void init_gaussian_kernel(T* __restrict__ out, double* __restrict__ w
 template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL>
 static hipError_t launch_stage_t(const StageArgs<T>& a, hipStream_t s) {
   int blocks = (a.ntiles + 3) / 4;
-  hipLaunchKernelGGL((fv2d_stage_kernel<T, RECON, FLUX, BC, FORM, MODE, CPL>), dim3(blocks), dim3(256), 0, s, a);
+  if (MODE == MODE_STAGE && a.last)
+    hipLaunchKernelGGL((fv2d_stage_kernel<T, RECON, FLUX, BC, FORM, MODE, CPL, true>), dim3(blocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((fv2d_stage_kernel<T, RECON, FLUX, BC, FORM, MODE, CPL, false>), dim3(blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -628,7 +641,12 @@ static hipError_t dispatch_scheme(const StageArgs<T>& a, int recon, int flux, in
 
 template <typename T, int RECON, int FLUX, int BC, int FORM, int MODE, int CPL>
 static hipError_t launch_batch_t(const BatchArgs<T>& b, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL((fv2d_batch_stage_kernel<T, RECON, FLUX, BC, FORM, MODE, CPL>), dim3(blocks), dim3(256), 0, s, b);
+  if (MODE == MODE_STAGE && b.last)
+    hipLaunchKernelGGL((fv2d_batch_stage_kernel<T, RECON, FLUX, BC, FORM, MODE, CPL, true>), dim3(blocks), dim3(256), 0,
+                       s, b);
+  else
+    hipLaunchKernelGGL((fv2d_batch_stage_kernel<T, RECON, FLUX, BC, FORM, MODE, CPL, false>), dim3(blocks), dim3(256), 0,
+                       s, b);
   return hipGetLastError();
 }
 
