@@ -12,9 +12,9 @@
 // One residual evaluation (compute_update, :1137-1479) is three launches:
 //   dgq_trace_kernel  one thread per cell: the traces at xi = -1, +1 and
 //                     eta = -1, +1 (:1253-1314) and their fluxes
-//                     (compute_flux_int :932-951), written face-major so that
+//                     (compute_flux_int :946-965), written face-major so that
 //                     the face pass reads them coalesced;
-//   dgq_face_kernel   one thread per face point: compute_num_flux (:977-1006)
+//   dgq_face_kernel   one thread per face point: compute_num_flux (:991-1006)
 //                     between the traces of the two cells, with the
 //                     reference's get_boundary_conditions(., 2) indexing;
 //   dgq_cell_kernel   one thread per cell: node values, volume integrals,
@@ -23,7 +23,7 @@
 //                     loop, the Runge-Kutta stage combination of evolve in
 //                     the reference's expression order.
 // dgq_limit_kernel is compute_positivity (2d/limiters.f90:478-654) per cell;
-// dgq_speed_kernel is compute_max_speed (:826-857) with its bookkeeping and
+// dgq_speed_kernel is compute_max_speed (:826-870) with its bookkeeping and
 // evolve's dt (:671) in one workgroup.
 #include <hip/hip_runtime.h>
 
@@ -154,7 +154,7 @@ static Tab make_tab(int o) {
 __device__ __forceinline__ double fmax_r(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double fmin_r(double a, double b) { return a < b ? a : b; }
 
-// compute_primitive (:877-888): density floored at the default-real 10e-10
+// compute_primitive (:891-902): density floored at the default-real 10e-10
 __device__ __forceinline__ void prim(double gamma, const double u[4], double w[4]) {
   const double fl = (double)10e-10f;
   w[0] = u[0] > fl ? u[0] : fl;
@@ -163,7 +163,7 @@ __device__ __forceinline__ void prim(double gamma, const double u[4], double w[4
   w[3] = (gamma - 1.0) * (u[3] - 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
 }
 
-// compute_flux (:906-930): f1 along x, f2 along y
+// compute_flux (:919-944): f1 along x, f2 along y
 __device__ __forceinline__ void flux_full(double gamma, const double u[4], double f1[4], double f2[4]) {
   double w[4];
   prim(gamma, u, w);
@@ -177,7 +177,7 @@ __device__ __forceinline__ void flux_full(double gamma, const double u[4], doubl
   f1[3] = w[1] * u[3] + w[1] * w[3];
 }
 
-// compute_flux_int (:932-951) in one direction: dir 0 along x, 1 along y
+// compute_flux_int (:946-965) in one direction: dir 0 along x, 1 along y
 __device__ __forceinline__ void flux_int(double gamma, const double u[4], int dir, double f[4]) {
   double w[4];
   prim(gamma, u, w);
@@ -194,7 +194,7 @@ __device__ __forceinline__ void flux_int(double gamma, const double u[4], int di
   }
 }
 
-// compute_speed (:859-875)
+// compute_speed (:873-889)
 __device__ __forceinline__ void speed(double gamma, const double u[4], double& cs, double& vx, double& vy,
                                       double& sp) {
   double w[4];
@@ -205,14 +205,14 @@ __device__ __forceinline__ void speed(double gamma, const double u[4], double& c
   sp = sqrt(w[1] * w[1] + w[2] * w[2]) + cs;
 }
 
-// compute_num_flux (:977-1006); flag 1 along x, 2 along y.  A flux_type of no
+// compute_num_flux (:991-1006); flag 1 along x, 2 along y.  A flux_type of no
 // branch (the shipped 'llf') and an HLLC state of no branch leave out at zero,
 // the value the reference's F and G start from (:1317-1318).
 __device__ void num_flux(const Geo& g, const double ul[4], const double ur[4], const double fl[4],
                          const double fr[4], int flag, double out[4]) {
   out[0] = out[1] = out[2] = out[3] = 0.0;
   const double gamma = g.gamma;
-  if (g.flux == FVWB_DGQ_FLUX_LLF1) {            // compute_llflux (:953-975)
+  if (g.flux == FVWB_DGQ_FLUX_LLF1) {            // compute_llflux (:968-988)
     double csl, vxl, vyl, spl, csr, vxr, vyr, spr;
     speed(gamma, ul, csl, vxl, vyl, spl);
     speed(gamma, ur, csr, vxr, vyr, spr);
@@ -654,8 +654,8 @@ __global__ __launch_bounds__(256) void dgq_project_kernel(Geo g, Tab T, const do
       }
 }
 
-// compute_max_speed (:826-857) over the cells in the reference's order
-// k = i ny + j (x outer) and, with a clock, evolve's dt (:665-671); one
+// compute_max_speed (:826-870) over the cells in the reference's order
+// k = i ny + j (x outer) and, with a clock, evolve's dt (:668-671); one
 // workgroup.  The maximum speed S, the last k whose speed equals S (the last
 // update of the reference's running ">=" test) and the smallest sound speed
 // from that cell on (cs_max restarts there and only decreases) are each exact
@@ -771,7 +771,7 @@ __global__ __launch_bounds__(SPEED_THREADS) void dgq_speed_kernel(Geo g, const d
   }
 }
 
-// t = t + dt (:745), after the step
+// t = t + dt (:749), after the step
 __global__ void dgq_tick_kernel(Clock* __restrict__ ck) {
   if (threadIdx.x != 0 || !ck->active) return;
   ck->t = ck->t + ck->dt;
@@ -940,7 +940,7 @@ extern "C" int fvwb_dgq_create(fvwb_dgq_handle* out, const fvwb_dgq_params* p) {
   h->T = make_tab(o);
   const double dx = p->boxlen_x / (double)p->nx;
   const int gll = h->T.gll;
-  // evolve's gll_w_1 (:660-664) in default REAL: 1e-10 vanishes against gll(gll-1)
+  // evolve's gll_w_1 (:650-655) in default REAL: 1e-10 vanishes against gll(gll-1)
   const double gll_w_1 = o == 1 ? 1. : 1. / (double)((float)(gll * (gll - 1)) + 1e-10f);
   const double c1 = 1. / (double)(2 * 4 + 1), c2 = gll_w_1 / 2.;
   h->g = Geo{p->nx, p->ny, p->flux_type, p->source, p->grad_phi_case, p->bc, p->ninit, p->use_limiter,
@@ -1105,7 +1105,7 @@ static int read_clock(fvwb_dgq* h, Clock* c) {
   return FVWB_OK;
 }
 
-// the 'RK4' (:683-708) and 'SS4' (:709-731) coefficients, default-real literals
+// the 'RK4' (:683-710) and 'SS4' (:711-735) coefficients, default-real literals
 static const float kRK4[16] = {0.391752226571890f, 0.444370493651235f, 0.555629506348765f, 0.368410593050371f,
                                0.620101851488403f, 0.379898148511597f, 0.251891774271694f, 0.178079954393132f,
                                0.821920045606868f, 0.544974750228521f, 0.00683325884039f,  0.51723167208978f,
@@ -1126,7 +1126,7 @@ static Epi epi(std::initializer_list<std::pair<double, const double*>> terms, do
   return e;
 }
 
-// One step of evolve's loop (:665-748): dt, the configured solver with
+// One step of evolve's loop (:666-769): dt, the configured solver with
 // apply_limiter after its stages, t = t + dt.
 static int step(fvwb_dgq* h, double t_end) {
   Clock* ck = h->clk;
@@ -1140,7 +1140,7 @@ static int step(fvwb_dgq* h, double t_end) {
   if (h->p.solver == FVWB_DGQ_SOLVER_EQL) {        // :672-681
     if ((rc = residual(h, du, x, y, epi({{1.0, du}}, 1.0), w1, ck)) || (rc = limit(h, w1, ck))) return rc;
     if ((rc = residual(h, w1, x, y, epi({{0.5, du}, {0.5, w1}}, 0.5), du, ck)) || (rc = limit(h, du, ck))) return rc;
-  } else if (h->p.solver == FVWB_DGQ_SOLVER_DEB) { // :733-742: w1 = du + dt L(du), limited, becomes du
+  } else if (h->p.solver == FVWB_DGQ_SOLVER_DEB) { // :737-747: w1 = du + dt L(du), limited, becomes du
     if ((rc = residual(h, du, x, y, epi({{1.0, du}}, 1.0), du, ck)) || (rc = limit(h, du, ck))) return rc;
   } else {                                          // 'RK4' / 'SS4'
     const float* cf = h->p.solver == FVWB_DGQ_SOLVER_SS4 ? kSS4 : kRK4;

@@ -56,7 +56,7 @@ program benchmark_2d_dg_fvwb
   close(unit)
 end program benchmark_2d_dg_fvwb
 
-! get_coords (:91-119): the Gauss points of each cell
+! get_coords (:93-120): the Gauss points of each cell
 subroutine get_coords(x, y, size_x, size_y, order_x, order_y)
   use parameters_dg_2d
   implicit none
@@ -81,7 +81,7 @@ subroutine get_coords(x, y, size_x, size_y, order_x, order_y)
   end do
 end subroutine get_coords
 
-! compute_primitive (:877-888) and compute_conservative (:890-904)
+! compute_primitive (:891-902) and compute_conservative (:905-917)
 subroutine compute_primitive(u, w, size_x, size_y, order_x, order_y)
   use parameters_dg_2d
   implicit none

@@ -85,7 +85,7 @@ subroutine compute_update(delta_u, x, y, u_eq, dudt)
   call fvwb_dgq_check(fvwb_dgq_compute_update(h, delta_u, x, y, u_eq, dudt), h, 'fvwb_dgq_compute_update')
 end subroutine compute_update
 
-! :826-857
+! :826-870
 subroutine compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max)
   use iso_c_binding
   use parameters_dg_2d
@@ -100,7 +100,7 @@ subroutine compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max)
        'fvwb_dgq_compute_max_speed')
 end subroutine compute_max_speed
 
-! :1512-1554 (limiter_type 'ONP': compute_positivity of 2d/limiters.f90:478-654)
+! :1516-1555 (limiter_type 'ONP': compute_positivity of 2d/limiters.f90:478-654)
 subroutine apply_limiter(u)
   use iso_c_binding
   use parameters_dg_2d

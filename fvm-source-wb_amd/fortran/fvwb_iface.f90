@@ -630,7 +630,7 @@ module fvwb_iface
        real(c_double), intent(in) :: delta_u(*), x(*), y(*), u_eq(*)
        real(c_double), intent(out) :: dudt(*)
      end function fvwb_dgq_compute_update
-     ! :826-857 compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max)
+     ! :826-870 compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max)
      integer(c_int) function fvwb_dgq_compute_max_speed(h, u, cs_max, v_xmax, v_ymax, speed_max) &
           bind(C, name='fvwb_dgq_compute_max_speed')
        import :: c_int, c_ptr, c_double
@@ -638,7 +638,7 @@ module fvwb_iface
        real(c_double), intent(in) :: u(*)
        real(c_double), intent(out) :: cs_max, v_xmax, v_ymax, speed_max
      end function fvwb_dgq_compute_max_speed
-     ! :1512-1554 apply_limiter(u), in place
+     ! :1516-1555 apply_limiter(u), in place
      integer(c_int) function fvwb_dgq_apply_limiter(h, u) bind(C, name='fvwb_dgq_apply_limiter')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: h

@@ -1037,14 +1037,14 @@ class SolverDGQ:
         return out
 
     def compute_max_speed(self, u):
-        """compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max) (:826-857) of u(4,nx,ny)."""
+        """compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max) (:826-870) of u(4,nx,ny)."""
         a = self._arr(u, (4, self.params.nx, self.params.ny))
         r = [ctypes.c_double() for _ in range(4)]
         _checkdgq(lib().fvwb_dgq_compute_max_speed(self._h, _dp(a), *[ctypes.byref(v) for v in r]), self._h)
         return tuple(v.value for v in r)
 
     def apply_limiter(self, u):
-        """apply_limiter(u) (:1512-1554), 'ONP': returns the limited modes."""
+        """apply_limiter(u) (:1516-1555), 'ONP': returns the limited modes."""
         a = self._arr(u).copy(order="F")
         _checkdgq(lib().fvwb_dgq_apply_limiter(self._h, _dp(a)), self._h)
         return a
@@ -1064,7 +1064,7 @@ class SolverDGQ:
                                         _dp(self._arr(y, self.cshape))), self._h)
 
     def advance(self, nsteps=(1 << 31) - 1, t_end=math.inf):
-        """evolve's loop (:665-748) while t < t_end, at most nsteps steps."""
+        """evolve's loop (:666-769) while t < t_end, at most nsteps steps."""
         _checkdgq(lib().fvwb_dgq_advance(self._h, int(nsteps), float(t_end)), self._h)
 
     def download(self):

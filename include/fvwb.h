@@ -454,7 +454,7 @@ int fvwb_dg1d_get_time(fvwb_dg1d_handle h, double* t, double* dt_last, int32_t* 
  * FP64, the reference's layouts: modes and nodal values
  * delta_u(1:4,1:nx,1:ny,1:order,1:order) (element (v,i,j,a,b) at
  * v + 4*(i + nx*(j + ny*(a + order*b))), mode / node indices slowest), node
- * coordinates x, y(1:nx,1:ny,1:order,1:order) as get_coords (:91-119) builds
+ * coordinates x, y(1:nx,1:ny,1:order,1:order) as get_coords (:93-120) builds
  * them.  The switches are the compile-time constants of 2d/parameters_dg_2d.f90.
  * The reference's residual indexes the x faces with the y bounds (:1338-1339)
  * and sizes get_source's work array nx x nx (:1563), so it is defined for
@@ -467,10 +467,10 @@ int fvwb_dg1d_get_time(fvwb_dg1d_handle h, double* t, double* dt_last, int32_t* 
 #define FVWB_DGQ_FLUX_LLF1 1   /* 'llf1': compute_llflux, signed normal velocity + c */
 #define FVWB_DGQ_FLUX_HLL2 2   /* 'hll2': compute_hllflux (:1008-1026)                */
 #define FVWB_DGQ_FLUX_HLLC 3   /* 'hllc': compute_hllcflux (:1030-1134) as written   */
-#define FVWB_DGQ_SOLVER_RK4 0  /* 'RK4' (shipped): five-stage SSP, :683-708          */
-#define FVWB_DGQ_SOLVER_SS4 1  /* 'SS4': the same scheme, other literals, :709-731  */
+#define FVWB_DGQ_SOLVER_RK4 0  /* 'RK4' (shipped): five-stage SSP, :683-710          */
+#define FVWB_DGQ_SOLVER_SS4 1  /* 'SS4': the same scheme, other literals, :711-735  */
 #define FVWB_DGQ_SOLVER_EQL 2  /* 'EQL': Heun, :672-681                              */
-#define FVWB_DGQ_SOLVER_DEB 3  /* 'DEB': one forward Euler stage, :733-742            */
+#define FVWB_DGQ_SOLVER_DEB 3  /* 'DEB': one forward Euler stage, :737-747            */
 
 typedef struct fvwb_dgq_params {
   int32_t nx, ny;          /* cells, nx == ny                                        */
@@ -506,13 +506,13 @@ const char* fvwb_dgq_last_error(fvwb_dgq_handle h);
  * u_eq is unused by the reference and may be NULL. */
 int fvwb_dgq_compute_update(fvwb_dgq_handle h, const double* delta_u, const double* x, const double* y,
                             const double* u_eq, double* dudt);
-/* :826-857 compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max) on
+/* :826-870 compute_max_speed(u, cs_max, v_xmax, v_ymax, speed_max) on
  * u(1:4,1:nx,1:ny), with the reference's bookkeeping: v_xmax, v_ymax from the
  * last cell (x outer, y inner) whose speed reaches the running maximum, cs_max
  * the smallest sound speed from that cell on. */
 int fvwb_dgq_compute_max_speed(fvwb_dgq_handle h, const double* u, double* cs_max, double* v_xmax,
                                double* v_ymax, double* speed_max);
-/* :1512-1554 apply_limiter(u) in place (compute_positivity when use_limiter) */
+/* :1516-1555 apply_limiter(u) in place (compute_positivity when use_limiter) */
 int fvwb_dgq_apply_limiter(fvwb_dgq_handle h, double* u);
 /* :497-592 get_modes_from_nodes(nodes, u) and get_nodes_from_modes(u, nodes) */
 int fvwb_dgq_modes_from_nodes(fvwb_dgq_handle h, const double* nodes, double* modes);

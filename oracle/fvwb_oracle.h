@@ -200,13 +200,13 @@ int    orc_refdg_valid(const fvwb_dgq_params* p);
 /* compute_update (:1137-1479): modes, x, y -> dudt */
 void   orc_refdg_update(const fvwb_dgq_params* p, const double* delta_u, const double* x, const double* y,
                         double* dudt);
-/* compute_max_speed (:826-857) on u(1:4,1:nx,1:ny): out = (cs_max, v_xmax, v_ymax, speed_max) */
+/* compute_max_speed (:826-870) on u(1:4,1:nx,1:ny): out = (cs_max, v_xmax, v_ymax, speed_max) */
 void   orc_refdg_max_speed(const fvwb_dgq_params* p, const double* u, double out[4]);
-/* evolve's time step (:665-671) from the means of the modes; ms = compute_max_speed's outputs */
+/* evolve's time step (:668-671) from the means of the modes; ms = compute_max_speed's outputs */
 double orc_refdg_dt(const fvwb_dgq_params* p, const double* modes, double t, double t_end, double ms[4]);
-/* apply_limiter (:1512-1554), 'ONP' */
+/* apply_limiter (:1516-1555), 'ONP' */
 void   orc_refdg_limiter(const fvwb_dgq_params* p, double* u);
-/* evolve's loop (:672-759) from a limited state; returns the steps taken */
+/* evolve's loop (:666-769) from a limited state; returns the steps taken */
 int    orc_refdg_evolve(const fvwb_dgq_params* p, double* delta_u, const double* x, const double* y,
                         double t_end, int max_steps, double* t_io, double* dts);
 /* compute_error (:23-89): out = (linf[4], l1[4], l2[4]) */

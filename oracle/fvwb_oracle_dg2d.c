@@ -11,7 +11,7 @@
  * volume and face quadrature with the reference's assembly
  * (2d/benchmark_2d_dg.f90:1455-1462), the configured FV face flux, walls, the
  * TVB limiter of 2d/limiters.f90:30-54 on the linear modes, SSP-RK3 or the
- * reference's SSP(5,4) (2d/benchmark_2d_dg.f90:683-708).  Pinned by
+ * reference's SSP(5,4) (2d/benchmark_2d_dg.f90:683-710).  Pinned by
  * invariants in tests/ (exact rest, degree 0 == first-order FV, convergence
  * orders, mass identity).  Built with -ffp-contract=off.  Not linked by the
  * product.

@@ -10,8 +10,8 @@
  *                                     balanced (u_eq unused), flux dispatch
  *                                     :991-1006, sources :1558-1644, assembly
  *                                     :1448-1466, special_boundary_conditions
- *                                     :1480-1510
- *   compute_max_speed     :826-875    with its cs_max / v_xmax bookkeeping
+ *                                     :1481-1514
+ *   compute_max_speed     :826-889    with its cs_max / v_xmax bookkeeping
  *   evolve                :624-775    dt rule :671, solvers 'RK4', 'SS4',
  *                                     'EQL', 'DEB', apply_limiter 'ONP'
  *   compute_error         :23-89      L_inf, Gauss-weighted L1 and L2
@@ -34,7 +34,7 @@
 /* the reference's default-real literal 10e-10 (compute_primitive's density floor) */
 static const double RHO_FLOOR = (double)10e-10f;
 
-/* compute_primitive :877-888 for one point */
+/* compute_primitive :891-902 for one point */
 static void prim(double gamma, const double u[4], double w[4])
 {
   w[0] = u[0] > RHO_FLOOR ? u[0] : RHO_FLOOR;
@@ -43,7 +43,7 @@ static void prim(double gamma, const double u[4], double w[4])
   w[3] = (gamma - 1.0) * (u[3] - 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
 }
 
-/* compute_flux :906-930 for one point: f1 along x, f2 along y */
+/* compute_flux :919-944 for one point: f1 along x, f2 along y */
 static void flux_full(double gamma, const double u[4], double f1[4], double f2[4])
 {
   double w[4];
@@ -58,7 +58,7 @@ static void flux_full(double gamma, const double u[4], double f1[4], double f2[4
   f1[3] = w[1] * u[3] + w[1] * w[3];
 }
 
-/* compute_flux_int :932-951: the face-trace flux, mass flux from u(1) */
+/* compute_flux_int :946-965: the face-trace flux, mass flux from u(1) */
 static void flux_int(double gamma, const double u[4], double f1[4], double f2[4])
 {
   double w[4];
@@ -73,7 +73,7 @@ static void flux_int(double gamma, const double u[4], double f1[4], double f2[4]
   f2[3] = w[2] * u[3] + w[2] * w[3];
 }
 
-/* compute_speed :859-875 */
+/* compute_speed :873-889 */
 static void speed(double gamma, const double u[4], double* cs, double* vx, double* vy, double* sp)
 {
   double w[4];
@@ -87,7 +87,7 @@ static void speed(double gamma, const double u[4], double* cs, double* vx, doubl
 static double fmax_f(double a, double b) { return a > b ? a : b; }
 static double fmin_f(double a, double b) { return a < b ? a : b; }
 
-/* compute_llflux :953-975 */
+/* compute_llflux :968-988 */
 static void llflux(double gamma, const double ul[4], const double ur[4], const double fl[4], const double fr[4],
                    double out[4], int flag)
 {
@@ -164,7 +164,7 @@ static void hllcflux(double gamma, const double ul[4], const double ur[4], doubl
   }
 }
 
-/* compute_num_flux :977-1006: 'llf1', 'hll2', 'hllc'; any other flux_type
+/* compute_num_flux :991-1006: 'llf1', 'hll2', 'hllc'; any other flux_type
  * (the shipped 'llf') matches no branch and leaves the face flux at zero */
 static void num_flux(const fvwb_dgq_params* p, const double ul[4], const double ur[4], const double fl[4],
                      const double fr[4], double out[4], int flag)
@@ -385,7 +385,7 @@ void orc_refdg_update(const fvwb_dgq_params* p, const double* du, const double* 
                                          odx * (e1[v] - e2[v]) - odx * (e3[v] - e4[v])) / 2. +
                                         sv[v] / 4.;
         }
-  if (p->ninit == 12) {                /* special_boundary_conditions :1480-1510 */
+  if (p->ninit == 12) {                /* special_boundary_conditions :1481-1514 */
     const double xc = p->boxlen_x / 2., yc = p->boxlen_y / 2.;
     for (size_t ic = 0; ic < nx; ++ic)
       for (size_t jc = 0; jc < ny; ++jc)
@@ -404,7 +404,7 @@ void orc_refdg_update(const fvwb_dgq_params* p, const double* du, const double* 
   free(F); free(G); free(s);
 }
 
-/* compute_max_speed :826-857 on u(1:4,1:nx,1:ny) (evolve passes the first
+/* compute_max_speed :826-870 on u(1:4,1:nx,1:ny) (evolve passes the first
  * mode, the cell means): the last cell, in the reference's x-outer order,
  * whose speed reaches the running maximum gives v_xmax and v_ymax; cs_max
  * restarts at that cell's sound speed and then keeps the minimum of every
@@ -427,7 +427,7 @@ void orc_refdg_max_speed(const fvwb_dgq_params* p, const double* u, double out[4
   out[0] = csmax; out[1] = vxm; out[2] = vym; out[3] = smax;
 }
 
-/* evolve :665-671: gll_w_1 (in default REAL: 1e-10 vanishes against
+/* evolve :668-671: gll_w_1 (in default REAL: 1e-10 vanishes against
  * gll(gll-1)) and the time step from compute_max_speed of the means; dy is
  * never used (both denominators take dx) */
 double orc_refdg_dt(const fvwb_dgq_params* p, const double* modes, double t, double t_end, double ms[4])
@@ -450,7 +450,7 @@ void orc_refdg_limiter(const fvwb_dgq_params* p, double* u)
   if (p->use_limiter) orc_ref_dg2d_positivity(p->nx, p->ny, p->order, p->gamma, p->eps, u);
 }
 
-/* evolve :672-759 from a limited state: at most max_steps steps while
+/* evolve :666-769 from a limited state: at most max_steps steps while
  * t < t_end; dt of every step into dts (if not NULL); returns the steps taken */
 int orc_refdg_evolve(const fvwb_dgq_params* p, double* du, const double* x, const double* y, double t_end,
                      int max_steps, double* t_io, double* dts)
@@ -458,7 +458,7 @@ int orc_refdg_evolve(const fvwb_dgq_params* p, double* du, const double* x, cons
   const size_t n = 4u * (size_t)p->nx * p->ny * p->order * p->order;
   double *L = malloc(sizeof(double) * n), *w1 = malloc(sizeof(double) * n), *w2 = malloc(sizeof(double) * n),
          *w3 = malloc(sizeof(double) * n), *w4 = malloc(sizeof(double) * n), *w5 = malloc(sizeof(double) * n);
-  /* the 'RK4' (:683-708) and 'SS4' (:709-731) literals, default REAL */
+  /* the 'RK4' (:683-710) and 'SS4' (:711-735) literals, default REAL */
   static const float rk4[16] = {0.391752226571890f, 0.444370493651235f, 0.555629506348765f, 0.368410593050371f,
                                 0.620101851488403f, 0.379898148511597f, 0.251891774271694f, 0.178079954393132f,
                                 0.821920045606868f, 0.544974750228521f, 0.00683325884039f, 0.51723167208978f,

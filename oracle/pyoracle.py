@@ -148,7 +148,7 @@ def make_dgq_params(n=8, *, order=2, flux_type=DGQ_FLUX_NONE, source=1, grad_phi
 
 
 def dgq_coords(p: DGQParams):
-    """get_coords (2d/benchmark_2d_dg.f90:91-119): x(i,j,a,b) = (i-0.5) dx +
+    """get_coords (2d/benchmark_2d_dg.f90:93-120): x(i,j,a,b) = (i-0.5) dx +
     dx/2 x_quad(a), with (i-0.5) in default REAL as the reference writes it."""
     nx, ny, o = p.nx, p.ny, p.order
     xq, _ = dg_gl_quadrature(o)
@@ -174,7 +174,7 @@ def refdg_update(p: DGQParams, delta_u, x, y):
 
 
 def refdg_max_speed(p: DGQParams, u):
-    """compute_max_speed (:826-857) on u(1:4,1:nx,1:ny): (cs_max, v_xmax, v_ymax, speed_max)."""
+    """compute_max_speed (:826-870) on u(1:4,1:nx,1:ny): (cs_max, v_xmax, v_ymax, speed_max)."""
     out = np.zeros(4)
     lib().orc_refdg_max_speed(ctypes.byref(p), _p(fortran(u)), _p(out))
     return tuple(out)
@@ -187,7 +187,7 @@ def refdg_limiter(p: DGQParams, u):
 
 
 def refdg_evolve(p: DGQParams, delta_u, x, y, t_end, max_steps=1 << 30, t0=0.0):
-    """evolve's loop (:672-759) from a limited state: (modes, t, steps, dts)."""
+    """evolve's loop (:666-769) from a limited state: (modes, t, steps, dts)."""
     du = fortran(delta_u).copy(order="F")
     t = ctypes.c_double(t0)
     nmax = int(min(max_steps, 1 << 20))

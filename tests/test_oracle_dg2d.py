@@ -350,7 +350,7 @@ def test_characteristic_limiter_on_an_entropy_wave_is_the_scalar_one():
     np.testing.assert_allclose(lc, ls, rtol=0, atol=1e-14)
 
 
-# ---- the reference's SSP(5,4) (2d/benchmark_2d_dg.f90:683-708) ------------------------
+# ---- the reference's SSP(5,4) (2d/benchmark_2d_dg.f90:683-710) ------------------------
 
 def _ssp54_butcher():
     """Butcher tableau (A, b) of the Shu-Osher stages of the oracle's SSP(5,4):
