@@ -83,10 +83,10 @@ struct Args {
   const float* un;     // U^n planes (stage combination), may be null when alpha == 0
   float* out;          // output planes (may alias un)
   const Consts* C;
-  const float* exn;    // e^{-k x} at x nodes   [3*nx]
-  const float* eyn;    // e^{-k y} at y nodes   [3*ny]
-  const float* exf;    // e^{-k x} at x faces   [nx+1]
-  const float* eyf;    // e^{-k y} at y faces   [ny+1]
+  const double* exn;   // e^{-k x} at x nodes   [3*nx]  (FP64)
+  const double* eyn;   // e^{-k y} at y nodes   [3*ny]
+  const double* exf;   // e^{-k x} at x faces   [nx+1]
+  const double* eyf;   // e^{-k y} at y faces   [ny+1]
   Ctrl* ctrl;
   long long plane;
   int pitch, nx, ny, parity;
@@ -101,8 +101,8 @@ struct Args {
   float* xout;
   float gam, pu, px, pin, pl;
   float gamma, g, odx, ody;
-  float rho0, p0, e0, ep0;   // rho0, p0, p0/(gamma-1), p0 gamma/(gamma-1): equilibrium rho, p, E, E+p per e^{-k(x+y)}
-  float gp0;                 // gamma p0: equilibrium gamma*p per e^{-k(x+y)}
+  double rho0;               // equilibrium density per e^{-k(x+y)}
+  float por, eor, epor, gpor;   // p, E = p/(gamma-1), E+p and gamma p of the equilibrium per unit density
   float ggm1, gfloor;        // gamma (gamma-1) and gamma*1e-10
 };
 
@@ -125,14 +125,16 @@ __device__ inline void point_terms(const Phys<float>& P, const FaceEq<float>& q,
   fy = {d.my, d.mx * v, d.my * v + dp, ep * v};
 }
 
-// Equilibrium at a point from the separable FP32 tables.
-__device__ inline FaceEq<float> eq_at(const Args& a, float e) {
+// Equilibrium at a point with e = e^{-kx} e^{-ky} from the separable FP64
+// tables: the density is formed in FP64 and rounded once; p, E (HLLC only),
+// E+p and gamma p are constant multiples of it (the isothermal atmosphere).
+__device__ inline FaceEq<float> eq_at(const Args& a, double e) {
   FaceEq<float> q;
-  q.rho = a.rho0 * e;
-  q.p = a.p0 * e;
-  q.e = a.e0 * e;      // used by HLLC only
-  q.ep = a.ep0 * e;
-  q.gp = a.gp0 * e;
+  q.rho = (float)(a.rho0 * e);
+  q.p = q.rho * a.por;
+  q.e = q.rho * a.eor;
+  q.ep = q.rho * a.epor;
+  q.gp = q.rho * a.gpor;
   return q;
 }
 
@@ -250,11 +252,11 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
 
   // equilibrium factors of this column: x nodes, the left x face, x node gq
   const int ic = min(max(i, 0), nx - 1), ifc = min(max(i, 0), nx);
-  float exr[3];
+  double exr[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) exr[r] = a.exn[3 * ic + r];
-  const float exf_i = a.exf[ifc];
-  const float exg = a.exn[3 * ic + gq];
+  const double exf_i = a.exf[ifc];
+  const double exg = a.exn[3 * ic + gq];
 
   auto fwd_face = [&](const float (&B)[4][2], v4f (&df)[4]) {
 #pragma unroll
@@ -338,8 +340,8 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
     for (int v = 0; v < 4; ++v) acc[v][2] = acc[v][3] = 0.f;
 
     // volume: fluxes at nodes (x_r, y_gq), fed back as k-slot g
-    const float* eyr = a.eyn + 3 * j;
-    const float eyg = g == 0 ? eyr[0] : (g == 1 ? eyr[1] : eyr[2]);
+    const double* eyr = a.eyn + 3 * j;
+    const double eyg = g == 0 ? eyr[0] : (g == 1 ? eyr[1] : eyr[2]);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const FaceEq<float> q = eq_at(a, exr[r] * eyg);
@@ -415,18 +417,18 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
 // out-of-range neighbours and masked lanes cost no address arithmetic.
 struct LimArgs {
   float* st;           // cell (0,0) of the 24 mode planes
-  const float* exn;    // e^{-k x} at x nodes [3*nx] (FP32)
-  const float* eyn;    // e^{-k y} at y nodes [3*ny] (FP32)
+  const double* exn;   // e^{-k x} at x nodes [3*nx] (FP64)
+  const double* eyn;   // e^{-k y} at y nodes [3*ny]
   Ctrl* ctrl;
   long long plane;
   int pitch, nx, ny, parity;
   int nrec;            // bytes from a variable's mode-0 cell (0,0) to the end of its six planes
   int limit, charlim, posit, cfl, bookkeeping, init;
-  const float* eqa;    // A_n(x cell) [3*nx] and B_n(y cell) [3*ny]: equilibrium modes
-  const float* eqb;    //   are rho0 A_a B_b (density) and eover A_a B_b (energy)
+  const double* eqa;   // A_n(x cell) [3*nx] and B_n(y cell) [3*ny] (FP64): equilibrium
+  const double* eqb;   //   modes are rho0 A_a B_b (density) and eover A_a B_b (energy)
   double t_end, cfl_over;
+  double rho0, eover;  // equilibrium rho and E per e^{-k(x+y)}
   float gamma, odx, ody, mdx2, mdy2;
-  float rho0, eover;   // equilibrium rho and E per e^{-k(x+y)}
   float phi1[3], phi2[3];   // phi_1, phi_2 at the Gauss nodes
 };
 
@@ -488,16 +490,16 @@ __device__ inline float solve_for_t(double gamma, double eps, const float uf[4],
 // tables; the set is {GLL x GL} u {GL x GLL} u {GL x GL}; thresholds 1e-5 of the
 // cell's equilibrium density and pressure.  Marks each variable it rewrites.
 __device__ inline void positivity_cell(float md[4][NM], bool chg[4], const LimArgs& a, int i, int j) {
-  const float ax[3] = {a.eqa[3 * i], a.eqa[3 * i + 1], a.eqa[3 * i + 2]};
-  const float by[3] = {a.eqb[3 * j], a.eqb[3 * j + 1], a.eqb[3 * j + 2]};
+  const double ax[3] = {a.eqa[3 * i], a.eqa[3 * i + 1], a.eqa[3 * i + 2]};
+  const double by[3] = {a.eqb[3 * j], a.eqb[3 * j + 1], a.eqb[3 * j + 2]};
   constexpr int MI_[NM] = {0, 1, 0, 2, 1, 0}, MJ_[NM] = {0, 0, 1, 0, 1, 2};
-  const float cv[4] = {a.rho0, 0.f, 0.f, a.eover};
+  const double cv[4] = {a.rho0, 0.0, 0.0, a.eover};
   float E[4][NM], F[4][NM];
 #pragma unroll
   for (int v = 0; v < 4; ++v)
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      E[v][m] = cv[v] * (ax[MI_[m]] * by[MJ_[m]]);
+      E[v][m] = (float)(cv[v] * (ax[MI_[m]] * by[MJ_[m]]));   // formed in FP64, rounded once
       F[v][m] = E[v][m] + md[v][m];
     }
   const float gm1 = a.gamma - 1.0f;
@@ -625,8 +627,9 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
       const float lw = ldm(rv[v], ow), le = ldm(rv[v], oe), ls = ldm(rv[v], os), ln = ldm(rv[v], on);
       nbw[v] = hw ? lw : mx; nbe[v] = he ? le : mx; nbs[v] = hs ? ls : my; nbn[v] = hn ? ln : my;
     }
-    const float e00 = a.eqa[3 * i] * a.eqb[3 * j];
-    const float ubar[4] = {a.rho0 * e00 + md[0][0], md[1][0], md[2][0], a.eover * e00 + md[3][0]};
+    const double e00 = a.eqa[3 * i] * a.eqb[3 * j];
+    const float ubar[4] = {(float)(a.rho0 * e00) + md[0][0], md[1][0], md[2][0],
+                           (float)(a.eover * e00) + md[3][0]};
     float lin[2][4];
     bool changed = false;
     auto limit_dir = [&](auto dirc, const float* up, const float* um, float mh2) {
@@ -687,7 +690,7 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
   if (a.cfl && in) {
     const float gm1 = a.gamma - 1.0f;
     const int ic = i;
-    float ex[NQ], ey[NQ];
+    double ex[NQ], ey[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) { ex[q] = a.exn[3 * ic + q]; ey[q] = a.eyn[3 * j + q]; }
     // node values by sum factorisation: d(x_a, y_b) = A0(a) + phi_1(y_b) A1(a) + phi_2(y_b) m5,
@@ -704,14 +707,14 @@ __global__ __launch_bounds__(256) void dg_limit_kernel(LimArgs a) {
     for (int qb = 0; qb < NQ; ++qb)
 #pragma unroll
       for (int qa = 0; qa < NQ; ++qa) {
-        const float e = ex[qa] * ey[qb];
+        const double e = ex[qa] * ey[qb];
         float d[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) d[v] = A0[v][qa] + a.phi1[qb] * A1[v][qa] + a.phi2[qb] * md[v][5];
-        const float rho = a.rho0 * e + d[0];
+        const float rho = (float)(a.rho0 * e) + d[0];
         const float inv = trcp(fmaxf(rho, 1e-10f));
         const float u = d[1] * inv, w = d[2] * inv;
-        const float p = gm1 * ((a.eover * e + d[3]) - 0.5f * rho * (u * u + w * w));
+        const float p = gm1 * (((float)(a.eover * e) + d[3]) - 0.5f * rho * (u * u + w * w));
         const float cs = tsqrt(a.gamma * fmaxf(p, 1e-10f) * inv);
         const float rt = (fabsf(u) + cs) * a.odx + (fabsf(w) + cs) * a.ody;
         bad |= !(rho > 0.f) || !(p > 0.f) || !(rt < 1e30f);
@@ -797,8 +800,7 @@ struct fvwb_dg {
   int nbuf = 3;
   double* host_stage = nullptr;   // device FP64 AoS staging
   double *exn = nullptr, *eyn = nullptr, *exf = nullptr, *eyf = nullptr;
-  float *exn32 = nullptr, *eyn32 = nullptr, *exf32 = nullptr, *eyf32 = nullptr;
-  float *eqa32 = nullptr, *eqb32 = nullptr;   // 1D Gauss projections of e^{-kx}, e^{-ky} per cell
+  double *eqa = nullptr, *eqb = nullptr;   // 1D Gauss projections of e^{-kx}, e^{-ky} per cell
   float* cell(int b) const { return buf[b] + org; }
   Consts* C = nullptr;
   Ctrl* ctrl = nullptr;
@@ -824,7 +826,7 @@ extern "C" const char* fvwb_dg_last_error(fvwb_dg_handle h) { return h ? h->err.
 extern "C" int fvwb_dg_destroy(fvwb_dg_handle h) {
   if (!h) return FVWB_OK;
   void* ps[] = {h->buf[0], h->buf[1], h->buf[2], h->buf[3], h->host_stage, h->exn, h->eyn, h->exf, h->eyf,
-                h->exn32, h->eyn32, h->exf32, h->eyf32, h->eqa32, h->eqb32, h->C, h->ctrl};
+                h->eqa, h->eqb, h->C, h->ctrl};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -874,9 +876,7 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
       hipMalloc(&h->exn, sizeof(double) * 3 * nx) != hipSuccess || hipMalloc(&h->eyn, sizeof(double) * 3 * ny) != hipSuccess ||
       hipMalloc(&h->exf, sizeof(double) * (nx + 1)) != hipSuccess ||
       hipMalloc(&h->eyf, sizeof(double) * (ny + 1)) != hipSuccess || hipMalloc(&h->C, sizeof(Consts)) != hipSuccess ||
-      hipMalloc(&h->exn32, sizeof(float) * 3 * nx) != hipSuccess || hipMalloc(&h->eyn32, sizeof(float) * 3 * ny) != hipSuccess ||
-      hipMalloc(&h->exf32, sizeof(float) * (nx + 1)) != hipSuccess || hipMalloc(&h->eyf32, sizeof(float) * (ny + 1)) != hipSuccess ||
-      hipMalloc(&h->eqa32, sizeof(float) * 3 * nx) != hipSuccess || hipMalloc(&h->eqb32, sizeof(float) * 3 * ny) != hipSuccess ||
+      hipMalloc(&h->eqa, sizeof(double) * 3 * nx) != hipSuccess || hipMalloc(&h->eqb, sizeof(double) * 3 * ny) != hipSuccess ||
       hipMalloc(&h->ctrl, sizeof(Ctrl)) != hipSuccess)
     return bail(FVWB_ERR_OOM, "device allocation failed");
   // padding columns and rows stay zero for the whole life of the handle
@@ -900,18 +900,18 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   // the equilibrium's modes for the positivity limiter: U_eq is separable, so
   // its Gauss projection on mode (a, b) is c_v A_a(x cell) B_b(y cell) with
   // A_a = 1/2 sum_q w_q e^{-k x_q} phi_a(xi_q)
-  std::vector<float> eqa(3 * nx), eqb(3 * ny);
+  std::vector<double> eqa(3 * nx), eqb(3 * ny);
   for (int i = 0; i < nx; ++i)
     for (int n = 0; n < 3; ++n) {
       double s = 0.0;
       for (int q = 0; q < 3; ++q) s += 0.5 * h->wq[q] * ex[3 * i + q] * leg(h->xq[q], n);
-      eqa[3 * i + n] = (float)s;
+      eqa[3 * i + n] = s;
     }
   for (int j = 0; j < ny; ++j)
     for (int n = 0; n < 3; ++n) {
       double s = 0.0;
       for (int q = 0; q < 3; ++q) s += 0.5 * h->wq[q] * ey[3 * j + q] * leg(h->xq[q], n);
-      eqb[3 * j + n] = (float)s;
+      eqb[3 * j + n] = s;
     }
   Consts C;
   std::memset(&C, 0, sizeof C);
@@ -947,8 +947,6 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
       C.back_face[3][row][k] = (float)(-0.5 * wq[k] * leg(xq[k], MI[m]) * leg(1.0, MJ[m]));
     }
   }
-  std::vector<float> ex32(ex.begin(), ex.end()), ey32(ey.begin(), ey.end()), fx32(fx.begin(), fx.end()),
-      fy32(fy.begin(), fy.end());
   // segments along y: as many waves as the device holds at once, at least 4 rows each
   {
     int dev = 0, cus = 0, blocks_per_cu = 0;
@@ -967,12 +965,8 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
       hipMemcpy(h->exf, fx.data(), sizeof(double) * fx.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->eyf, fy.data(), sizeof(double) * fy.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->C, &C, sizeof C, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->exn32, ex32.data(), sizeof(float) * ex32.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->eyn32, ey32.data(), sizeof(float) * ey32.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->exf32, fx32.data(), sizeof(float) * fx32.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->eyf32, fy32.data(), sizeof(float) * fy32.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->eqa32, eqa.data(), sizeof(float) * eqa.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->eqb32, eqb.data(), sizeof(float) * eqb.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->eqa, eqa.data(), sizeof(double) * eqa.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->eqb, eqb.data(), sizeof(double) * eqb.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemsetAsync(h->ctrl, 0, sizeof(Ctrl), h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess)
     return bail(FVWB_ERR_HIP, "copying tables failed");
   *out = h;
@@ -982,15 +976,17 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
 static Args stage_args(fvwb_dg* h) {
   const fvwb_dg_params& p = h->p;
   Args a{};
-  a.C = h->C; a.exn = h->exn32; a.eyn = h->eyn32; a.exf = h->exf32; a.eyf = h->eyf32; a.ctrl = h->ctrl;
+  a.C = h->C; a.exn = h->exn; a.eyn = h->eyn; a.exf = h->exf; a.eyf = h->eyf; a.ctrl = h->ctrl;
   a.plane = h->plane; a.pitch = h->pitch; a.nx = p.nx; a.ny = p.ny;
   a.ntx = h->ntx; a.nseg = h->nseg; a.seg_rows = h->seg_rows;
   a.cfl_over = p.cfl / 5.0;
   a.gamma = (float)p.gamma; a.g = (float)p.g;
   a.odx = (float)(p.nx / p.boxlen_x); a.ody = (float)(p.ny / p.boxlen_y);
-  a.rho0 = (float)p.rho0; a.p0 = (float)p.p0; a.e0 = (float)(p.p0 / (p.gamma - 1.0));
-  a.ep0 = (float)(p.p0 * p.gamma / (p.gamma - 1.0));
-  a.gp0 = (float)(p.p0 * p.gamma);
+  a.rho0 = p.rho0;
+  a.por = (float)(p.p0 / p.rho0);
+  a.eor = (float)(p.p0 / (p.rho0 * (p.gamma - 1.0)));
+  a.epor = (float)(p.p0 * p.gamma / (p.rho0 * (p.gamma - 1.0)));
+  a.gpor = (float)(p.p0 * p.gamma / p.rho0);
   a.ggm1 = (float)(p.gamma * (p.gamma - 1.0));
   a.gfloor = (float)(p.gamma * 1e-10);
   a.t_end = INFINITY;
@@ -1002,7 +998,7 @@ static Args stage_args(fvwb_dg* h) {
 static LimArgs lim_args(fvwb_dg* h, float* st) {
   const fvwb_dg_params& p = h->p;
   LimArgs a{};
-  a.st = st; a.exn = h->exn32; a.eyn = h->eyn32; a.ctrl = h->ctrl;
+  a.st = st; a.exn = h->exn; a.eyn = h->eyn; a.ctrl = h->ctrl;
   a.plane = h->plane; a.pitch = h->pitch; a.nx = p.nx; a.ny = p.ny;
   a.nrec = (int)((NM * h->plane - h->org) * 4);
   a.cfl_over = p.cfl / 5.0;
@@ -1010,12 +1006,12 @@ static LimArgs lim_args(fvwb_dg* h, float* st) {
   a.odx = (float)(p.nx / p.boxlen_x); a.ody = (float)(p.ny / p.boxlen_y);
   const double dx = p.boxlen_x / p.nx, dy = p.boxlen_y / p.ny;
   a.mdx2 = (float)(p.tvb_m * (dx * dx)); a.mdy2 = (float)(p.tvb_m * (dy * dy));
-  a.rho0 = (float)p.rho0; a.eover = (float)(p.p0 / (p.gamma - 1.0));
+  a.rho0 = p.rho0; a.eover = p.p0 / (p.gamma - 1.0);
   a.limit = p.limiter == FVWB_DG_LIMITER_TVB || p.limiter == FVWB_DG_LIMITER_TVB_POSITIVITY;
   a.charlim = p.limiter == FVWB_DG_LIMITER_CHARACTERISTIC || p.limiter == FVWB_DG_LIMITER_CHARACTERISTIC_POSITIVITY;
   a.posit = p.limiter == FVWB_DG_LIMITER_POSITIVITY || p.limiter == FVWB_DG_LIMITER_TVB_POSITIVITY ||
             p.limiter == FVWB_DG_LIMITER_CHARACTERISTIC_POSITIVITY;
-  a.eqa = h->eqa32; a.eqb = h->eqb32;
+  a.eqa = h->eqa; a.eqb = h->eqb;
   a.t_end = INFINITY;
   for (int q = 0; q < 3; ++q) {
     a.phi1[q] = (float)leg(h->xq[q], 1);
