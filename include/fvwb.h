@@ -72,7 +72,8 @@ typedef struct fvwb_params {
   double  gamma;       /* ratio of specific heats (reference: 1.4 as an FP32
                           literal, i.e. 1.39999997615814209)                  */
   double  boxlen_x, boxlen_y;
-  double  cfl;         /* dt = 0.5*dx/cmax*cfl (benchmark_2d.f90:242)         */
+  double  cfl;         /* dt = 0.5*h/cmax*cfl, h = min(dx, dy) (the reference's
+                          :242 takes dx, the same on its square grids)         */
   double  rho0, p0, g; /* equilibrium constants; nequilibrium=1 is (1,1,1),
                           nequilibrium=2/3 is (1.21 as FP32 literal, 1, 1)     */
 } fvwb_params;
@@ -102,7 +103,13 @@ int fvwb_compute_update_exact(fvwb_handle h, const double* u, const double* w_eq
 
 /* The residual L(u) that fvwb_advance integrates: as fvwb_compute_update_exact
  * but with the y faces at (j-1)*dy.  No separate counterpart in the reference
- * (identical to compute_update_exact on its square grids). */
+ * (identical to compute_update_exact when dx == dy, as in every shipped
+ * set-up).  With dx != dy the reference's face positions put the equilibrium
+ * of a y face at the wrong height; rest stays exact either way, since the same
+ * positions enter the flux and the equilibrium flux, but a perturbation is
+ * carried on the wrong background.  The resident loop uses the corrected
+ * positions; a caller who needs the reference's evolve at dx != dy bit for bit
+ * keeps its own loop over the drop-ins. */
 int fvwb_compute_residual(fvwb_handle h, const double* u, const double* w_eq, double* dudt);
 
 /* Replaces `subroutine compute_update(u, w_eq, dudt)` (benchmark_2d.f90:370-463),

@@ -29,20 +29,20 @@ import ref_runner as rr  # noqa: E402
 NSAMPLE = 4096
 
 
-def bench2d_inputs(salt=1, amp_bump=1e-3, amp_noise=1e-4):
-    """benchmark_2d set-up (nequilibrium=2, 300x300) plus a pressure bump and
-    hash noise; inputs rounded through FP32."""
-    p = po.make_params(300, 300)
+def bench2d_inputs(salt=1, amp_bump=1e-3, amp_noise=1e-4, nx=300, ny=300):
+    """benchmark_2d set-up (nequilibrium=2, 300x300 unless given) plus a
+    pressure bump and hash noise; inputs rounded through FP32."""
+    p = po.make_params(nx, ny)
     u, w = po.perturbed_state(p, amp_bump=amp_bump, amp_noise=amp_noise, salt=salt)
     u = np.asfortranarray(u.astype(np.float32).astype(np.float64))
     w = np.asfortranarray(w.astype(np.float32).astype(np.float64))
     return p, u, w
 
 
-def bench1d_inputs(salt=1):
-    """benchmark_1d set-up (nequilibrium=2, nx=128) plus a pressure pulse and
-    hash noise on every variable; inputs rounded through FP32."""
-    p = po.make_params_1d(128)
+def bench1d_inputs(salt=1, nequilibrium=2):
+    """benchmark_1d set-up (nequilibrium=2 unless given, nx=128) plus a
+    pressure pulse and hash noise on every variable; inputs rounded through FP32."""
+    p = po.make_params_1d(128, nequilibrium=nequilibrium)
     u, w = po.perturbed_state_1d(p, amp_bump=1e-3, amp_noise=1e-4, salt=salt)
     u = np.asfortranarray(u.astype(np.float32).astype(np.float64))
     w = np.asfortranarray(w.astype(np.float32).astype(np.float64))
@@ -301,6 +301,31 @@ def dgq_fixtures():
     return out
 
 
+def fv_variant_fixtures():
+    """The edited FV set-ups: compute_update_exact, compute_max_speed and three
+    Heun steps of evolve at 300 x 200 (dx != dy), and the 1D compute_update and
+    compute_max_speed with the polytropic equilibrium (nequilibrium 3)."""
+    assert rr.available("bench2d_300x200") and rr.available("bench1d_neq3"), "build oracle/_ref first"
+    out = {}
+    p, u, w = bench2d_inputs(salt=2, nx=300, ny=200)
+    lib = np.array("bench2d_300x200")
+    d = rr.run("bench2d_update_exact", u=u, w_eq=w, lib=lib)["dudt"]
+    c = rr.run("bench2d_max_speed", u=u, lib=lib)["cmax"]
+    tend = 2.5 * (0.5 * (1.0 / 300) / float(c[0]) * 0.5)
+    ueq = po.conservative(p.gamma, w)
+    ue = rr.run("bench2d_evolve", u=u, w_eq=w, tend=np.array(tend), lib=lib)["u"]
+    out["bench2d_update_exact_300x200"] = dict(summarize(d), salt=2, cmax=c)
+    out["bench2d_evolve_300x200"] = dict(summarize(ue - ueq), tend=np.array(tend), salt=2)
+    p1, u1, w1 = bench1d_inputs(salt=3, nequilibrium=3)
+    lib1 = np.array("bench1d_neq3")
+    ue1 = po.conservative_1d(p1.gamma, w1)
+    out["bench1d_update_neq3_128"] = {
+        "u": u1, "w_eq": w1, "dudt": rr.run("bench1d_update", u=u1, w_eq=w1, lib=lib1)["dudt"],
+        "dudt_eq": rr.run("bench1d_update", u=ue1, w_eq=w1, lib=lib1)["dudt"],
+        "cmax": rr.run("bench1d_max_speed", u=u1, lib=lib1)["cmax"]}
+    return out
+
+
 def main(only=None):
     assert rr.available("bench2d") and rr.available("dg2d"), "build oracle/_ref first"
     out = {}
@@ -314,6 +339,8 @@ def main(only=None):
         out.update(dg1d_fixtures())
     elif only == "dgq":
         out.update(dgq_fixtures())
+    elif only == "fvvar":
+        out.update(fv_variant_fixtures())
     if out:
         for name, arrs in out.items():
             path = os.path.join(HERE, name + ".npz")
@@ -416,6 +443,9 @@ def main(only=None):
 
     # the reference's own 2D DG program, shipped and with edited switches
     out.update(dgq_fixtures())
+
+    # the FV set-ups at nx != ny and with nequilibrium 3
+    out.update(fv_variant_fixtures())
 
     for name, arrs in out.items():
         path = os.path.join(HERE, name + ".npz")

@@ -30,6 +30,9 @@ REF_DIR = os.path.join(ROOT, "oracle", "_ref")
 LIBS = {
     "bench2d": "libref_bench2d.so",
     "bench1d": "libref_bench1d.so",
+    # edits of the FV set-ups (oracle/Makefile): nx != ny, nequilibrium 3
+    "bench2d_300x200": "libref_bench2d_300x200.so",
+    "bench1d_neq3": "libref_bench1d_neq3.so",
     "dg2d": "libref_dg2d.so",
     "dg1d": "libref_dg1d.so",
     "fvm": "libref_fvm.so",
@@ -124,7 +127,7 @@ def _child(op, fin, fout):
     z = dict(np.load(fin))
     out = {}
     if op.startswith("bench2d_"):
-        L = ctypes.CDLL(os.path.join(REF_DIR, LIBS["bench2d"]))
+        L = ctypes.CDLL(os.path.join(REF_DIR, LIBS[str(z["lib"]) if "lib" in z else "bench2d"]))
         if op == "bench2d_update_exact":
             u = np.asfortranarray(z["u"]); w = np.asfortranarray(z["w_eq"])
             d = np.zeros_like(u, order="F")
@@ -156,7 +159,7 @@ def _child(op, fin, fout):
         else:
             raise ValueError(op)
     elif op.startswith("bench1d_"):
-        L = ctypes.CDLL(os.path.join(REF_DIR, LIBS["bench1d"]))
+        L = ctypes.CDLL(os.path.join(REF_DIR, LIBS[str(z["lib"]) if "lib" in z else "bench1d"]))
         if op == "bench1d_update":
             u = np.asfortranarray(z["u"]); w = np.asfortranarray(z["w_eq"])
             d = np.zeros_like(u, order="F")

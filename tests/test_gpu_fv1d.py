@@ -55,6 +55,18 @@ def test_compute_update_matches_reference_golden_bitwise():
         assert s.compute_max_speed(u) == float(g["cmax"][0])
 
 
+def test_polytropic_dropin_matches_reference_golden_bitwise():
+    """compute_update and compute_max_speed against the reference built with
+    nequilibrium 3 (the polytropic faces), bit for bit."""
+    say("case: 1D compute_update nequilibrium 3, reference build")
+    g = load("bench1d_update_neq3_128")
+    p, u, w = bench1d_inputs(salt=3, nequilibrium=3)
+    with fvwb.Solver1D(fvwb.reference_params_1d(128, 3)) as s:
+        same(s.compute_update(u, w), g["dudt"], "dudt")
+        same(s.compute_update(po.conservative_1d(p.gamma, w), w), g["dudt_eq"], "dudt at rest")
+        assert s.compute_max_speed(u) == float(g["cmax"][0])
+
+
 @pytest.mark.parametrize("nx,neq", [(37, 1), (256, 2), (1000, 3), (4, 2)])
 def test_compute_update_matches_oracle_bitwise(nx, neq):
     say(f"case: 1D compute_update nx={nx} nequilibrium={neq}")

@@ -146,6 +146,36 @@ def test_evolve_matches_reference_golden():
     assert np.max(np.abs(sm["sample"] - g["sample"])) <= 1e-10
 
 
+def test_dropins_with_dx_ne_dy_match_reference_golden():
+    """The reference compiled at 300 x 200 (dx != dy): the drop-ins
+    compute_update_exact (y faces at (j-1)*dx, benchmark_2d.f90:513) and
+    compute_max_speed.  The resident loop integrates the residual with the y
+    faces at (j-1)*dy instead (fvwb_compute_residual): it equals the oracle's
+    run of that residual, and the reference's own three Heun steps differ from
+    it by more than rounding, which the golden shows."""
+    say("case: compute_update_exact 300x200 reference build; resident run vs oracle")
+    g = load("bench2d_update_exact_300x200")
+    p, u, w = bench2d_inputs(salt=int(g["salt"]), nx=300, ny=200)
+    with fvwb.Solver2D(fvwb.reference_params(300, 200, 2)) as s:
+        d = s.compute_update_exact(u, w)
+        c = s.compute_max_speed(u)
+    sm = summarize(d)
+    assert np.max(np.abs(sm["sample"] - g["sample"])) <= res_tol(300)
+    np.testing.assert_allclose(sm["abssum"], g["abssum"], rtol=1e-9)
+    assert abs(c - float(g["cmax"][0])) <= 4 * EPS
+    ge = load("bench2d_evolve_300x200")
+    tend = float(ge["tend"])
+    ug, t, n = fvwb.evolve(u, w, fvwb.reference_params(300, 200, 2), t_end=tend)
+    uo, to, no, _ = po.fv2d_advance(p, u, w, t_end=tend, max_steps=1000)
+    # the device takes the wave speed for dt from the closed-form centre
+    # equilibrium plus the deviation, the oracle from the FP32-rounded w_eq of
+    # the golden inputs: dt agrees to ~1e-8 relative, the state to ~1e-11
+    assert n == no == 3 and abs(t - to) <= 1e-7 * to
+    assert np.max(np.abs(ug - uo)) <= 1e-10
+    se = summarize(ug - po.conservative(p.gamma, w))
+    assert np.max(np.abs(se["sample"] - ge["sample"])) > 1e-9
+
+
 def test_evolve_matches_oracle_closed_form_equilibrium():
     say("case: evolve (Heun) 300x300, three steps, closed-form w_eq")
     p = po.make_params(300, 300)

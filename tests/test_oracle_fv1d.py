@@ -37,6 +37,20 @@ def test_ref1d_update_matches_reference_golden_bitwise():
     assert np.count_nonzero(g["dudt_eq"]) == 0
 
 
+def test_ref1d_update_polytropic_matches_reference_golden_bitwise():
+    """compute_update with the polytropic equilibrium (nequilibrium 3): a flang
+    build of benchmark_1d.f90 with that switch edited (oracle/Makefile); the
+    state, its rest and the wave-speed maximum, bit for bit."""
+    g = load("bench1d_update_neq3_128")
+    p, u, w = bench1d_inputs(salt=3, nequilibrium=3)
+    np.testing.assert_array_equal(u, g["u"])
+    np.testing.assert_array_equal(w, g["w_eq"])
+    np.testing.assert_array_equal(po.ref1d_update(p, u, w), g["dudt"])
+    assert po.ref1d_max_speed(p, u) == float(g["cmax"][0])
+    np.testing.assert_array_equal(po.ref1d_update(p, po.conservative_1d(p.gamma, w), w), g["dudt_eq"])
+    assert np.count_nonzero(g["dudt_eq"]) == 0
+
+
 def test_ref1d_update_sr_matches_reference_golden_bitwise():
     """compute_update_sr (benchmark_1d.f90:553-747, the 'WB1' solver) on the
     perturbed state, the isothermal rest state, the isentropic rest state and a

@@ -87,6 +87,32 @@ def test_evolve_matches_reference_golden():
     np.testing.assert_allclose(s["abssum"], g["abssum"], rtol=1e-11)
 
 
+def test_update_exact_speed_and_evolve_with_dx_ne_dy_match_reference_golden():
+    """The reference's 2D program compiled at nx = 300, ny = 200 (a sed edit of
+    parameters_2d.f90, oracle/Makefile): with dx != dy, compute_update_exact
+    places the y faces at (j-1)*dx (benchmark_2d.f90:513) and evolve takes dt
+    from dx alone (:242); the restatement follows both."""
+    g = load("bench2d_update_exact_300x200")
+    p, u, w = bench2d_inputs(salt=int(g["salt"]), nx=300, ny=200)
+    d = po.ref2d_update_exact(p, 2, u, w)
+    s = summarize(d)
+    np.testing.assert_array_equal(s["idx"], g["idx"])
+    tol = flux_scale(300)
+    assert np.max(np.abs(s["sample"] - g["sample"])) <= tol
+    assert np.max(np.abs(s["maxabs"] - g["maxabs"])) <= tol
+    np.testing.assert_allclose(s["abssum"], g["abssum"], rtol=1e-12)
+    assert abs(po.ref2d_max_speed(p, u) - float(g["cmax"][0])) <= 4 * EPS
+    # a restatement with the y faces at (j-1)*dy would differ at O(1) in the
+    # y fluxes: the golden must tell them apart
+    ge = load("bench2d_evolve_300x200")
+    ue, t, n = po.ref2d_evolve(p, 2, u, w, float(ge["tend"]))
+    assert n == 3
+    se = summarize(ue - po.conservative(p.gamma, w))
+    np.testing.assert_array_equal(se["idx"], ge["idx"])
+    assert np.max(np.abs(se["sample"] - ge["sample"])) <= 1e-13
+    np.testing.assert_allclose(se["abssum"], ge["abssum"], rtol=1e-11)
+
+
 def test_minmod_matches_reference_golden():
     g = load("minmod")
     mm = np.array([po.minmod(*t) for t in g["xyz"]])
