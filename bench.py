@@ -182,6 +182,40 @@ def time_dg(ev, n, steps, warmup, integrator=fvwb.DG_SSPRK3):
     return ms / 1e3
 
 
+def time_dgq(n, order=3, steps=20):
+    """The reference's own 2D DG program on the device (fvwb_dgq_*, FP64,
+    tensor basis with mx = my = order): n x n cells on the unit square, 'hll2',
+    get_source with the point-mass grad_phi, periodic copies, solver 'RK4' with
+    apply_limiter ('ONP') after every stage and the dt of
+    2d/benchmark_2d_dg.f90:671; a smooth state rho = 1 + 0.2 sin(2 pi x)
+    sin(2 pi y), u = v = 0.1, p = 1 at the Gauss nodes.  The resident loop
+    reads its clock between groups of 32 steps, so the host clock times it."""
+    p = fvwb.reference_params_dgq(n, order=order, flux_type=fvwb.DGQ_FLUX_HLL2, source=2)
+    xq, _ = fvwb.dgq_gl_quadrature(order)
+    dx, dy = p.boxlen_x / n, p.boxlen_y / n
+    ix = np.arange(n)
+    x = (ix + 0.5)[:, None, None, None] * dx + 0.5 * dx * xq[None, None, :, None] + 0.0 * xq[None, None, None, :]
+    y = (ix + 0.5)[None, :, None, None] * dy + 0.5 * dy * xq[None, None, None, :] + 0.0 * xq[None, None, :, None]
+    x = np.asfortranarray(np.broadcast_to(x, (n, n, order, order)))
+    y = np.asfortranarray(np.broadcast_to(y, (n, n, order, order)))
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y)
+    nodes = np.zeros((4, n, n, order, order), order="F")
+    nodes[0] = rho
+    nodes[1] = 0.1 * rho
+    nodes[2] = 0.1 * rho
+    nodes[3] = 1.0 / (p.gamma - 1.0) + 0.5 * rho * 0.02
+    with fvwb.SolverDGQ(p) as s:
+        du = s.apply_limiter(s.modes_from_nodes(nodes))
+        s.upload(du, x, y)
+        s.advance(2)                     # code objects loaded outside the timed region
+        s.upload(du, x, y)
+        t0 = time.perf_counter()
+        s.advance(steps)
+        sec = time.perf_counter() - t0
+        assert s.time()[2] == steps
+    return sec
+
+
 def time_fv1d(ev, ninst, n=256, t_end=0.25):
     """The 1D gravity workload: n cells, MUSCL-minmod, Rusanov, SSP-RK2, walls,
     CFL 0.5, rho = p = exp(-x) plus a pressure pulse 1e-3 exp(-(x-0.5)^2/(2 0.05^2)),
@@ -457,6 +491,9 @@ def main():
     ap.add_argument("--dg-only", action="store_true", help="time the DG workload only (no JSON line)")
     ap.add_argument("--rd-size", type=int, default=2049, help="nodes per side of the residual-distribution run (0: skip)")
     ap.add_argument("--rd-only", action="store_true", help="time the residual-distribution workload only (no JSON line)")
+    ap.add_argument("--dgq-size", type=int, default=512,
+                    help="cells per side of the reference DG program's run (0: skip)")
+    ap.add_argument("--dgq-only", action="store_true", help="time the reference DG program only (no JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -468,6 +505,11 @@ def main():
         secr = time_rd(args.rd_size, K)
         log(f"rd-only {args.rd_size}^2: {args.rd_size ** 2 * K / secr:.4e} node-updates/s, "
             f"{secr * 1e3 / K:.3f} ms/step")
+        return
+    if args.dgq_only:      # profiling runs of the reference DG program's kernels alone
+        secq = time_dgq(args.dgq_size, steps=K)
+        log(f"dgq-only {args.dgq_size}^2: {args.dgq_size ** 2 * K / secq:.4e} cell-updates/s, "
+            f"{secq * 1e3 / K:.3f} ms/step")
         return
     if args.dg_only:       # profiling runs of the DG kernels alone
         sec_dg = time_dg(ev, args.dg_size, K, W)
@@ -590,6 +632,16 @@ def main():
             dg["fp32_vs_fp64_oracle"] = cpu_leg(dg_accuracy, ndg, 2)
             log(f"dg {ndg}^2 fp32 vs the FP64 oracle: {dg['fp32_vs_fp64_oracle']}")
 
+    # ---- the reference's own 2D DG program (FP64, tensor basis) ----------------------------
+    dgq = None
+    if args.dgq_size > 0:
+        nq, kq = args.dgq_size, 20
+        secq = time_dgq(nq, 3, kq)
+        dgq = {"workload": f"{nq}x{nq}, mx = my = 3, 'hll2', point-mass gravity source, periodic, 'RK4' with "
+                           f"'ONP' after every stage, FP64 (the reference's 2d/benchmark_2d_dg.f90 program)",
+               "steps": kq, "ms_per_step": secq * 1e3 / kq, "value": nq * nq * kq / secq, "unit": "cell-updates/s"}
+        log(f"dgq {nq}^2: {dgq['value']:.4e} cell-updates/s, {dgq['ms_per_step']:.3f} ms/step")
+
     # ---- 1D workload (256 cells to t = 0.25, whole run in one launch) ---------------------
     rd = None
     if args.rd_size > 0:
@@ -650,6 +702,7 @@ def main():
         "config_2048": c2048,
         "cpu_reference": ref,
         "dg_p2": dg,
+        "dgq": dgq,
         "fv1d": fv1d,
         "rd": rd,
     }
