@@ -23,8 +23,8 @@
 //                     loop, the Runge-Kutta stage combination of evolve in
 //                     the reference's expression order.
 // dgq_limit_kernel is compute_positivity (2d/limiters.f90:478-654) per cell;
-// dgq_speed_kernel is compute_max_speed (:826-870) with its bookkeeping and
-// evolve's dt (:671) in one workgroup.
+// the dgq_speed_*_kernels are compute_max_speed (:826-870) with its
+// bookkeeping and evolve's dt (:671), as exact block reductions.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -414,109 +414,132 @@ __global__ __launch_bounds__(256) void dgq_face_kernel(Geo g, const double* __re
 // fluxes are recomputed per variable so the accumulators of one variable fit
 // in registers), then the stage combination.  Accumulation orders are the
 // reference's: over the nodes a outer, b inner for the volume and source
-// integrals, over the face points for the edges.
+// integrals, over the face points for the edges.  The residual of the
+// block's cells waits in LDS until all four variables are done: for a node
+// (i, j) the block's cells and variables are one contiguous run of the
+// output (element v + 4 (c + nc (i + O j))), which the block then combines
+// and stores in full cache lines.
+constexpr int CELL_THREADS = 128;
+
 template <int O>
-__global__ __launch_bounds__(256) void dgq_cell_kernel(Geo g, Tab T, const double* __restrict__ in,
-                                                       const double* __restrict__ x, const double* __restrict__ y,
-                                                       const double* __restrict__ F, const double* __restrict__ G,
-                                                       Epi e, double* out, const Clock* __restrict__ ck) {
+__global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void dgq_cell_kernel(
+    Geo g, Tab T, const double* __restrict__ in, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ F, const double* __restrict__ G, Epi e, double* out, const Clock* __restrict__ ck) {
   if (ck && !ck->active) return;
+  __shared__ double sres[O * O * CELL_THREADS * 4];
   const long long nc = (long long)g.nx * g.ny;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  const int ic = (int)(c % g.nx), jc = (int)(c / g.nx);
-  double m[4][O][O];
-#pragma unroll
-  for (int i = 0; i < O; ++i)
-#pragma unroll
-    for (int j = 0; j < O; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) m[v][i][j] = in[at<O>(v, c, nc, i, j)];
+  const long long c0 = (long long)blockIdx.x * CELL_THREADS;
+  const int tid = threadIdx.x;
+  const long long c = c0 + tid;
   const double dt = ck ? ck->dt : 0.0;
-  const double odx = g.odx;
+  if (c < nc) {
+    const int ic = (int)(c % g.nx), jc = (int)(c / g.nx);
+    double m[4][O][O];
+#pragma unroll
+    for (int i = 0; i < O; ++i)
+#pragma unroll
+      for (int j = 0; j < O; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) m[v][i][j] = in[at<O>(v, c, nc, i, j)];
+    const double odx = g.odx;
 #pragma unroll 1
-  for (int v = 0; v < 4; ++v) {
-    double fv1[O][O], fv2[O][O], sv[O][O];
+    for (int v = 0; v < 4; ++v) {
+      double fv1[O][O], fv2[O][O], sv[O][O];
 #pragma unroll
-    for (int i = 0; i < O; ++i)
+      for (int i = 0; i < O; ++i)
 #pragma unroll
-      for (int j = 0; j < O; ++j) fv1[i][j] = fv2[i][j] = sv[i][j] = 0.0;
+        for (int j = 0; j < O; ++j) fv1[i][j] = fv2[i][j] = sv[i][j] = 0.0;
+      // the node loops stay rolled: unrolled, the kernel held every table entry
+      // and node value at once (496 VGPRs, one wave per SIMD); the order of the
+      // sums is the same either way
+#pragma unroll 1
+      for (int a = 0; a < O; ++a)
+#pragma unroll 1
+        for (int b = 0; b < O; ++b) {
+          // get_nodes_from_modes (:544-592): modes a' outer, b' inner
+          double u[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int a = 0; a < O; ++a)
+          for (int ma = 0; ma < O; ++ma)
 #pragma unroll
-      for (int b = 0; b < O; ++b) {
-        // get_nodes_from_modes (:544-592): modes a' outer, b' inner
-        double u[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int mb = 0; mb < O; ++mb)
 #pragma unroll
-        for (int ma = 0; ma < O; ++ma)
-#pragma unroll
-          for (int mb = 0; mb < O; ++mb)
-#pragma unroll
-            for (int w = 0; w < 4; ++w) u[w] = u[w] + m[w][ma][mb] * T.L[a][ma] * T.L[b][mb];
-        double f1[4], f2[4];
-        flux_full(g.gamma, u, f1, f2);
-        double s = 0.0;
-        bool has_s = false;
-        if (g.source == 2) {                     // get_source (:1558-1576)
-          double w[4], g0, g1;
-          prim(g.gamma, u, w);
-          grad_phi(g.gpc, x[c + nc * (a + O * b)], y[c + nc * (a + O * b)], g0, g1);
-          s = v == 0 ? 0. : v == 1 ? w[0] * g0 : v == 2 ? w[0] * g1 : w[0] * (w[1] * g0 + w[2] * g1);
-          has_s = true;
-        } else if (g.source == 3) {              // get_adv_source (:1579-1596)
-          s = v == 0 ? -1.0 * u[0] : 0.0;
-          has_s = true;
-        }
-#pragma unroll
-        for (int i = 0; i < O; ++i)
-#pragma unroll
-          for (int j = 0; j < O; ++j) {
-            fv1[i][j] = fv1[i][j] + f1[v] * T.P[a][i] * T.wq[a] * T.L[b][j] * T.wq[b];
-            fv2[i][j] = fv2[i][j] + f2[v] * T.P[b][j] * T.wq[b] * T.L[a][i] * T.wq[a];
-            if (has_s) sv[i][j] = sv[i][j] + s * T.L[a][i] * T.wq[a] * T.L[b][j] * T.wq[b];
+              for (int w = 0; w < 4; ++w) u[w] = u[w] + m[w][ma][mb] * T.L[a][ma] * T.L[b][mb];
+          double f1[4], f2[4];
+          flux_full(g.gamma, u, f1, f2);
+          double s = 0.0;
+          bool has_s = false;
+          if (g.source == 2) {                     // get_source (:1558-1576)
+            double w[4], g0, g1;
+            prim(g.gamma, u, w);
+            grad_phi(g.gpc, x[c + nc * (a + O * b)], y[c + nc * (a + O * b)], g0, g1);
+            s = v == 0 ? 0. : v == 1 ? w[0] * g0 : v == 2 ? w[0] * g1 : w[0] * (w[1] * g0 + w[2] * g1);
+            has_s = true;
+          } else if (g.source == 3) {              // get_adv_source (:1579-1596)
+            s = v == 0 ? -1.0 * u[0] : 0.0;
+            has_s = true;
           }
-      }
-    // face values of this variable: F at the cell's left (ic) and right (ic+1)
-    // x faces, G at its bottom (jc) and top (jc+1) y faces
-    double fr[O], fle[O], gt[O], gb[O];
 #pragma unroll
-    for (int q = 0; q < O; ++q) {
-      fle[q] = F[4 * (((long long)ic + (long long)(g.nx + 1) * jc) * O + q) + v];
-      fr[q] = F[4 * (((long long)ic + 1 + (long long)(g.nx + 1) * jc) * O + q) + v];
-      gb[q] = G[4 * (((long long)ic + (long long)g.nx * jc) * O + q) + v];
-      gt[q] = G[4 * (((long long)ic + (long long)g.nx * (jc + 1)) * O + q) + v];
+          for (int i = 0; i < O; ++i)
+#pragma unroll
+            for (int j = 0; j < O; ++j) {
+              fv1[i][j] = fv1[i][j] + f1[v] * T.P[a][i] * T.wq[a] * T.L[b][j] * T.wq[b];
+              fv2[i][j] = fv2[i][j] + f2[v] * T.P[b][j] * T.wq[b] * T.L[a][i] * T.wq[a];
+              if (has_s) sv[i][j] = sv[i][j] + s * T.L[a][i] * T.wq[a] * T.L[b][j] * T.wq[b];
+            }
+        }
+      // face values of this variable: F at the cell's left (ic) and right (ic+1)
+      // x faces, G at its bottom (jc) and top (jc+1) y faces
+      double fr[O], fle[O], gt[O], gb[O];
+#pragma unroll
+      for (int q = 0; q < O; ++q) {
+        fle[q] = F[4 * (((long long)ic + (long long)(g.nx + 1) * jc) * O + q) + v];
+        fr[q] = F[4 * (((long long)ic + 1 + (long long)(g.nx + 1) * jc) * O + q) + v];
+        gb[q] = G[4 * (((long long)ic + (long long)g.nx * jc) * O + q) + v];
+        gt[q] = G[4 * (((long long)ic + (long long)g.nx * (jc + 1)) * O + q) + v];
+      }
+#pragma unroll
+      for (int i = 0; i < O; ++i)
+#pragma unroll
+        for (int j = 0; j < O; ++j) {
+          double e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0;
+#pragma unroll
+          for (int q = 0; q < O; ++q) {            // :1377-1386
+            e1 = e1 + fr[q] * T.lp[i] * T.L[q][j] * T.wq[q];
+            e2 = e2 + fle[q] * T.lm[i] * T.L[q][j] * T.wq[q];
+          }
+#pragma unroll
+          for (int q = 0; q < O; ++q) {            // :1391-1408
+            e3 = e3 + gt[q] * T.lp[j] * T.L[q][i] * T.wq[q];
+            e4 = e4 + gb[q] * T.lm[j] * T.L[q][i] * T.wq[q];
+          }
+          // :1455-1462 (oneoverdx for the y terms too)
+          double d = (odx * fv1[i][j] + odx * fv2[i][j] - odx * (e1 - e2) - odx * (e3 - e4)) / 2. + sv[i][j] / 4.;
+          if (g.ninit == 12) {                     // special_boundary_conditions (:1480-1510)
+            const double xd = x[c + nc * (i + O * j)] - g.boxlen_x / 2.;
+            const double yd = y[c + nc * (i + O * j)] - g.boxlen_y / 2.;
+            if (sqrt(xd * xd + yd * yd) > 2.0) d = 0.0;
+          }
+          sres[((i + O * j) * CELL_THREADS + tid) * 4 + v] = d;
+        }
     }
-#pragma unroll
-    for (int i = 0; i < O; ++i)
-#pragma unroll
-      for (int j = 0; j < O; ++j) {
-        double e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0;
-#pragma unroll
-        for (int q = 0; q < O; ++q) {            // :1377-1386
-          e1 = e1 + fr[q] * T.lp[i] * T.L[q][j] * T.wq[q];
-          e2 = e2 + fle[q] * T.lm[i] * T.L[q][j] * T.wq[q];
-        }
-#pragma unroll
-        for (int q = 0; q < O; ++q) {            // :1391-1408
-          e3 = e3 + gt[q] * T.lp[j] * T.L[q][i] * T.wq[q];
-          e4 = e4 + gb[q] * T.lm[j] * T.L[q][i] * T.wq[q];
-        }
-        // :1455-1462 (oneoverdx for the y terms too)
-        double d = (odx * fv1[i][j] + odx * fv2[i][j] - odx * (e1 - e2) - odx * (e3 - e4)) / 2. + sv[i][j] / 4.;
-        if (g.ninit == 12) {                     // special_boundary_conditions (:1480-1510)
-          const double xd = x[c + nc * (i + O * j)] - g.boxlen_x / 2.;
-          const double yd = y[c + nc * (i + O * j)] - g.boxlen_y / 2.;
-          if (sqrt(xd * xd + yd * yd) > 2.0) d = 0.0;
-        }
-        const long long idx = at<O>(v, c, nc, i, j);
-        if (e.nterms == 0) {
-          out[idx] = d;
-        } else {
-          double sacc = e.a[0] * e.X[0][idx];
-          for (int k = 1; k < e.nterms; ++k) sacc = sacc + e.a[k] * e.X[k][idx];
-          out[idx] = sacc + e.cdt * dt * d;
-        }
+  }
+  __syncthreads();
+  // node (i, j): elements [4 (c0 + nc (i + O j)), + 4 ncells) of every array
+  const int ncell = (int)(nc - c0 < CELL_THREADS ? nc - c0 : CELL_THREADS);
+#pragma unroll 1
+  for (int ij = 0; ij < O * O; ++ij) {
+    const long long base = 4 * (c0 + nc * ij);
+    for (int q = tid; q < 4 * ncell; q += CELL_THREADS) {
+      const long long idx = base + q;
+      const double d = sres[ij * CELL_THREADS * 4 + q];
+      if (e.nterms == 0) {
+        out[idx] = d;
+      } else {
+        double sacc = e.a[0] * e.X[0][idx];
+        for (int k = 1; k < e.nterms; ++k) sacc = sacc + e.a[k] * e.X[k][idx];
+        out[idx] = sacc + e.cdt * dt * d;
       }
+    }
   }
 }
 
@@ -655,13 +678,17 @@ __global__ __launch_bounds__(256) void dgq_project_kernel(Geo g, Tab T, const do
 }
 
 // compute_max_speed (:826-870) over the cells in the reference's order
-// k = i ny + j (x outer) and, with a clock, evolve's dt (:668-671); one
-// workgroup.  The maximum speed S, the last k whose speed equals S (the last
-// update of the reference's running ">=" test) and the smallest sound speed
-// from that cell on (cs_max restarts there and only decreases) are each exact
-// reductions, so their order does not matter.  res = (cs_max, v_xmax,
-// v_ymax, speed_max).
-constexpr int SPEED_THREADS = 1024;
+// k = i ny + j (x outer) and, with a clock, evolve's dt (:668-671).  The
+// maximum speed S, the last k whose speed equals S (the last update of the
+// reference's running ">=" test) and the smallest sound speed from that cell
+// on (cs_max restarts there and only decreases) are each exact reductions, so
+// their order does not matter.  Three launches: per block the pair (S_b, last
+// k with speed S_b); per block the smallest sound speed at k >= k* (every
+// block first combines the pairs itself); one block that combines everything,
+// re-evaluates the cell k* and sets dt.  res = (cs_max, v_xmax, v_ymax,
+// speed_max); part holds three doubles per block.
+constexpr int SPEED_THREADS = 256;
+constexpr int SPEED_MAX_BLOCKS = 1024;
 
 __device__ __forceinline__ void cell_speed(const Geo& g, const double* __restrict__ u, long long m, double& cs,
                                            double& vx, double& vy, double& sp) {
@@ -671,65 +698,89 @@ __device__ __forceinline__ void cell_speed(const Geo& g, const double* __restric
   speed(g.gamma, q, cs, vx, vy, sp);
 }
 
-__global__ __launch_bounds__(SPEED_THREADS) void dgq_speed_kernel(Geo g, const double* __restrict__ u,
-                                                                 double* __restrict__ res, Clock* __restrict__ ck,
-                                                                 double t_end) {
+// (s, k) pairs: the larger speed wins, equal speeds keep the larger k
+__device__ __forceinline__ void pair_max(double& s, long long& k, double so, long long ko) {
+  if (so > s) { s = so; k = ko; }
+  else if (so == s && ko > k) k = ko;
+}
+
+__device__ __forceinline__ void wave_pair_max(double& s, long long& k) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double so = __shfl_xor(s, off);
+    const long long ko = __shfl_xor(k, off);
+    pair_max(s, k, so, ko);
+  }
+}
+
+// a step that will not run (clock stopped or past t_end) skips the passes
+__device__ __forceinline__ bool speed_runs(const Clock* ck, double t_end) {
+  return !ck || (!ck->err && ck->t < t_end);
+}
+
+// the pairs of every block combined (each thread of the block strides over
+// them, then the block reduces): S and k* (-1 when no speed reached S >= 0)
+__device__ void combine_pairs(const double* __restrict__ part, int nblk, double& S, long long& ks) {
   __shared__ double sd[SPEED_THREADS / 64];
   __shared__ long long sk[SPEED_THREADS / 64];
-  __shared__ int run;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  if (tid == 0) run = ck ? (!ck->err && ck->t < t_end) : 1;
+  double s = 0.0;
+  long long k = -1;
+  for (int b = tid; b < nblk; b += SPEED_THREADS) pair_max(s, k, part[3 * b], (long long)part[3 * b + 1]);
+  wave_pair_max(s, k);
+  if (lane == 0) { sd[wv] = s; sk[wv] = k; }
   __syncthreads();
-  if (!run) {
-    if (tid == 0) ck->active = 0;
-    return;
-  }
+  s = sd[0];
+  k = sk[0];
+  for (int w = 1; w < SPEED_THREADS / 64; ++w) pair_max(s, k, sd[w], sk[w]);
+  S = s;
+  ks = k;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(SPEED_THREADS) void dgq_speed_pairs_kernel(Geo g, const double* __restrict__ u,
+                                                                       double* __restrict__ part,
+                                                                       const Clock* __restrict__ ck, double t_end) {
+  if (!speed_runs(ck, t_end)) return;
+  __shared__ double sd[SPEED_THREADS / 64];
+  __shared__ long long sk[SPEED_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const long long nc = (long long)g.nx * g.ny;
-  // S
-  double smax = 0.0;
-  for (long long m = tid; m < nc; m += SPEED_THREADS) {
+  // speeds start at 0 with k = -1: a speed that is 0 joins the pair at 0, a
+  // NaN never compares, as in the reference's running test from 0
+  double s = 0.0;
+  long long k = -1;
+  for (long long m = (long long)blockIdx.x * SPEED_THREADS + tid; m < nc; m += (long long)gridDim.x * SPEED_THREADS) {
     double cs, vx, vy, sp;
     cell_speed(g, u, m, cs, vx, vy, sp);
-    if (sp >= smax) smax = sp;
+    const long long km = (m % g.nx) * g.ny + m / g.nx;
+    if (sp > s) { s = sp; k = km; }
+    else if (sp == s && km > k) k = km;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    const double o = __shfl_xor(smax, off);
-    if (o > smax) smax = o;
-  }
-  if (lane == 0) sd[wv] = smax;
+  wave_pair_max(s, k);
+  if (lane == 0) { sd[wv] = s; sk[wv] = k; }
   __syncthreads();
   if (tid == 0) {
-    double s = sd[0];
-    for (int w = 1; w < SPEED_THREADS / 64; ++w) if (sd[w] > s) s = sd[w];
-    sd[0] = s;
+    for (int w = 1; w < SPEED_THREADS / 64; ++w) pair_max(s, k, sd[w], sk[w]);
+    part[3 * blockIdx.x] = s;
+    part[3 * blockIdx.x + 1] = (double)k;   // exact: k < 2^53
   }
-  __syncthreads();
-  const double S = sd[0];
-  __syncthreads();
-  // k*
-  long long kbest = -1;
-  for (long long m = tid; m < nc; m += SPEED_THREADS) {
-    double cs, vx, vy, sp;
-    cell_speed(g, u, m, cs, vx, vy, sp);
-    const long long k = (m % g.nx) * g.ny + m / g.nx;
-    if (sp == S && k > kbest) kbest = k;
-  }
-  for (int off = 32; off > 0; off >>= 1) kbest = max(kbest, (long long)__shfl_xor(kbest, off));
-  if (lane == 0) sk[wv] = kbest;
-  __syncthreads();
-  if (tid == 0) {
-    long long b = sk[0];
-    for (int w = 1; w < SPEED_THREADS / 64; ++w) b = max(b, sk[w]);
-    sk[0] = b;
-  }
-  __syncthreads();
-  const long long ks = sk[0];
-  // cs_max: min of cs over k >= k*
+}
+
+__global__ __launch_bounds__(SPEED_THREADS) void dgq_speed_csmin_kernel(Geo g, const double* __restrict__ u,
+                                                                       double* __restrict__ part,
+                                                                       const Clock* __restrict__ ck, double t_end) {
+  if (!speed_runs(ck, t_end)) return;
+  __shared__ double sd[SPEED_THREADS / 64];
+  double S;
+  long long ks;
+  combine_pairs(part, gridDim.x, S, ks);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const long long nc = (long long)g.nx * g.ny;
   double csm = INFINITY;
   if (ks >= 0)
-    for (long long m = tid; m < nc; m += SPEED_THREADS) {
-      const long long k = (m % g.nx) * g.ny + m / g.nx;
-      if (k < ks) continue;
+    for (long long m = (long long)blockIdx.x * SPEED_THREADS + tid; m < nc; m += (long long)gridDim.x * SPEED_THREADS) {
+      const long long km = (m % g.nx) * g.ny + m / g.nx;
+      if (km < ks) continue;
       double cs, vx, vy, sp;
       cell_speed(g, u, m, cs, vx, vy, sp);
       if (csm > cs) csm = cs;
@@ -738,12 +789,38 @@ __global__ __launch_bounds__(SPEED_THREADS) void dgq_speed_kernel(Geo g, const d
     const double o = __shfl_xor(csm, off);
     if (csm > o) csm = o;
   }
-  __syncthreads();
   if (lane == 0) sd[wv] = csm;
   __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < SPEED_THREADS / 64; ++w) if (csm > sd[w]) csm = sd[w];
+    part[3 * blockIdx.x + 2] = csm;
+  }
+}
+
+__global__ __launch_bounds__(SPEED_THREADS) void dgq_speed_final_kernel(Geo g, const double* __restrict__ u,
+                                                                       const double* __restrict__ part, int nblk,
+                                                                       double* __restrict__ res,
+                                                                       Clock* __restrict__ ck, double t_end) {
+  if (!speed_runs(ck, t_end)) {
+    if (threadIdx.x == 0) ck->active = 0;
+    return;
+  }
+  double S;
+  long long ks;
+  combine_pairs(part, nblk, S, ks);
+  // the smallest of the blocks' sound-speed minima, in parallel
+  __shared__ double sc[SPEED_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double cmin = INFINITY;
+  for (int b = tid; b < nblk; b += SPEED_THREADS) if (cmin > part[3 * b + 2]) cmin = part[3 * b + 2];
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(cmin, off);
+    if (cmin > o) cmin = o;
+  }
+  if (lane == 0) sc[wv] = cmin;
+  __syncthreads();
   if (tid != 0) return;
-  double cmin = sd[0];
-  for (int w = 1; w < SPEED_THREADS / 64; ++w) if (cmin > sd[w]) cmin = sd[w];
+  for (int w = 1; w < SPEED_THREADS / 64; ++w) if (cmin > sc[w]) cmin = sc[w];
   double cs_max = 0.0, vxm = 0.0, vym = 0.0;
   if (ks >= 0) {
     const long long mb = (ks / g.ny) + (long long)g.nx * (ks % g.ny);
@@ -918,6 +995,22 @@ extern "C" int fvwb_dgq_destroy(fvwb_dgq_handle h) {
 
 static long long err_blocks(long long nc) { return (nc + ERR_THREADS - 1) / ERR_THREADS; }
 
+// compute_max_speed (and, with a clock, evolve's dt) as three launches; the
+// per-block partials use the handle's partial-sum buffer (3 doubles per block,
+// at most err_blocks(nc) blocks)
+static int speed_blocks(long long nc) {
+  return (int)std::max(1LL, std::min<long long>((nc + SPEED_THREADS - 1) / SPEED_THREADS, SPEED_MAX_BLOCKS));
+}
+static void launch_speed(fvwb_dgq* h, const double* u, double* res, Clock* ck, double t_end) {
+  const int nb = speed_blocks(h->nc);
+  hipLaunchKernelGGL(dgq_speed_pairs_kernel, dim3(nb), dim3(SPEED_THREADS), 0, h->stream, h->g, u, h->part,
+                     (const Clock*)ck, t_end);
+  hipLaunchKernelGGL(dgq_speed_csmin_kernel, dim3(nb), dim3(SPEED_THREADS), 0, h->stream, h->g, u, h->part,
+                     (const Clock*)ck, t_end);
+  hipLaunchKernelGGL(dgq_speed_final_kernel, dim3(1), dim3(SPEED_THREADS), 0, h->stream, h->g, u,
+                     (const double*)h->part, nb, res, ck, t_end);
+}
+
 extern "C" int fvwb_dgq_create(fvwb_dgq_handle* out, const fvwb_dgq_params* p) {
   if (!out || !p) return failq(nullptr, FVWB_ERR_ARG, "NULL argument");
   *out = nullptr;
@@ -969,6 +1062,7 @@ extern "C" int fvwb_dgq_create(fvwb_dgq_handle* out, const fvwb_dgq_params* p) {
 }
 
 static dim3 cells_grid(const fvwb_dgq* h) { return dim3((unsigned)((h->nc + 255) / 256)); }
+static dim3 cell_grid(const fvwb_dgq* h) { return dim3((unsigned)((h->nc + CELL_THREADS - 1) / CELL_THREADS)); }
 
 // compute_update of `in` into `out` with the stage combination e (nterms = 0:
 // dudt itself); the clock gates and provides dt in the resident loop.
@@ -979,13 +1073,13 @@ static int residual(fvwb_dgq* h, const double* in, const double* x, const double
   if (h->p.order == 2) {
     hipLaunchKernelGGL(dgq_trace_kernel<2>, cells_grid(h), dim3(256), 0, h->stream, h->g, h->T, in, h->tr, ck);
     hipLaunchKernelGGL(dgq_face_kernel<2>, fg, dim3(256), 0, h->stream, h->g, h->tr, h->F, h->G, ck);
-    hipLaunchKernelGGL(dgq_cell_kernel<2>, cells_grid(h), dim3(256), 0, h->stream, h->g, h->T, in, x, y, h->F,
-                       h->G, e, out, ck);
+    hipLaunchKernelGGL(dgq_cell_kernel<2>, cell_grid(h), dim3(CELL_THREADS), 0, h->stream, h->g, h->T, in, x, y,
+                       h->F, h->G, e, out, ck);
   } else {
     hipLaunchKernelGGL(dgq_trace_kernel<3>, cells_grid(h), dim3(256), 0, h->stream, h->g, h->T, in, h->tr, ck);
     hipLaunchKernelGGL(dgq_face_kernel<3>, fg, dim3(256), 0, h->stream, h->g, h->tr, h->F, h->G, ck);
-    hipLaunchKernelGGL(dgq_cell_kernel<3>, cells_grid(h), dim3(256), 0, h->stream, h->g, h->T, in, x, y, h->F,
-                       h->G, e, out, ck);
+    hipLaunchKernelGGL(dgq_cell_kernel<3>, cell_grid(h), dim3(CELL_THREADS), 0, h->stream, h->g, h->T, in, x, y,
+                       h->F, h->G, e, out, ck);
   }
   HIPQ(h, hipGetLastError());
   return FVWB_OK;
@@ -1046,8 +1140,7 @@ extern "C" int fvwb_dgq_compute_max_speed(fvwb_dgq_handle h, const double* u, do
                                           double* v_ymax, double* speed_max) {
   if (!h || !u || !cs_max || !v_xmax || !v_ymax || !speed_max) return failq(h, FVWB_ERR_ARG, "NULL argument");
   HIPQ(h, hipMemcpyAsync(h->s_in, u, sizeof(double) * 4 * (size_t)h->nc, hipMemcpyHostToDevice, h->stream));
-  hipLaunchKernelGGL(dgq_speed_kernel, dim3(1), dim3(SPEED_THREADS), 0, h->stream, h->g, h->s_in, h->res,
-                     (Clock*)nullptr, 0.0);
+  launch_speed(h, h->s_in, h->res, nullptr, 0.0);
   HIPQ(h, hipGetLastError());
   double r[4];
   HIPQ(h, hipMemcpyAsync(r, h->res, sizeof r, hipMemcpyDeviceToHost, h->stream));
@@ -1132,8 +1225,7 @@ static int step(fvwb_dgq* h, double t_end) {
   Clock* ck = h->clk;
   double* du = h->du;
   double *w1 = h->w[0], *w2 = h->w[1], *w3 = h->w[2], *w4 = h->w[3], *w5 = h->w[4];
-  hipLaunchKernelGGL(dgq_speed_kernel, dim3(1), dim3(SPEED_THREADS), 0, h->stream, h->g, (const double*)du,
-                     (double*)nullptr, ck, t_end);
+  launch_speed(h, du, nullptr, ck, t_end);
   HIPQ(h, hipGetLastError());
   int rc = 0;
   const double *x = h->x, *y = h->y;
