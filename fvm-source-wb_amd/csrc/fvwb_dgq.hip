@@ -442,13 +442,18 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(2)
 #pragma unroll
         for (int v = 0; v < 4; ++v) m[v][i][j] = in[at<O>(v, c, nc, i, j)];
     const double odx = g.odx;
+    // two variables per pass: the nodal state and its fluxes, which every
+    // variable needs, are formed once for the pair; each accumulator still
+    // takes its terms in the reference's order
 #pragma unroll 1
-    for (int v = 0; v < 4; ++v) {
-      double fv1[O][O], fv2[O][O], sv[O][O];
+    for (int vp = 0; vp < 4; vp += 2) {
+      double fv1[2][O][O], fv2[2][O][O], sv[2][O][O];
 #pragma unroll
-      for (int i = 0; i < O; ++i)
+      for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int j = 0; j < O; ++j) fv1[i][j] = fv2[i][j] = sv[i][j] = 0.0;
+        for (int i = 0; i < O; ++i)
+#pragma unroll
+          for (int j = 0; j < O; ++j) fv1[r][i][j] = fv2[r][i][j] = sv[r][i][j] = 0.0;
       // the node loops stay rolled: unrolled, the kernel held every table entry
       // and node value at once (496 VGPRs, one wave per SIMD); the order of the
       // sums is the same either way
@@ -466,61 +471,73 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(2)
               for (int w = 0; w < 4; ++w) u[w] = u[w] + m[w][ma][mb] * T.L[a][ma] * T.L[b][mb];
           double f1[4], f2[4];
           flux_full(g.gamma, u, f1, f2);
-          double s = 0.0;
+          double s[2] = {0.0, 0.0};
           bool has_s = false;
           if (g.source == 2) {                     // get_source (:1558-1576)
             double w[4], g0, g1;
             prim(g.gamma, u, w);
             grad_phi(g.gpc, x[c + nc * (a + O * b)], y[c + nc * (a + O * b)], g0, g1);
-            s = v == 0 ? 0. : v == 1 ? w[0] * g0 : v == 2 ? w[0] * g1 : w[0] * (w[1] * g0 + w[2] * g1);
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+              const int v = vp + r;
+              s[r] = v == 0 ? 0. : v == 1 ? w[0] * g0 : v == 2 ? w[0] * g1 : w[0] * (w[1] * g0 + w[2] * g1);
+            }
             has_s = true;
           } else if (g.source == 3) {              // get_adv_source (:1579-1596)
-            s = v == 0 ? -1.0 * u[0] : 0.0;
+#pragma unroll
+            for (int r = 0; r < 2; ++r) s[r] = vp + r == 0 ? -1.0 * u[0] : 0.0;
             has_s = true;
           }
 #pragma unroll
-          for (int i = 0; i < O; ++i)
+          for (int r = 0; r < 2; ++r)
 #pragma unroll
-            for (int j = 0; j < O; ++j) {
-              fv1[i][j] = fv1[i][j] + f1[v] * T.P[a][i] * T.wq[a] * T.L[b][j] * T.wq[b];
-              fv2[i][j] = fv2[i][j] + f2[v] * T.P[b][j] * T.wq[b] * T.L[a][i] * T.wq[a];
-              if (has_s) sv[i][j] = sv[i][j] + s * T.L[a][i] * T.wq[a] * T.L[b][j] * T.wq[b];
+            for (int i = 0; i < O; ++i)
+#pragma unroll
+              for (int j = 0; j < O; ++j) {
+                fv1[r][i][j] = fv1[r][i][j] + f1[vp + r] * T.P[a][i] * T.wq[a] * T.L[b][j] * T.wq[b];
+                fv2[r][i][j] = fv2[r][i][j] + f2[vp + r] * T.P[b][j] * T.wq[b] * T.L[a][i] * T.wq[a];
+                if (has_s) sv[r][i][j] = sv[r][i][j] + s[r] * T.L[a][i] * T.wq[a] * T.L[b][j] * T.wq[b];
+              }
+        }
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int v = vp + r;
+        // face values of this variable: F at the cell's left (ic) and right
+        // (ic+1) x faces, G at its bottom (jc) and top (jc+1) y faces
+        double fr[O], fle[O], gt[O], gb[O];
+#pragma unroll
+        for (int q = 0; q < O; ++q) {
+          fle[q] = F[4 * (((long long)ic + (long long)(g.nx + 1) * jc) * O + q) + v];
+          fr[q] = F[4 * (((long long)ic + 1 + (long long)(g.nx + 1) * jc) * O + q) + v];
+          gb[q] = G[4 * (((long long)ic + (long long)g.nx * jc) * O + q) + v];
+          gt[q] = G[4 * (((long long)ic + (long long)g.nx * (jc + 1)) * O + q) + v];
+        }
+#pragma unroll
+        for (int i = 0; i < O; ++i)
+#pragma unroll
+          for (int j = 0; j < O; ++j) {
+            double e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0;
+#pragma unroll
+            for (int q = 0; q < O; ++q) {          // :1377-1386
+              e1 = e1 + fr[q] * T.lp[i] * T.L[q][j] * T.wq[q];
+              e2 = e2 + fle[q] * T.lm[i] * T.L[q][j] * T.wq[q];
             }
-        }
-      // face values of this variable: F at the cell's left (ic) and right (ic+1)
-      // x faces, G at its bottom (jc) and top (jc+1) y faces
-      double fr[O], fle[O], gt[O], gb[O];
 #pragma unroll
-      for (int q = 0; q < O; ++q) {
-        fle[q] = F[4 * (((long long)ic + (long long)(g.nx + 1) * jc) * O + q) + v];
-        fr[q] = F[4 * (((long long)ic + 1 + (long long)(g.nx + 1) * jc) * O + q) + v];
-        gb[q] = G[4 * (((long long)ic + (long long)g.nx * jc) * O + q) + v];
-        gt[q] = G[4 * (((long long)ic + (long long)g.nx * (jc + 1)) * O + q) + v];
+            for (int q = 0; q < O; ++q) {          // :1391-1408
+              e3 = e3 + gt[q] * T.lp[j] * T.L[q][i] * T.wq[q];
+              e4 = e4 + gb[q] * T.lm[j] * T.L[q][i] * T.wq[q];
+            }
+            // :1455-1462 (oneoverdx for the y terms too)
+            double d = (odx * fv1[r][i][j] + odx * fv2[r][i][j] - odx * (e1 - e2) - odx * (e3 - e4)) / 2. +
+                       sv[r][i][j] / 4.;
+            if (g.ninit == 12) {                   // special_boundary_conditions (:1481-1514)
+              const double xd = x[c + nc * (i + O * j)] - g.boxlen_x / 2.;
+              const double yd = y[c + nc * (i + O * j)] - g.boxlen_y / 2.;
+              if (sqrt(xd * xd + yd * yd) > 2.0) d = 0.0;
+            }
+            sres[((i + O * j) * CELL_THREADS + tid) * 4 + v] = d;
+          }
       }
-#pragma unroll
-      for (int i = 0; i < O; ++i)
-#pragma unroll
-        for (int j = 0; j < O; ++j) {
-          double e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0;
-#pragma unroll
-          for (int q = 0; q < O; ++q) {            // :1377-1386
-            e1 = e1 + fr[q] * T.lp[i] * T.L[q][j] * T.wq[q];
-            e2 = e2 + fle[q] * T.lm[i] * T.L[q][j] * T.wq[q];
-          }
-#pragma unroll
-          for (int q = 0; q < O; ++q) {            // :1391-1408
-            e3 = e3 + gt[q] * T.lp[j] * T.L[q][i] * T.wq[q];
-            e4 = e4 + gb[q] * T.lm[j] * T.L[q][i] * T.wq[q];
-          }
-          // :1455-1462 (oneoverdx for the y terms too)
-          double d = (odx * fv1[i][j] + odx * fv2[i][j] - odx * (e1 - e2) - odx * (e3 - e4)) / 2. + sv[i][j] / 4.;
-          if (g.ninit == 12) {                     // special_boundary_conditions (:1480-1510)
-            const double xd = x[c + nc * (i + O * j)] - g.boxlen_x / 2.;
-            const double yd = y[c + nc * (i + O * j)] - g.boxlen_y / 2.;
-            if (sqrt(xd * xd + yd * yd) > 2.0) d = 0.0;
-          }
-          sres[((i + O * j) * CELL_THREADS + tid) * 4 + v] = d;
-        }
     }
   }
   __syncthreads();
