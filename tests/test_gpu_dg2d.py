@@ -156,6 +156,32 @@ def test_run_matches_oracle(limiter, integrator):
     assert err <= 1e-5 * scale and l1 <= 1e-6 and l1g <= 1e-6 and linfg <= 1e-5 * scale
 
 
+@pytest.mark.parametrize("shape", [(70, 45), (300, 130)])
+def test_tvb_positivity_equals_tvb_away_from_vacuum(shape):
+    """The TVB-plus-positivity limiter pass is a separate compiled variant of
+    the TVB pass.  On a smooth state far from vacuum its positivity part changes
+    nothing, so whole runs with either limiter must agree bit for bit.  M = 0 and
+    noise make the TVB part act on many cells, across strip and segment edges."""
+    nx, ny = shape
+    say(f"case: DG TVB vs TVB+positivity {nx}x{ny}")
+    op = po.make_dg_params(nx, ny, tvb_m=0.0, g=1.0)
+    d = smooth_noisy(op, amp=1e-3, noise=1e-4, seed=nx)
+    lim = po.dg2d_limit(op, d)
+    assert np.count_nonzero(np.any(lim != d, axis=(0, 3))) > nx * ny // 10
+    out = {}
+    for limiter in (po.DG_LIMITER_TVB, po.DG_LIMITER_TVB_POSITIVITY):
+        op.limiter = limiter
+        with fvwb.SolverDG(gpu_p(op)) as s:
+            s.upload(lim)
+            s.advance(12)
+            out[limiter] = (s.download(), s.time())
+    a, ta = out[po.DG_LIMITER_TVB]
+    b, tb = out[po.DG_LIMITER_TVB_POSITIVITY]
+    say(f"   max |a - b| {np.max(np.abs(a - b)):.3e}, t {ta[0]:.6e} / {tb[0]:.6e}")
+    assert ta == tb
+    np.testing.assert_array_equal(a, b)
+
+
 def test_fourier_initial_state_and_t_end():
     say("case: DG init_fourier, t_end clip")
     op = po.make_dg_params(32, 32)
