@@ -42,7 +42,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 
 // Row access through a buffer descriptor built on the scalar unit: base = the
-// row's first element (uniform), extent = the allocated row (pitch), per-lane
+// row's first element (uniform), extent = one variable's row, per-lane
 // 32-bit byte offset.  No vector address arithmetic, and the hardware range
 // check bounds every access to the row.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* row, int row_bytes) {
@@ -135,8 +135,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
   const bool fast = xw >= 0 && xw + 64 * CPL <= nx;
   const long long plane = a.plane;
   const int pitch = a.pitch;
-  const int rowb = pitch * (int)sizeof(T);
-  const unsigned vboff = (unsigned)clampi(x0, 0, pitch - CPL) * (unsigned)sizeof(T);
+  // one variable's row holds min(pitch, plane) elements: pitch when the
+  // variables are separate planes (plane = pitch * ny), plane when their rows
+  // lie side by side (pitch = 4 * plane); every row access is bounded to it
+  const int rowl = (long long)pitch < plane ? pitch : (int)plane;
+  const int rowb = rowl * (int)sizeof(T);
+  const unsigned vboff = (unsigned)clampi(x0, 0, rowl - CPL) * (unsigned)sizeof(T);
 
   auto load_row = [&](const T* base, int j) -> Blk<T, CPL> {
     int jc;

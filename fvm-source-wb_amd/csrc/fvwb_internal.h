@@ -33,8 +33,8 @@ struct StageArgs {
   const double* eyc;  // exp(-k y) at centres        [ny]
   const double* eyf;  // exp(-k y) at y faces        [ny+1]
   Ctrl* ctrl;
-  long long plane;    // elements per variable plane
-  int pitch, nx, ny;
+  long long plane;    // element stride between the variables (pitch * ny for separate planes)
+  int pitch, nx, ny;  // pitch: element stride between grid rows
   int rows;           // rows per wave strip
   int ntx, ntiles;    // strips along x, total strips
   int parity, first, last;
@@ -46,8 +46,8 @@ struct StageArgs {
 // Per-instance description of a batch member (device array).
 struct InstDesc {
   long long off;      // element offset of the instance's 4-plane block in a pool buffer
-  long long plane;    // elements per plane (pitch * ny)
-  int pitch, nx, ny;
+  long long plane;    // element stride between the variables (P: rows side by side)
+  int pitch, nx, ny;  // pitch: element stride between grid rows (4 P)
   int rows, ntx, ntiles;
   int tx_off, ty_off; // offsets of the x tables (rxf[nx+1], rxc[nx]) and y tables (eyc[ny], eyf[ny+1])
   double h, cfl, gamma, g, p_over_rho, inv_gm1, odx, ody;
