@@ -2,7 +2,7 @@
 //
 // Replaces the external procedures compute_update_exact / compute_max_speed
 // and the stepping loop of evolve (benchmark_2d.f90:221-279, 465-618).  The
-// handle keeps the deviation dU = U - U_eq in SoA planes (FP64 or FP32), the
+// handle keeps the deviation dU = U - U_eq per variable (FP64 or FP32), the
 // caller's primitive equilibrium (for download), separable FP64 equilibrium
 // tables, a device step-control block and one HIP stream.
 #include <cmath>
@@ -42,8 +42,13 @@ using namespace fvwb;
 struct fvwb_ctx {
   fvwb_params p{};
   hipStream_t stream = nullptr;
-  int pitch = 0;
-  long long plane = 0;
+  // The four variables' rows of one grid row lie side by side: element
+  // (v, i, j) of a state copy is at v plane + j pitch + i, with plane = nx
+  // rounded up to 64 and pitch = 4 plane, so a strip's four row loads fall in
+  // one span instead of a quarter of the copy apart.
+  int pitch = 0;            // element stride between grid rows
+  long long plane = 0;      // element stride between the variables
+  size_t elems = 0;         // elements per state copy (pitch * ny)
   size_t esize = 8;
   void* buf[3] = {nullptr, nullptr, nullptr};  // A = dU^n, B, C
   double* w_eq = nullptr;                      // AoS(4,nx,ny) primitive equilibrium (device)
@@ -126,15 +131,16 @@ extern "C" int fvwb_create(fvwb_handle* out, const fvwb_params* p) {
   if (!h) return fail(nullptr, FVWB_ERR_OOM, "host allocation failed");
   h->p = *p;
   const int nx = p->nx, ny = p->ny;
-  h->pitch = (nx + 63) / 64 * 64;
-  h->plane = (long long)h->pitch * ny;
+  h->plane = (nx + 63) / 64 * 64;
+  h->pitch = 4 * (int)h->plane;
+  h->elems = (size_t)h->pitch * ny;
   h->esize = p->precision == FVWB_FP64 ? 8 : 4;
   int rc = FVWB_OK;
   auto check = [&](int r) { if (r != FVWB_OK && rc == FVWB_OK) rc = r; };
   hipError_t e = hipStreamCreate(&h->stream);
   if (e != hipSuccess) { g_create_err = hipGetErrorString(e); delete h; return FVWB_ERR_HIP; }
   const size_t nc = (size_t)nx * ny;
-  for (int b = 0; b < 3; ++b) check(alloc_dev(h, &h->buf[b], 4 * (size_t)h->plane * h->esize));
+  for (int b = 0; b < 3; ++b) check(alloc_dev(h, &h->buf[b], h->elems * h->esize));
   check(alloc_dev(h, (void**)&h->w_eq, 4 * nc * sizeof(double)));
   check(alloc_dev(h, (void**)&h->stage_u, 4 * nc * sizeof(double)));
   check(alloc_dev(h, (void**)&h->rxf, (nx + 1) * sizeof(double)));
@@ -226,8 +232,8 @@ static StageArgs<T> base_args(fvwb_ctx* h) {
 static int ensure_residual_buffers(fvwb_ctx* h) {
   const size_t nc = (size_t)h->p.nx * h->p.ny;
   if (!h->stage_w) FVWB_HIP(h, hipMalloc((void**)&h->stage_w, 4 * nc * sizeof(double)));
-  if (!h->res_in) FVWB_HIP(h, hipMalloc((void**)&h->res_in, 4 * (size_t)h->plane * sizeof(double)));
-  if (!h->res_out) FVWB_HIP(h, hipMalloc((void**)&h->res_out, 4 * (size_t)h->plane * sizeof(double)));
+  if (!h->res_in) FVWB_HIP(h, hipMalloc((void**)&h->res_in, h->elems * sizeof(double)));
+  if (!h->res_out) FVWB_HIP(h, hipMalloc((void**)&h->res_out, h->elems * sizeof(double)));
   return FVWB_OK;
 }
 
@@ -460,7 +466,7 @@ extern "C" int fvwb_checkpoint_save(fvwb_handle h, const char* path) {
   if (!h || !path) return fail(h, FVWB_ERR_ARG, "NULL argument");
   if (!h->uploaded) return fail(h, FVWB_ERR_STATE, "checkpoint before upload/init");
   const fvwb_params& p = h->p;
-  const fvwb_ckpt::Segment seg[3] = {{h->buf[0], 4 * (size_t)h->plane * h->esize},
+  const fvwb_ckpt::Segment seg[3] = {{h->buf[0], h->elems * h->esize},
                                      {h->w_eq, 4 * (size_t)p.nx * p.ny * sizeof(double)},
                                      {h->ctrl, sizeof(Ctrl)}};
   std::string err;
@@ -472,7 +478,7 @@ extern "C" int fvwb_checkpoint_save(fvwb_handle h, const char* path) {
 extern "C" int fvwb_checkpoint_load(fvwb_handle h, const char* path) {
   if (!h || !path) return fail(h, FVWB_ERR_ARG, "NULL argument");
   const fvwb_params& p = h->p;
-  const fvwb_ckpt::Segment seg[3] = {{h->buf[0], 4 * (size_t)h->plane * h->esize},
+  const fvwb_ckpt::Segment seg[3] = {{h->buf[0], h->elems * h->esize},
                                      {h->w_eq, 4 * (size_t)p.nx * p.ny * sizeof(double)},
                                      {h->ctrl, sizeof(Ctrl)}};
   std::string err;

@@ -4,7 +4,9 @@
 // the next, written byte for byte, so a handle that loads it continues exactly
 // as the one that saved it would have.  Layout (little endian, as the host):
 //   char     magic[8]    "FVWBCKPT"
-//   uint32   version     1
+//   uint32   version     2 (2: the 2D FV state holds the four variables' rows
+//                        of a grid row side by side; version 1 files, with
+//                        separate planes, are rejected)
 //   uint32   kind        which handle type wrote it (Kind below)
 //   uint64   pbytes      size of the parameter block that follows
 //   uint32   nseg        number of state segments
@@ -37,7 +39,7 @@ struct Segment {
 };
 
 constexpr char kMagic[8] = {'F', 'V', 'W', 'B', 'C', 'K', 'P', 'T'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;
 constexpr size_t kChunk = (size_t)64 << 20;   // host staging per copy
 
 inline int save(const char* path, Kind kind, const void* params, uint64_t pbytes, int32_t parity,

@@ -11,8 +11,9 @@
  * [0,boxlen_x]x[0,boxlen_y].  The stored hydrostatic equilibrium is the
  * isothermal atmosphere rho = rho0*exp(-k*(x+y)), p = p0*exp(-k*(x+y)),
  * k = rho0*g/p0, at rest (benchmark_2d.f90:174-218, get_equilibrium_solution).
- * The device state is the deviation dU = U - U_eq in SoA planes; the residual is
- * the exact deviation form of benchmark_2d.f90:465-618 (compute_update_exact).
+ * The device state is the deviation dU = U - U_eq, one row per variable and grid
+ * row (the four rows of a grid row side by side); the residual is the exact
+ * deviation form of benchmark_2d.f90:465-618 (compute_update_exact).
  *
  * Threading: a handle owns device memory and one HIP stream; calls on one handle
  * must not overlap.  Host-array entry points block until the result is on the host.
