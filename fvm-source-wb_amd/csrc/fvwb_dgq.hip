@@ -69,12 +69,20 @@ struct Clock {
 };
 
 // One Runge-Kutta stage of evolve: out = ((a0 X0 + a1 X1) + ...) + (cdt dt) L,
-// the reference's left-to-right order; nterms = 0 writes L itself.
+// the reference's left-to-right order; nterms = 0 writes L itself.  With part
+// set, the stage also stores part = (b0 Y0 + b1 Y1) + ... (npart terms: the
+// leading terms of a later stage that takes the same L) and lval = L, so that
+// the later stage finishes from them instead of evaluating L again.
 struct Epi {
   const double* X[4];
   double a[4];
   double cdt;
   int nterms;
+  const double* Y[3];
+  double b[3];
+  int npart;
+  double* part;
+  double* lval;
 };
 
 // ---- host: 2d/legendre.f90 as the reference evaluates it ------------------------
@@ -555,8 +563,30 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(2)
         double sacc = e.a[0] * e.X[0][idx];
         for (int k = 1; k < e.nterms; ++k) sacc = sacc + e.a[k] * e.X[k][idx];
         out[idx] = sacc + e.cdt * dt * d;
+        if (e.part) {
+          double pacc = e.b[0] * e.Y[0][idx];
+          for (int k = 1; k < e.npart; ++k) pacc = pacc + e.b[k] * e.Y[k][idx];
+          e.part[idx] = pacc;
+          e.lval[idx] = d;
+        }
       }
     }
+  }
+}
+
+// The rest of a stage whose leading terms and L a previous stage stored (Epi
+// part / lval): out = (part + a X) + (cdt dt) L, the same operations in the
+// same order as the stage evaluated in one piece.  out may alias part.
+__global__ __launch_bounds__(256) void dgq_finish_kernel(long long n, const double* part, double a,
+                                                         const double* __restrict__ X, double cdt,
+                                                         const double* __restrict__ lval, double* out,
+                                                         const Clock* __restrict__ ck) {
+  if (!ck->active) return;
+  const double dt = ck->dt;
+  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+    double sacc = part[idx];
+    sacc = sacc + a * X[idx];
+    out[idx] = sacc + cdt * dt * lval[idx];
   }
 }
 
@@ -943,7 +973,7 @@ struct fvwb_dgq {
   hipStream_t stream = nullptr;
   long long nc = 0, ns = 0;                  // cells, doubles per state
   double* du = nullptr;                      // resident state
-  double* w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* w[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // stage states; w[5] holds L(w3)
   double* s_in = nullptr;                    // drop-in staging
   double* s_out = nullptr;
   double *x = nullptr, *y = nullptr;         // resident coordinates
@@ -1001,7 +1031,7 @@ extern "C" int fvwb_dgq_params_reference(fvwb_dgq_params* p) {
 extern "C" int fvwb_dgq_destroy(fvwb_dgq_handle h) {
   if (!h) return FVWB_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* ps[] = {h->du, h->w[0], h->w[1], h->w[2], h->w[3], h->w[4], h->s_in, h->s_out, h->x, h->y, h->xi,
+  void* ps[] = {h->du, h->w[0], h->w[1], h->w[2], h->w[3], h->w[4], h->w[5], h->s_in, h->s_out, h->x, h->y, h->xi,
                 h->yi, h->tr, h->F, h->G, h->res, h->part, h->clk};
   for (void* q : ps)
     if (q) (void)hipFree(q);
@@ -1069,7 +1099,7 @@ extern "C" int fvwb_dgq_create(fvwb_dgq_handle* out, const fvwb_dgq_params* p) {
             hipMalloc(&h->G, gb) == hipSuccess && hipMalloc(&h->res, 16 * sizeof(double)) == hipSuccess &&
             hipMalloc(&h->part, sizeof(double) * 12 * (size_t)err_blocks(h->nc)) == hipSuccess &&
             hipMalloc(&h->clk, sizeof(Clock)) == hipSuccess;
-  for (int k = 0; ok && k < 5; ++k) ok = hipMalloc(&h->w[k], sb) == hipSuccess;
+  for (int k = 0; ok && k < 6; ++k) ok = hipMalloc(&h->w[k], sb) == hipSuccess;
   if (!ok) {
     fvwb_dgq_destroy(h);
     return failq(nullptr, FVWB_ERR_OOM, "device allocation failed");
@@ -1080,6 +1110,7 @@ extern "C" int fvwb_dgq_create(fvwb_dgq_handle* out, const fvwb_dgq_params* p) {
 
 static dim3 cells_grid(const fvwb_dgq* h) { return dim3((unsigned)((h->nc + 255) / 256)); }
 static dim3 cell_grid(const fvwb_dgq* h) { return dim3((unsigned)((h->nc + CELL_THREADS - 1) / CELL_THREADS)); }
+static dim3 finish_grid(const fvwb_dgq* h) { return dim3((unsigned)std::min<long long>((h->ns + 255) / 256, 8192)); }
 
 // compute_update of `in` into `out` with the stage combination e (nterms = 0:
 // dudt itself); the clock gates and provides dt in the resident loop.
@@ -1241,7 +1272,7 @@ static Epi epi(std::initializer_list<std::pair<double, const double*>> terms, do
 static int step(fvwb_dgq* h, double t_end) {
   Clock* ck = h->clk;
   double* du = h->du;
-  double *w1 = h->w[0], *w2 = h->w[1], *w3 = h->w[2], *w4 = h->w[3], *w5 = h->w[4];
+  double *w1 = h->w[0], *w2 = h->w[1], *w3 = h->w[2], *w4 = h->w[3], *w5 = h->w[4], *l3 = h->w[5];
   launch_speed(h, du, nullptr, ck, t_end);
   HIPQ(h, hipGetLastError());
   int rc = 0;
@@ -1258,10 +1289,19 @@ static int step(fvwb_dgq* h, double t_end) {
     if ((rc = residual(h, du, x, y, epi({{1.0, du}}, c[0]), w1, ck)) || (rc = limit(h, w1, ck))) return rc;
     if ((rc = residual(h, w1, x, y, epi({{c[1], du}, {c[2], w1}}, c[3]), w2, ck)) || (rc = limit(h, w2, ck))) return rc;
     if ((rc = residual(h, w2, x, y, epi({{c[4], du}, {c[5], w2}}, c[6]), w3, ck)) || (rc = limit(h, w3, ck))) return rc;
-    if ((rc = residual(h, w3, x, y, epi({{c[7], du}, {c[8], w3}}, c[9]), w4, ck)) || (rc = limit(h, w4, ck))) return rc;
-    // L(w3) once more (:700, :723) for w5, then L(w4) for the new state
-    if ((rc = residual(h, w3, x, y, epi({{c[10], du}, {c[11], w2}, {c[12], w3}, {c[13], w4}}, c[14]), w5, ck)))
-      return rc;
+    // w4 = c7 u + c8 w3 + c9 dt L(w3).  The reference evaluates L(w3) once more
+    // (:700, :723) for w5 = c10 u + c11 w2 + c12 w3 + c13 w4 + c14 dt L(w3),
+    // with w4 limited; here the first evaluation also keeps L(w3) and the sum
+    // of w5's first three terms, and w5 is finished once w4 is limited.
+    Epi e4 = epi({{c[7], du}, {c[8], w3}}, c[9]);
+    e4.Y[0] = du; e4.Y[1] = w2; e4.Y[2] = w3;
+    e4.b[0] = c[10]; e4.b[1] = c[11]; e4.b[2] = c[12];
+    e4.npart = 3; e4.part = w5; e4.lval = l3;
+    if ((rc = residual(h, w3, x, y, e4, w4, ck)) || (rc = limit(h, w4, ck))) return rc;
+    hipLaunchKernelGGL(dgq_finish_kernel, finish_grid(h), dim3(256), 0, h->stream, h->ns, (const double*)w5, c[13],
+                       (const double*)w4, c[14], (const double*)l3, w5, (const Clock*)ck);
+    HIPQ(h, hipGetLastError());
+    // then L(w4) for the new state
     if ((rc = residual(h, w4, x, y, epi({{1.0, w5}}, c[15]), du, ck)) || (rc = limit(h, du, ck))) return rc;
   }
   hipLaunchKernelGGL(dgq_tick_kernel, dim3(1), dim3(64), 0, h->stream, ck);
