@@ -112,10 +112,16 @@ def test_load_rejects_foreign_and_damaged_files(tmp_path):
         with pytest.raises(fvwb.FvwbError, match="another handle type"):
             s.checkpoint_load(path)
     data = open(path, "rb").read()
-    cut = str(tmp_path / "cut.ckpt")
+    assert data[:8] == b"FVWBCKPT" and int.from_bytes(data[8:12], "little") == 2
+    cut, v1 = str(tmp_path / "cut.ckpt"), str(tmp_path / "v1.ckpt")
     with open(cut, "wb") as f:
         f.write(data[: len(data) // 2])
+    # version 1 held the 2D state as separate planes: same sizes, other layout
+    with open(v1, "wb") as f:
+        f.write(data[:8] + (1).to_bytes(4, "little") + data[12:])
     with fvwb.Solver2D(p) as s:
+        with pytest.raises(fvwb.FvwbError, match="version"):
+            s.checkpoint_load(v1)
         with pytest.raises(fvwb.FvwbError, match="truncated"):
             s.checkpoint_load(cut)
         with pytest.raises(fvwb.FvwbError, match="cannot open"):
