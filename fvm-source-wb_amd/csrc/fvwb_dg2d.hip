@@ -106,6 +106,12 @@ struct Args {
   float ggm1, gfloor;        // gamma (gamma-1) and gamma*1e-10
 };
 
+// A table entry at a wave-uniform index, read through the constant address
+// space so that it becomes a scalar load.
+__device__ __forceinline__ double ctab(const double* t, int k) {
+  return ((const __attribute__((address_space(4))) double*)t)[k];
+}
+
 __device__ inline float ctrl_rate(const Ctrl* c, int par) {
   uint32_t m = 0u;
 #pragma unroll
@@ -236,6 +242,18 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
       B[v][1] = ldm(r, st1);
     }
   };
+  // the same, reading 0 everywhere when ok is false (base must still be a
+  // buffer of the frame: without U^n it is the input)
+  const float* un_base = with_un ? a.un : a.in;
+  auto load_own_if = [&](bool ok, const float* base, int j, float (&B)[4][2]) {
+    const unsigned o0 = ok ? st0 : MASKED, o1 = ok ? st1 : MASKED;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const __amdgpu_buffer_rsrc_t r = rs(base, v, j);
+      B[v][0] = ldm(r, o0);
+      B[v][1] = ldm(r, o1);
+    }
+  };
 
   // MFMA operands
   const Consts& C = *a.C;
@@ -274,22 +292,25 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   };
   // finish row j: add the flux through its top face, combine and store
   float Uo[4][2], Xo[4][2];   // the two-output stage: U and X of the row being finished
-  auto finish = [&](int j, v4f (&acc)[4], const Vec4<float>& Ft, const float (&N)[4][2]) {
+  // (ok false: the row below the segment's first row, whose stores are dropped
+  // through their offsets rather than skipped by a branch)
+  auto finish = [&](int j, bool ok, v4f (&acc)[4], const Vec4<float>& Ft, const float (&N)[4][2]) {
     const float ft[4] = {Ft.r, Ft.mx, Ft.my, Ft.e};
+    const unsigned o0 = ok ? st0 : MASKED, o1 = ok ? st1 : MASKED;
 #pragma unroll
     for (int v = 0; v < 4; ++v) acc[v] = mfma(Abf[3], ft[v], acc[v]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const __amdgpu_buffer_rsrc_t r = rs(a.out, v, j);
       if constexpr (!DUAL) {
-        stm(r, st0, alpha * N[v][0] + beta * acc[v][0]);
-        stm(r, st1, alpha * N[v][1] + beta * acc[v][1]);
+        stm(r, o0, alpha * N[v][0] + beta * acc[v][0]);
+        stm(r, o1, alpha * N[v][1] + beta * acc[v][1]);
       } else {
-        stm(r, st0, alpha * N[v][0] + beta * Uo[v][0] + a.gam * acc[v][0]);
-        stm(r, st1, alpha * N[v][1] + beta * Uo[v][1] + a.gam * acc[v][1]);
+        stm(r, o0, alpha * N[v][0] + beta * Uo[v][0] + a.gam * acc[v][0]);
+        stm(r, o1, alpha * N[v][1] + beta * Uo[v][1] + a.gam * acc[v][1]);
         const __amdgpu_buffer_rsrc_t rx = rs(a.xout, v, j);
-        stm(rx, st0, a.pu * N[v][0] + a.px * Xo[v][0] + a.pin * Uo[v][0] + a.pl * acc[v][0]);
-        stm(rx, st1, a.pu * N[v][1] + a.px * Xo[v][1] + a.pin * Uo[v][1] + a.pl * acc[v][1]);
+        stm(rx, o0, a.pu * N[v][0] + a.px * Xo[v][0] + a.pin * Uo[v][0] + a.pl * acc[v][0]);
+        stm(rx, o1, a.pu * N[v][1] + a.px * Xo[v][1] + a.pin * Uo[v][1] + a.pl * acc[v][1]);
       }
     }
   };
@@ -308,14 +329,20 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) N[v][0] = N[v][1] = 0.f;
   v4f accA[4], accB[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) accA[v] = accB[v] = v4f{0.f, 0.f, 0.f, 0.f};
 
   // one row: Bc holds its modes, Bn receives the next row's, accp holds the
   // accumulator of the row below (finished here), acc receives this row's
   auto row = [&](int j, const float (&Bc)[4][2], float (&Bn)[4][2], v4f (&accp)[4], v4f (&acc)[4]) {
-    // loads in flight during this row: the next row, and U^n of the row finished here
+    // loads in flight during this row: the next row, and U^n of the row finished
+    // here.  They are issued on every row (without U^n, through offsets that
+    // read 0), and the first row's finish only drops its stores, so the body of
+    // a row pair has no branch and the scheduler can overlap one row's products
+    // with the next row's.
     load_row(a.in, j + 1, Bn);
-    if (with_un && j > y0) load_own(a.un, j - 1, N);
-    if (DUAL && j > y0) {
+    load_own_if(with_un, un_base, j - 1, N);
+    if constexpr (DUAL) {
       load_own(a.in, j - 1, Uo);
       load_own(a.xin, j - 1, Xo);
     }
@@ -340,8 +367,10 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
     for (int v = 0; v < 4; ++v) acc[v][2] = acc[v][3] = 0.f;
 
     // volume: fluxes at nodes (x_r, y_gq), fed back as k-slot g
-    const double* eyr = a.eyn + 3 * j;
-    const double eyg = g == 0 ? eyr[0] : (g == 1 ? eyr[1] : eyr[2]);
+    // the three y nodes' factors as scalar loads, then a per-lane choice (a
+    // choice between the loads themselves becomes a branch around them)
+    const double ey0 = ctab(a.eyn, 3 * j), ey1 = ctab(a.eyn, 3 * j + 1), ey2 = ctab(a.eyn, 3 * j + 2);
+    const double eyg = g == 0 ? ey0 : (g == 1 ? ey1 : ey2);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const FaceEq<float> q = eq_at(a, exr[r] * eyg);
@@ -374,7 +403,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
       acc[v] = mfma(Abf[2], fb[v], acc[v]);
     }
     // the face below row j is the top face of row j-1
-    if (j > y0) finish(j - 1, accp, Fb, N);
+    finish(j - 1, j > y0, accp, Fb, N);
     Tt = {df[0][3], df[1][3], df[2][3], df[3][3]};
   };
 
@@ -393,7 +422,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
     } else {
       yr = mirror_y(Tt);
     }
-    finish(y1 - 1, accp, flux_y(y1, Tt, yr), N);
+    finish(y1 - 1, true, accp, flux_y(y1, Tt, yr), N);
   };
 
   // rows in pairs with the register sets swapped, so nothing is copied between rows
