@@ -13,13 +13,14 @@
 // dimension:
 //   forward:  [16 points x 8 modes] x [8 modes x 16 cells], once for the 9
 //             volume nodes and once for the 12 face points;
-//   backward: [16 outputs x 4 inputs] x [4 inputs x 16 cells] per step, 6 steps
-//             for the flux terms at the nodes and 4 for the face fluxes; the
-//             backward operands carry dt/dx or dt/dy.
-// Lane l = 16 g + c handles cell c; in the forward products it receives node
-// (r, g) (r = 0..2) and the left, right, bottom and top traces at Gauss point g
-// (g = 0..2); in the backward products it feeds exactly those values as k-slot
-// g, so the point-wise flux work stays in the lane.  The output rows of the
+//   backward: [16 outputs x 4 inputs] x [4 inputs x 16 cells] per step, 8
+//             steps per variable for the 18 node flux terms and the 12 face
+//             flux terms; the backward operands carry dt/dx or dt/dy.
+// Lane l = 16 g + c handles cell c; in the forward products it receives two
+// volume nodes (three in group 3) and the left, right, bottom and top traces
+// at Gauss point g (g = 0..2); in the backward products it feeds exactly those
+// values' fluxes as k-slot g, so the point-wise flux work stays in the lane and
+// all four k-slots carry data.  The output rows of the
 // backward products are ordered so that lane (g, c) receives modes g and g+4 of
 // cell c: the layout in which it loaded them, so the stage combination needs no
 // data movement.  x neighbours are one DPP row shift away; the trace of the row
@@ -53,12 +54,23 @@ constexpr unsigned MASKED = 0x80000000u;   // buffer offset past every extent: l
 
 // Per-lane MFMA operand constants, built on the host from the basis.  Output row
 // 4g + s of a backward product is mode g (s = 0) or mode g + 4 (s = 1, g < 2).
+//
+// The nine volume nodes are spread over all four lane groups: group g takes
+// node n = 2g + t in step t = 0, 1 (g < 3) and group 3 takes n = 6 + t in
+// steps t = 0, 1, 2, where node n is (x_{n%3}, y_{n/3}).  Every group then
+// feeds at most seven values per variable into the backward products of its
+// row (groups 0-2: F_x, F_y of two nodes and the left, right and bottom face
+// fluxes; group 3: F_x, F_y of three nodes), so seven products per variable do,
+// plus the top face's with the row's finish.
 struct Consts {
-  float fwd_node[16][8];        // [row 4g + r: node (x_r, y_g)][mode]; rows with g or r = 3 are zero
+  float fwd_node[16][8];        // [row 4g + t: node of group g in step t][mode]; rows without a node are zero
   float fwd_face[16][8];        // [row 4q + s: point q, s = 0 left, 1 right, 2 bottom, 3 top][mode]
-  float back_vol[2][3][16][4];  // [F_x | F_y][step r][output row][k = g: node (x_r, y_g)]
-  float back_face[4][16][4];    // [left | right | bottom | top][output row][k = q]
+  float back[7][16][4];         // [slot][output row][k = lane group of the value]
+  float back_top[16][4];        // top face at point k (k < 3)
 };
+
+// the volume node of lane group g in step t (see Consts)
+__host__ __device__ constexpr int vol_node(int g, int t) { return g < 3 ? 2 * g + (t < 1 ? t : 1) : 6 + t; }
 
 // The CFL rate max((|u|+c)/dx + (|v|+c)/dy) of parity p is the maximum of
 // rate[p][0..RSLOTS-1]: workgroup b reduces into slot b % RSLOTS, each slot on a
@@ -259,20 +271,28 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
   const Consts& C = *a.C;
   const float An0 = C.fwd_node[c][g], An1 = C.fwd_node[c][g + 4];
   const float Af0 = C.fwd_face[c][g], Af1 = C.fwd_face[c][g + 4];
-  float Abx[3], Aby[3], Abf[4];
+  // backward slots: 0-3 take F_x, F_y, F_x, F_y of steps 0 and 1; 4 and 5 the
+  // left and right face fluxes (groups 0-2) or step 2's F_x and F_y (group 3);
+  // 6 the bottom face flux (groups 0-2).  x-direction terms carry dt/dx.
+  float Ab[7];
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    Abx[r] = (sdt * a.odx) * C.back_vol[0][r][c][g];
-    Aby[r] = (sdt * a.ody) * C.back_vol[1][r][c][g];
+  for (int s = 0; s < 7; ++s) {
+    const bool xd = s == 0 || s == 2 || s == 4 || (s == 5 && g < 3);
+    Ab[s] = (sdt * (xd ? a.odx : a.ody)) * C.back[s][c][g];
   }
-#pragma unroll
-  for (int s = 0; s < 4; ++s) Abf[s] = (sdt * (s < 2 ? a.odx : a.ody)) * C.back_face[s][c][g];
+  const float Abt = (sdt * a.ody) * C.back_top[c][g];
 
-  // equilibrium factors of this column: x nodes, the left x face, x node gq
+  // equilibrium factors of this column: the x factor of this group's node in
+  // each step, the left x face, x node gq; the y index of each step's node
   const int ic = min(max(i, 0), nx - 1), ifc = min(max(i, 0), nx);
   double exr[3];
+  int nyb[3];
 #pragma unroll
-  for (int r = 0; r < 3; ++r) exr[r] = a.exn[3 * ic + r];
+  for (int t = 0; t < 3; ++t) {
+    const int n = vol_node(g, t);
+    exr[t] = a.exn[3 * ic + n % 3];
+    nyb[t] = n / 3;
+  }
   const double exf_i = a.exf[ifc];
   const double exg = a.exn[3 * ic + gq];
 
@@ -298,7 +318,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
     const float ft[4] = {Ft.r, Ft.mx, Ft.my, Ft.e};
     const unsigned o0 = ok ? st0 : MASKED, o1 = ok ? st1 : MASKED;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] = mfma(Abf[3], ft[v], acc[v]);
+    for (int v = 0; v < 4; ++v) acc[v] = mfma(Abt, ft[v], acc[v]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const __amdgpu_buffer_rsrc_t r = rs(a.out, v, j);
@@ -366,22 +386,29 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) acc[v][2] = acc[v][3] = 0.f;
 
-    // volume: fluxes at nodes (x_r, y_gq), fed back as k-slot g
-    // the three y nodes' factors as scalar loads, then a per-lane choice (a
-    // choice between the loads themselves becomes a branch around them)
+    // volume: fluxes at this group's node of each step, fed back as k-slot g.
+    // The three y nodes' factors are scalar loads, then a per-lane choice (a
+    // choice between the loads themselves becomes a branch around them).
     const double ey0 = ctab(a.eyn, 3 * j), ey1 = ctab(a.eyn, 3 * j + 1), ey2 = ctab(a.eyn, 3 * j + 2);
     const double eyg = g == 0 ? ey0 : (g == 1 ? ey1 : ey2);
+    float X2[4], Y2[4];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const FaceEq<float> q = eq_at(a, exr[r] * eyg);
+    for (int t = 0; t < 3; ++t) {
+      const double eyt = nyb[t] == 0 ? ey0 : (nyb[t] == 1 ? ey1 : ey2);
+      const FaceEq<float> q = eq_at(a, exr[t] * eyt);
       Vec4<float> fx, fy;
-      point_terms(P, q, Vec4<float>{dn[0][r], dn[1][r], dn[2][r], dn[3][r]}, fx, fy);
+      point_terms(P, q, Vec4<float>{dn[0][t], dn[1][t], dn[2][t], dn[3][t]}, fx, fy);
       const float X[4] = {fx.r, fx.mx, fx.my, fx.e};
       const float Y[4] = {fy.r, fy.mx, fy.my, fy.e};
+      if (t < 2) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] = mfma(Abx[r], X[v], acc[v]);
+        for (int v = 0; v < 4; ++v) acc[v] = mfma(Ab[2 * t], X[v], acc[v]);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] = mfma(Aby[r], Y[v], acc[v]);
+        for (int v = 0; v < 4; ++v) acc[v] = mfma(Ab[2 * t + 1], Y[v], acc[v]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { X2[v] = X[v]; Y2[v] = Y[v]; }
+      }
     }
 
     // x face on the left of column i at point gq: right trace of column i-1 | left trace of i
@@ -396,11 +423,13 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(Args a) {
     const Vec4<float> yr = {df[0][2], df[1][2], df[2][2], df[3][2]};
     const Vec4<float> Fb = flux_y(j, j == 0 ? mirror_y(yr) : Tt, yr);
     const float fb[4] = {Fb.r, Fb.mx, Fb.my, Fb.e};
+    const bool g3 = g == 3;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      acc[v] = mfma(Abf[0], fl[v], acc[v]);
-      acc[v] = mfma(Abf[1], from_col_right(fl[v]), acc[v]);   // right face = left face of column i+1
-      acc[v] = mfma(Abf[2], fb[v], acc[v]);
+      const float fr = from_col_right(fl[v]);   // right face = left face of column i+1
+      acc[v] = mfma(Ab[4], g3 ? X2[v] : fl[v], acc[v]);
+      acc[v] = mfma(Ab[5], g3 ? Y2[v] : fr, acc[v]);
+      acc[v] = mfma(Ab[6], fb[v], acc[v]);
     }
     // the face below row j is the top face of row j-1
     finish(j - 1, j > y0, accp, Fb, N);
@@ -946,10 +975,19 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
   std::memset(&C, 0, sizeof C);
   const double* xq = h->xq;
   const double* wq = h->wq;
+  // volume weights of node n: 0.5 w_a w_b dphi/dx phi and phi dphi/dy there
+  auto wvol = [&](int n, int m, bool xdir) {
+    const int na = n % 3, nb = n / 3;
+    const double w = wq[na] * wq[nb];
+    return xdir ? 0.5 * w * legp(xq[na], MI[m]) * leg(xq[nb], MJ[m]) : 0.5 * w * leg(xq[na], MI[m]) * legp(xq[nb], MJ[m]);
+  };
   for (int row = 0; row < 16; ++row) {
     const int g = row >> 2, r = row & 3;
     for (int m = 0; m < NM; ++m) {
-      if (g < 3 && r < 3) C.fwd_node[row][m] = (float)(leg(xq[r], MI[m]) * leg(xq[g], MJ[m]));   // node (x_r, y_g)
+      if ((g < 3 && r < 2) || (g == 3 && r < 3)) {
+        const int n = vol_node(g, r);
+        C.fwd_node[row][m] = (float)(leg(xq[n % 3], MI[m]) * leg(xq[n / 3], MJ[m]));
+      }
       if (g < 3) {
         const int q = g;
         double v;
@@ -963,17 +1001,23 @@ extern "C" int fvwb_dg_create(fvwb_dg_handle* out, const fvwb_dg_params* p) {
     // output row -> mode: row 4g is mode g, row 4g+1 mode g+4 (g < 2), others none
     const int m = r == 0 ? g : (r == 1 && g < 2 ? g + 4 : -1);
     if (m < 0) continue;
-    for (int k = 0; k < 3; ++k) {
-      for (int st = 0; st < 3; ++st) {     // backward volume step st feeds node (x_st, y_k) as k-slot k
-        const double w = wq[st] * wq[k];
-        C.back_vol[0][st][row][k] = (float)(0.5 * w * legp(xq[st], MI[m]) * leg(xq[k], MJ[m]));
-        C.back_vol[1][st][row][k] = (float)(0.5 * w * leg(xq[st], MI[m]) * legp(xq[k], MJ[m]));
+    for (int k = 0; k < 4; ++k) {          // k = the lane group whose value takes the slot
+      // steps 0 and 1: F_x and F_y of the group's node
+      for (int t = 0; t < 2; ++t) {
+        C.back[2 * t][row][k] = (float)wvol(vol_node(k, t), m, true);
+        C.back[2 * t + 1][row][k] = (float)wvol(vol_node(k, t), m, false);
       }
-      // face fluxes (pre-scaled by 1/dx or 1/dy) at point k: +-0.5 w phi at the face
-      C.back_face[0][row][k] = (float)(0.5 * wq[k] * leg(-1.0, MI[m]) * leg(xq[k], MJ[m]));
-      C.back_face[1][row][k] = (float)(-0.5 * wq[k] * leg(1.0, MI[m]) * leg(xq[k], MJ[m]));
-      C.back_face[2][row][k] = (float)(0.5 * wq[k] * leg(xq[k], MI[m]) * leg(-1.0, MJ[m]));
-      C.back_face[3][row][k] = (float)(-0.5 * wq[k] * leg(xq[k], MI[m]) * leg(1.0, MJ[m]));
+      if (k < 3) {
+        // face fluxes (pre-scaled by 1/dx or 1/dy) at point k: +-0.5 w phi at the face
+        C.back[4][row][k] = (float)(0.5 * wq[k] * leg(-1.0, MI[m]) * leg(xq[k], MJ[m]));
+        C.back[5][row][k] = (float)(-0.5 * wq[k] * leg(1.0, MI[m]) * leg(xq[k], MJ[m]));
+        C.back[6][row][k] = (float)(0.5 * wq[k] * leg(xq[k], MI[m]) * leg(-1.0, MJ[m]));
+        C.back_top[row][k] = (float)(-0.5 * wq[k] * leg(xq[k], MI[m]) * leg(1.0, MJ[m]));
+      } else {
+        // group 3's third node
+        C.back[4][row][k] = (float)wvol(vol_node(3, 2), m, true);
+        C.back[5][row][k] = (float)wvol(vol_node(3, 2), m, false);
+      }
     }
   }
   // segments along y: as many waves as the device holds at once, at least 4 rows each
