@@ -195,11 +195,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T>& a, const int tile
     for (int k = 0; k < CPL; ++k) b.c[k] = Vec4<T>{v0.v[k], v1.v[k], v2.v[k], v3.v[k]};
     return b;
   };
-  // cell-centred rows (clamped into the grid; rows past the strip are masked)
+  // U^n rows: only owned cells of the tile's own rows use them, so halo lanes,
+  // lanes past nx and the ring's row beyond the tile read 0 through an offset
+  // past the extent instead of memory (0.7 % faster in FP32, no result changes)
   auto load_cell = [&](const T* base, int j) -> Blk<T, CPL> {
     const T* r = base + (long long)clampi(j, 0, ny - 1) * pitch;
-    Lanes<T, CPL> v0 = ldv<T, CPL>(r, vboff, rowb), v1 = ldv<T, CPL>(r + plane, vboff, rowb),
-                  v2 = ldv<T, CPL>(r + 2 * plane, vboff, rowb), v3 = ldv<T, CPL>(r + 3 * plane, vboff, rowb);
+    const unsigned uo = (own && j >= y0 && j < y1) ? vboff : 0x80000000u;
+    Lanes<T, CPL> v0 = ldv<T, CPL>(r, uo, rowb), v1 = ldv<T, CPL>(r + plane, uo, rowb),
+                  v2 = ldv<T, CPL>(r + 2 * plane, uo, rowb), v3 = ldv<T, CPL>(r + 3 * plane, uo, rowb);
     Blk<T, CPL> b;
 #pragma unroll
     for (int k = 0; k < CPL; ++k) b.c[k] = Vec4<T>{v0.v[k], v1.v[k], v2.v[k], v3.v[k]};
